@@ -1,0 +1,216 @@
+"""Benchmark: node-substeps/s of the visionary-sim substep on one MI355X.
+
+Default workload: 256 independent scenes (jelly blocks, lattice side ~ U{8..48},
+10k-500k Gaussians each, 10 substeps per 1/60 s frame), stepped batched -- one
+launch per pass across every scene, the frame replayed from a hipGraph -- and,
+beside it, the plain per-scene launch path on the same scenes. A step is one
+frame of every scene: 10 substeps (predict, 8 colour passes, finalize) and the
+splat deformation. Inputs are synthetic and resident on the device before the
+timed region.
+
+Prints ONE JSON line. `value` is the batched whole-job node-substeps/s;
+`plain`, `speedup`, frame latency p50/p99, the per-pass profile, the roofline
+of the dominant kernel (the elastic colour pass) and the CPU baseline (the f32
+oracle on a bounded sample, one core) ride along.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "visionary-sim_amd", "py"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (SoA f32) per item of each pass, per launch:
+ELASTIC_BYTES_PER_CELL = 8 * (12 + 12 + 4) + 8 * 12 + 4   # gather p, prev, w; scatter p; cell id
+PREDICT_BYTES_PER_NODE = 64
+FINALIZE_BYTES_PER_NODE = 52
+SPLAT_BYTES = 76
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def sync(world):
+    world.sync()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def make_world(scs, **kw):
+    from vsim import native
+    nodes = sum(s.n_nodes for s in scs)
+    cells = sum(s.n_cells for s in scs)
+    splats = sum(s.n_splats for s in scs)
+    driven = sum(len(s.drive.nodes) for s in scs if s.drive is not None)
+    return native.World(max_scenes=len(scs) + 1, max_bodies=sum(len(s.bodies) for s in scs) + 1,
+                        max_nodes=nodes + 1024, max_cells=cells + 1024, max_surface=1024, max_buckets=1024,
+                        max_driven=driven + 64, max_splats=splats + 1024, **kw)
+
+
+def set_targets(world, ids, scs, frame):
+    for sid, sc in zip(ids, scs):
+        if sc.drive is not None:
+            t, v = sc.drive.frame_target(frame, sc.substeps * sc.dt)
+            world.set_target(sid, t, v)
+
+
+def cpu_baseline(scs, budget_s):
+    """The f32 oracle, one core, on whole frames of the first scenes until
+    the time budget is spent."""
+    from oracle.oracle import OracleScene
+    done_ns, t0, frames, scenes_used = 0, time.perf_counter(), 0, 0
+    elapsed = 0.0
+    for sc in scs:
+        o = OracleScene(sc, "f32")
+        scenes_used += 1
+        for f in range(2):
+            a = time.perf_counter()
+            o.frame(sc.substeps, splats=True)
+            elapsed += time.perf_counter() - a
+            done_ns += sc.n_nodes * sc.substeps
+            frames += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    return dict(value=done_ns / elapsed, unit="node-substeps/s", cores=1, kind="port",
+                sample=f"{scenes_used} scenes x 2 frames ({frames} scene-frames, {done_ns} node-substeps, "
+                       f"{elapsed:.1f} s) of the same workload, splats included")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="batched256", choices=["batched256", "c1", "c2"])
+    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); scenes shard across GPUs with no "
+              "communication, so the whole-job figure is N x the one-GPU figure by construction.", file=sys.stderr)
+        sys.exit(2)
+
+    from vsim import scenes
+
+    t_gen = time.perf_counter()
+    if args.workload == "batched256":
+        scs = scenes.c4_scenes(args.scenes)
+        config = {"workload": f"{args.scenes} scenes, lattice side U{{8..48}}, 10k-500k Gaussians each, "
+                              "E 1e5 nu 0.3 rho 1000, h 0.016 m, bottom layer pinned, 10 substeps per 1/60 s frame",
+                  "scenes": args.scenes}
+    elif args.workload == "c1":
+        scs = [scenes.c1_scene()]
+        config = {"workload": "16^3 block, 50k Gaussians, 10 substeps per frame", "scenes": 1}
+    else:
+        scs = [scenes.c2_scene()]
+        config = {"workload": "64^3 block, 2M Gaussians, 27-node sinusoid handle, 10 substeps per frame", "scenes": 1}
+    nodes = sum(s.n_nodes for s in scs)
+    cells = sum(s.n_cells for s in scs)
+    splats = sum(s.n_splats for s in scs)
+    substeps = scs[0].substeps
+    log(f"[bench] {len(scs)} scenes: {nodes} nodes, {cells} cells, {splats} splats (generated in "
+        f"{time.perf_counter() - t_gen:.1f} s)")
+
+    world = make_world(scs)
+    ids = []
+    for sc in scs:
+        ids.append(world.add(sc))
+    sync(world)
+    log("[bench] scenes resident")
+    tune = None
+    if args.autotune:
+        tune = json.loads(world.autotune(substeps))
+        log(f"[bench] autotune best {tune['best']}")
+
+    # ---- batched, graph-replayed frames
+    for f in range(args.warmup):
+        set_targets(world, ids, scs, f)
+        world.frame(substeps, splats=True, graph=True)
+    sync(world)
+    marks = [world.record()]
+    t0 = time.perf_counter()
+    for f in range(args.steps):
+        set_targets(world, ids, scs, args.warmup + f)
+        world.frame(substeps, splats=True, graph=True)
+        marks.append(world.record())
+    sync(world)
+    wall = time.perf_counter() - t0
+    frame_ms = np.array([world.elapsed_ms(marks[i], marks[i + 1]) for i in range(args.steps)])
+    value = nodes * substeps * args.steps / wall
+    log(f"[bench] batched: {wall * 1e3 / args.steps:.3f} ms/frame, {value:.4e} node-substeps/s")
+
+    # ---- per-pass profile of one batched frame (HIP events around each launch)
+    prof = world.profile(substeps, splats=True)
+    el_ms, el_n = prof["elastic"]
+    el_avg_s = el_ms / el_n / 1e3
+    el_bytes = cells * substeps * ELASTIC_BYTES_PER_CELL / el_n
+    achieved = el_bytes / el_avg_s / 1e9
+    log(f"[bench] profile: " + ", ".join(f"{k} {v[0]:.3f} ms/{v[1]}" for k, v in prof.items() if v[1]))
+
+    # ---- plain per-scene launches (the reference's dispatch shape)
+    for sid in ids:
+        world.frame_scene(sid, substeps, splats=True)
+    sync(world)
+    t1 = time.perf_counter()
+    for f in range(args.plain_steps):
+        set_targets(world, ids, scs, f)
+        for sid in ids:
+            world.frame_scene(sid, substeps, splats=True)
+    sync(world)
+    plain_wall = time.perf_counter() - t1
+    plain = nodes * substeps * args.plain_steps / plain_wall
+    log(f"[bench] plain: {plain_wall * 1e3 / args.plain_steps:.3f} ms/frame, {plain:.4e} node-substeps/s")
+    world.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scs, args.cpu_budget)
+        log(f"[bench] cpu baseline: {cpu['value']:.4e} node-substeps/s ({cpu['sample']})")
+
+    out = {
+        "metric": "node-substeps/s (batched; plain per-scene beside it), f32 at <=1e-5 rel vs the f32 oracle",
+        "value": value,
+        "unit": "node-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded lattices and Gaussians, resident in HBM)",
+        "config": config,
+        "plain": plain,
+        "speedup_vs_plain": value / plain,
+        "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
+        "nodes": nodes, "cells": cells, "splats": splats, "substeps_per_frame": substeps,
+        "profile_ms": {k: {"total": v[0], "launches": v[1]} for k, v in prof.items() if v[1]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_elastic (colour pass)", "bytes_per_launch": el_bytes,
+                     "avg_launch_us": el_avg_s * 1e6},
+        "cpu_baseline": cpu,
+    }
+    if tune:
+        out["autotune"] = tune["best"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,300 @@
+/*
+ * vsim.h -- C ABI of the MI355X solver for the visionary-sim substep.
+ *
+ * Two layers, both plain C (pointers, sizes, integer handles):
+ *
+ *  1. The compute backend: device buffers and named kernel launches. It is the
+ *     drop-in for the kernel package's `ComputeBackend`
+ *     (packages/visionary-sim/src/sim/compute/backend.js:114-163), which is the
+ *     only place the reference touches the GPU. The JS host binds it through
+ *     the Node-API addon in visionary-sim_amd/node/.
+ *
+ *  2. The world: many scenes resident on the device at once, stepped as one
+ *     batch per pass (one persistent launch per colour across every scene,
+ *     a hipGraph per frame), plus the plain per-scene launch path and a
+ *     continuous-batching server. The reference has no counterpart beyond its
+ *     co-location of bodies in one dispatch (sim/state/particles.js:18-21,
+ *     sim/discretize/lattice/elasticCells.js:143-163).
+ *
+ * Every entry point returns 0 on success and a negative VS_E_* code on
+ * failure; vs_last_error() names the failure in the same words the reference's
+ * backend uses for its Error messages where one exists.
+ *
+ * Node state is structure-of-arrays: a "node buffer" holds planes of `plane`
+ * floats each (x plane, then y, then z), not the reference's vec4f records
+ * (sim/state/state.wgsl:16-19). Buffer layout is private to the solver in the
+ * reference too (deformation/bind.js:38-47).
+ */
+#ifndef VSIM_H
+#define VSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+#define VS_OK 0
+#define VS_E_INVALID -1   /* bad argument, unknown kernel or parameter     */
+#define VS_E_DEVICE -2    /* HIP runtime error                              */
+#define VS_E_NOMEM -3     /* allocation beyond the device or budget limit    */
+#define VS_E_DESTROYED -4 /* use after destroy                              */
+
+/* ------------------------------------------------------- shared records */
+
+/* Per-body constants. Replaces CellParams (lattice/wgsl/cells.wgsl:18-31,
+ * packed by elasticCells.js:222-249) and NodeParams (state/state.wgsl:24-33,
+ * packed by particles.js:145-159), plus the lattice origin the splat sampler
+ * reads (lattice/sample.wgsl:17-21). 80 bytes. `damp` is exp(-damping * dt),
+ * the factor finalize.wgsl:72 applies, formed on the host once per dt. */
+typedef struct vs_body {
+    uint32_t cell_dims[3];
+    uint32_t node_offset;
+    uint32_t node_dims[3];
+    uint32_t scene;
+    float h, alpha_dev, alpha_hyd, gamma;
+    float relaxation, max_stretch, friction, damp;
+    float origin[3];
+    uint32_t node_count;
+} vs_body;
+
+/* Per-scene uniforms: the Params blocks of predict.wgsl:1-5,
+ * finalize.wgsl:1-6, contacts.wgsl:1-8 and drive.wgsl:1-7 in one record.
+ * 112 bytes. */
+typedef struct vs_scene_params {
+    float dt;
+    float gravity[3];
+    float tank_min[3];
+    float gain;
+    float tank_max[3];
+    float bucket_size;
+    float grid_origin[3];
+    uint32_t bucket_count;
+    uint32_t grid_dims[3];
+    uint32_t grid_base;    /* first bucket of this scene in the grid buffers */
+    float target[3];       /* grab target (drive.wgsl:2)                     */
+    float response_time;
+    float target_vel[3];
+    uint32_t drive_law;    /* 0 hard, 1 compliant                            */
+    uint32_t surf_base;    /* first surface record of this scene             */
+    uint32_t surf_count;
+    uint32_t bucket_slots; /* 32, BUCKET_SLOTS of contacts.wgsl:75           */
+    uint32_t flags;
+} vs_scene_params;
+
+/* One unit of work for a launch: `count` consecutive items from `begin` in the
+ * pass's item array (nodes, cells, surface records, driven records, splats),
+ * all of one body (or, for contact and drive, one scene). A tile buffer is a
+ * 16-byte header {count, 0, 0, 0} followed by the tiles. */
+typedef struct vs_tile {
+    uint32_t begin, count, body, scene;
+} vs_tile;
+
+/* ------------------------------------------------- layer 1: the backend */
+
+typedef struct vs_backend vs_backend;
+
+/* Kernel ids, keyed by the reference's pass labels. */
+#define VS_K_PREDICT 0          /* sim-predict          predict.wgsl:19-36        */
+#define VS_K_FINALIZE 1         /* sim-finalize         finalize.wgsl:44-74       */
+#define VS_K_ELASTIC 2          /* sim-elastic-cells    project.wgsl:35-184       */
+#define VS_K_CONTACT_BIN 3      /* sim-contact-bin      bin.wgsl:9-23             */
+#define VS_K_CONTACT_SORT 4     /* sim-contact-sort     sort.wgsl:14-53           */
+#define VS_K_CONTACT_COLLIDE 5  /* sim-contact-collide  collide.wgsl:16-125       */
+#define VS_K_CONTACT_SETTLE 6   /* sim-contact-settle   settle.wgsl:20-36         */
+#define VS_K_DRIVE_HARD 7       /* sim-drive-hard       driveHard.wgsl:15-35      */
+#define VS_K_DRIVE_COMPLIANT 8  /* sim-drive-compliant  driveCompliant.wgsl:24-41 */
+#define VS_K_SPLAT 9            /* splat deformation    sample.wgsl:40-80 + covariance.wgsl:30-42 */
+#define VS_K_COUNT 10
+
+/* Buffer slots of a launch. Node buffers hold 3 planes of `plane` floats. */
+#define VS_B_POS 0        /* nodePos  (3 planes)                              */
+#define VS_B_PREV 1       /* nodePrev (3 planes)                              */
+#define VS_B_VEL 2        /* nodeVel  (3 planes)                              */
+#define VS_B_INVMASS 3    /* nodeInvMass                                      */
+#define VS_B_BODIES 4     /* vs_body[]                                        */
+#define VS_B_SCENES 5     /* vs_scene_params[] (the pass's uniforms)          */
+#define VS_B_TILES 6      /* tile buffer                                      */
+#define VS_B_ITEMS 7      /* elastic: u32 linear cell per item;
+                             drive: 4 planes (u32 node, offset x, y, z);
+                             contact: 9 planes (u32 node, u32 body, radius, u32 key,
+                                      u32 rank, delta x, y, z, hits);
+                             splats: float4 quaternion xyzw per splat, then 6 planes
+                                     (mean x, y, z, scale x, y, z)                */
+#define VS_B_GRID 8       /* contact: counters then slots                     */
+#define VS_B_OUT 9        /* splats: 9 planes (mean x, y, z, cov xx xy xz yy yz zz);
+                             contact collide: optional pair log, i32 count per
+                             record then pair_cap u32 mate ids per record     */
+#define VS_B_COUNT 10
+
+typedef struct vs_launch {
+    int32_t kernel;     /* VS_K_*                                             */
+    uint32_t grid;      /* workgroups, at least 1 (one per tile, or fewer: persistent) */
+    uint32_t block;     /* threads per workgroup; 0 = 64                      */
+    uint32_t plane;     /* floats per plane of the node buffers               */
+    uint32_t items;     /* floats per plane of the item buffer (contact, drive, splats) */
+    uint32_t flags;     /* elastic: 1 = limit-only sweep                      */
+    uint32_t buckets;   /* contact: buckets in the grid buffer (counters, then 32 slots each) */
+    uint32_t pair_cap;  /* contact collide: pair log capacity per record (0 = off) */
+    uint64_t buffers[VS_B_COUNT];
+} vs_launch;
+
+/* Creates a backend on HIP device `device` with its own stream.
+ * Mirrors `new ComputeBackend(device)` (backend.js:115). */
+int vs_backend_create(int device, vs_backend **out);
+/* Frees every allocation the backend owns; idempotent (backend.js:159-162). */
+void vs_backend_destroy(vs_backend *b);
+/* Last error message of this backend, or of the last failed create. */
+const char *vs_last_error(const vs_backend *b);
+
+/* `buffer(size)` (backend.js:116-123): rejects sizes beyond the device limit
+ * with "Simulation buffer requires <n> bytes; device storage/buffer limit is <m>".
+ * The handle is an opaque 64-bit id (the device address). */
+int vs_buffer_alloc(vs_backend *b, uint64_t bytes, uint64_t *handle);
+/* `Allocation.destroy()` (backend.js:58-61); freeing twice is a no-op. */
+int vs_buffer_free(vs_backend *b, uint64_t handle);
+/* `Allocation.write(offset, data)` (backend.js:39-41): stream-ordered; the
+ * source may be reused as soon as the call returns. */
+int vs_buffer_write(vs_backend *b, uint64_t handle, uint64_t offset, const void *src, uint64_t bytes);
+/* `Allocation.read(offset, size, target)` (backend.js:42-57): waits for every
+ * queued launch, then copies. A zero-size read is a no-op. */
+int vs_buffer_read(vs_backend *b, uint64_t handle, uint64_t offset, uint64_t bytes, void *dst);
+/* Kernel id for a reference pass label ("sim-predict", "sim-elastic-cells",
+ * ...), or VS_E_INVALID. Stands in for `kernel(name, wgsl)` (backend.js:124-136);
+ * the WGSL text is not needed. */
+int vs_kernel_id(const char *name);
+/* `dispatch(sequence, label)` (backend.js:138-158): validates every launch,
+ * then enqueues them in order on the backend's stream with no host wait. */
+int vs_dispatch(vs_backend *b, const vs_launch *launches, uint32_t count);
+/* Waits until every enqueued launch has finished. */
+int vs_synchronize(vs_backend *b);
+
+/* ------------------------------------------------------ layer 2: world */
+
+typedef struct vs_world vs_world;
+
+/* Capacities of the device pools, i.e. the HBM budget. */
+typedef struct vs_world_config {
+    int32_t device;
+    uint32_t max_scenes;
+    uint32_t max_bodies;
+    uint64_t max_nodes;
+    uint64_t max_cells;
+    uint64_t max_surface;
+    uint64_t max_buckets;
+    uint64_t max_driven;
+    uint64_t max_splats;
+    uint32_t tile_nodes;    /* items per tile for node passes (0 = 256)        */
+    uint32_t tile_cells;    /* items per tile for the colour passes (0 = 256)  */
+    uint32_t block_cells;   /* threads per workgroup of the colour passes      */
+    uint32_t persistent;    /* workgroups of a persistent launch (0 = 1 per tile) */
+    uint32_t xcd_chunked;   /* 1: give each XCD a contiguous run of tiles      */
+    uint32_t pad0;
+} vs_world_config;
+
+/* One body of a scene, in the reference's host layout: the dense node grid
+ * of buildLattice (lattice.js:166-239) and its colour-grouped occupied cells. */
+typedef struct vs_body_desc {
+    uint32_t cell_dims[3];
+    uint32_t n_cells;            /* occupied cells                          */
+    const uint32_t *cells;       /* linear cell ids grouped by colour        */
+    uint32_t color_count[8];     /* cells of each colour, in order           */
+    float origin[3];
+    float h;
+    const float *rest;           /* 3 * node_count, xyz per node            */
+    const float *inv_mass;       /* node_count, prescribed nodes already 0  */
+    float velocity[3];           /* initial velocity of every node (0 = at rest) */
+    float alpha_dev, alpha_hyd, gamma, relaxation, max_stretch, friction, damping;
+    uint32_t n_surface;          /* contact records of this body            */
+    const uint32_t *surface;     /* body-local node ids, ascending          */
+    float contact_radius;
+    uint32_t n_splats;
+    const float *splat_mean;     /* 3 * n_splats                            */
+    const float *splat_quat;     /* 4 * n_splats, xyzw                      */
+    const float *splat_scale;    /* 3 * n_splats                            */
+} vs_body_desc;
+
+typedef struct vs_scene_desc {
+    uint32_t n_bodies;
+    const vs_body_desc *bodies;
+    float dt;
+    float gravity[3];
+    float tank_min[3];
+    float tank_max[3];
+    int32_t contact;             /* 0 off; 1 the four contact passes when the scene
+                                    has two bodies or more; 2 always            */
+    float bucket_size;           /* grid edge, 2 x the widest contact radius (bodyContact.js:87) */
+    float grid_origin[3];        /* tank min - edge (broadphase.js:46-58)   */
+    uint32_t grid_dims[3];       /* 0: derive origin and dims from the tank  */
+    uint32_t n_driven;
+    const uint32_t *driven;      /* scene-global node ids                   */
+    const float *driven_offset;  /* 3 per record                            */
+    uint32_t drive_law;
+    float response_time;
+} vs_scene_desc;
+
+int vs_world_create(const vs_world_config *cfg, vs_world **out);
+void vs_world_destroy(vs_world *w);
+const char *vs_world_error(const vs_world *w);
+/* Admits a scene: allocates its ranges in every pool and uploads its state.
+ * VS_E_NOMEM when a pool (the HBM budget) cannot hold it. */
+int vs_world_add_scene(vs_world *w, const vs_scene_desc *desc, uint32_t *scene_id);
+/* Releases a scene's ranges. */
+int vs_world_remove_scene(vs_world *w, uint32_t scene_id);
+/* Restores a scene to rest (particles.js:245-249). */
+int vs_world_reset_scene(vs_world *w, uint32_t scene_id);
+/* Per-frame grab target of a scene's drive (simulator.js:309-315). */
+int vs_world_set_target(vs_world *w, uint32_t scene_id, const float target[3], const float velocity[3]);
+/* One frame of every admitted scene: `substeps` substeps as one batch per
+ * pass, then splat deformation when `splats` is set. `graph` replays a
+ * captured hipGraph of the frame. Asynchronous. */
+int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph);
+/* The plain path: one scene's frame with per-scene launches, as the
+ * reference dispatches a scene (stepper.js:101-108). Asynchronous. */
+int vs_world_frame_scene(vs_world *w, uint32_t scene_id, uint32_t substeps, int32_t splats);
+int vs_world_synchronize(vs_world *w);
+/* Records an event after the work enqueued so far; returns its slot. */
+int vs_world_record(vs_world *w, uint32_t *slot);
+/* Milliseconds between two recorded events (after they completed). */
+int vs_world_elapsed(vs_world *w, uint32_t from, uint32_t to, float *ms);
+/* Copies a scene's node positions (3 planes of node_count) and velocities. */
+int vs_world_read_nodes(vs_world *w, uint32_t scene_id, float *pos, float *vel);
+/* Copies a scene's deformed splats: means (3 planes) and covariances (6 planes). */
+int vs_world_read_splats(vs_world *w, uint32_t scene_id, float *mean, float *cov);
+/* Contact results of the last substep: bucket of each record and, when the
+ * world was asked to log pairs (vs_world_log_pairs), up to `cap` mates each. */
+int vs_world_log_pairs(vs_world *w, uint32_t cap);
+int vs_world_read_contacts(vs_world *w, uint32_t scene_id, uint32_t *keys, int32_t *counts, uint32_t *pairs);
+/* Counters for the measurement harness. */
+typedef struct vs_world_stats {
+    uint64_t scenes, bodies, nodes, cells, surface, buckets, splats, driven;
+    uint64_t bytes_reserved;
+    uint32_t graph_captures, pad0;
+} vs_world_stats;
+int vs_world_stats_get(const vs_world *w, vs_world_stats *out);
+/* Pass kinds of vs_world_profile. */
+#define VS_P_PREDICT 0
+#define VS_P_DRIVE 1
+#define VS_P_ELASTIC 2
+#define VS_P_BIN 3
+#define VS_P_SORT 4
+#define VS_P_COLLIDE 5
+#define VS_P_SETTLE 6
+#define VS_P_FINALIZE 7
+#define VS_P_SPLAT 8
+#define VS_P_COUNT 9
+/* One frame of every admitted scene, batched but not captured, with HIP
+ * events around every launch: ms[k] is the summed duration of the launches of
+ * kind k, launches[k] their number. Synchronous. */
+int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, uint32_t *launches);
+/* Re-times the colour-pass launch shapes on the admitted scenes and keeps the
+ * fastest (block size x cells per tile x persistent grid x XCD mapping). */
+int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t report_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,657 @@
+/*
+ * oracle.c -- CPU restatement of the visionary-sim per-substep hot path.
+ *
+ * TEST INFRASTRUCTURE (see oracle.h). Every function follows one WGSL kernel
+ * of the reference, operation by operation, in the evaluation order this
+ * project fixes for f32 (left-to-right sums, `dot(a,a) = (ax*ax + ay*ay) +
+ * az*az`, WGSL matrix products column by column, the mat3.js determinant).
+ * The reference's own bit-level order belongs to the browser's WGSL compiler
+ * and is not recoverable, so this file is the definition the HIP kernels are
+ * held to bit-for-bit; against the reference itself the bar is 1e-5 relative.
+ *
+ * Build: REAL=float (liboracle_f32.so) and REAL=double (liboracle_f64.so),
+ * both with -ffp-contract=off and no fast-math (oracle/Makefile).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+typedef REAL real;
+
+/* Correctly rounded square root and exact floor in the working precision. */
+#define SQRT(x) ((real)sqrt((double)(x)))
+#define FLOOR(x) ((real)floor((double)(x)))
+
+typedef struct { real x, y, z; } v3;
+typedef struct { v3 c[3]; } m3; /* column-major, like WGSL mat3x3f */
+
+static inline v3 mk(real x, real y, real z) { v3 r = { x, y, z }; return r; }
+static inline v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 scale(v3 a, real s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline real dot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline v3 cross(v3 a, v3 b)
+{
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline real length3(v3 a) { return SQRT(dot(a, a)); }
+
+/* (m * n)[:, j] = m[:,0] * n[j].x + m[:,1] * n[j].y + m[:,2] * n[j].z */
+static m3 mmul(m3 m, m3 n)
+{
+    m3 r;
+    for (int j = 0; j < 3; j++) {
+        r.c[j] = add(add(scale(m.c[0], n.c[j].x), scale(m.c[1], n.c[j].y)), scale(m.c[2], n.c[j].z));
+    }
+    return r;
+}
+
+static m3 mscale(m3 m, real s)
+{
+    m3 r = { { scale(m.c[0], s), scale(m.c[1], s), scale(m.c[2], s) } };
+    return r;
+}
+
+static m3 transpose(m3 m)
+{
+    m3 r = { { mk(m.c[0].x, m.c[1].x, m.c[2].x),
+               mk(m.c[0].y, m.c[1].y, m.c[2].y),
+               mk(m.c[0].z, m.c[1].z, m.c[2].z) } };
+    return r;
+}
+
+/* mat3Determinant, discretize/mat3.js:30-36, m[col*3+row]. */
+static real det3(m3 m)
+{
+    real m0 = m.c[0].x, m1 = m.c[0].y, m2 = m.c[0].z;
+    real m3_ = m.c[1].x, m4 = m.c[1].y, m5 = m.c[1].z;
+    real m6 = m.c[2].x, m7 = m.c[2].y, m8 = m.c[2].z;
+    return m0 * (m4 * m8 - m7 * m5) - m3_ * (m1 * m8 - m7 * m2) + m6 * (m1 * m5 - m4 * m2);
+}
+
+/* KUHN_TETS, discretize/lattice/kuhn.js:28-35; corner c = (c&1, (c>>1)&1, (c>>2)&1)
+ * per corners.js:10-19. */
+static const int KUHN_TETS[6][4] = {
+    { 0, 1, 3, 7 }, { 0, 5, 1, 7 }, { 0, 3, 2, 7 }, { 0, 2, 6, 7 }, { 0, 4, 5, 7 }, { 0, 6, 4, 7 }
+};
+
+/* Unit-cube rest shape inverses (kuhn.js:78-80 = mat3Inverse of the corner
+ * edge matrices). Every Dm has det 1 and integer entries, so the inverse is
+ * the adjugate and exact in any precision. Built once, lazily. */
+static m3 KUHN_INV[6];
+static real STRETCH_LEN[28];
+static int STRETCH_A[28], STRETCH_B[28];
+static int tables_ready;
+
+static void build_tables(void)
+{
+    if (tables_ready) return;
+    for (int t = 0; t < 6; t++) {
+        double e[3][3]; /* e[k] = corner(t,k+1) - corner(t,0), as columns */
+        int c0 = KUHN_TETS[t][0];
+        for (int k = 0; k < 3; k++) {
+            int c = KUHN_TETS[t][k + 1];
+            e[k][0] = (double)((c & 1) - (c0 & 1));
+            e[k][1] = (double)(((c >> 1) & 1) - ((c0 >> 1) & 1));
+            e[k][2] = (double)(((c >> 2) & 1) - ((c0 >> 2) & 1));
+        }
+        /* column-major m[col*3+row] */
+        double m[9] = { e[0][0], e[0][1], e[0][2], e[1][0], e[1][1], e[1][2], e[2][0], e[2][1], e[2][2] };
+        double det = m[0] * (m[4] * m[8] - m[7] * m[5]) - m[3] * (m[1] * m[8] - m[7] * m[2]) +
+                     m[6] * (m[1] * m[5] - m[4] * m[2]);
+        double s = 1.0 / det;
+        double inv[9] = {
+            (m[4] * m[8] - m[7] * m[5]) * s, (m[7] * m[2] - m[1] * m[8]) * s, (m[1] * m[5] - m[4] * m[2]) * s,
+            (m[6] * m[5] - m[3] * m[8]) * s, (m[0] * m[8] - m[6] * m[2]) * s, (m[3] * m[2] - m[0] * m[5]) * s,
+            (m[3] * m[7] - m[6] * m[4]) * s, (m[6] * m[1] - m[0] * m[7]) * s, (m[0] * m[4] - m[3] * m[1]) * s
+        };
+        for (int col = 0; col < 3; col++) {
+            KUHN_INV[t].c[col] = mk((real)inv[col * 3], (real)inv[col * 3 + 1], (real)inv[col * 3 + 2]);
+        }
+    }
+    /* STRETCH_EDGES, stretch.js:5-7: every corner pair i<j with its unit-cell
+     * length, stored as an f32 literal by the kernel generator. */
+    int k = 0;
+    for (int i = 0; i < 8; i++) {
+        for (int j = i + 1; j < 8; j++) {
+            double dx = (double)((i & 1) - (j & 1));
+            double dy = (double)(((i >> 1) & 1) - ((j >> 1) & 1));
+            double dz = (double)(((i >> 2) & 1) - ((j >> 2) & 1));
+            STRETCH_A[k] = i;
+            STRETCH_B[k] = j;
+            STRETCH_LEN[k] = (real)(float)sqrt(dx * dx + dy * dy + dz * dz);
+            k++;
+        }
+    }
+    tables_ready = 1;
+}
+
+/* xpbdCouple, sim/constraints/wgsl/xpbdCouple.wgsl:58-71. */
+static void xpbd_couple(real cD, real cH, real aD, real aH, real relax, real rD, real rH,
+                        real gx, real gy, real gz, real *lD, real *lH)
+{
+    real stiffened = (real)1 + relax;
+    real a00 = gx * stiffened + aD;
+    real a01 = gy * stiffened;
+    real a11 = gz * stiffened + aH;
+    real det = a00 * a11 - a01 * a01;
+    if (det < (real)(float)1e-20) {
+        *lD = 0;
+        *lH = 0;
+        return;
+    }
+    real bD = -(cD + rD * relax);
+    real bH = -(cH + rH * relax);
+    *lD = (a11 * bD - a01 * bH) / det;
+    *lH = (a00 * bH - a01 * bD) / det;
+}
+
+/* One Kuhn tetrahedron of project.wgsl:75-153 on four gathered corners.
+ * aD/aH are the compliances already divided by dt^2. */
+static void project_tet(v3 *p, const v3 *prev, const real *w, const int *tet, m3 B,
+                        real gamma, real aD, real aH, real relax)
+{
+    int i0 = tet[0], i1 = tet[1], i2 = tet[2], i3 = tet[3];
+    real w0 = w[i0], w1 = w[i1], w2 = w[i2], w3 = w[i3];
+    if (((w0 + w1) + w2) + w3 == (real)0) return;
+
+    m3 Bt = transpose(B);
+    v3 base = p[i0];
+    m3 Ds = { { sub(p[i1], base), sub(p[i2], base), sub(p[i3], base) } };
+    m3 F = mmul(Ds, B);
+
+    real cDev = SQRT((dot(F.c[0], F.c[0]) + dot(F.c[1], F.c[1])) + dot(F.c[2], F.c[2]));
+    real cHyd = det3(F) - gamma;
+
+    real k = cDev > (real)(float)1e-9 ? (real)1 / cDev : (real)0;
+    m3 mDev = mmul(mscale(F, k), Bt);
+    m3 H = { { cross(F.c[1], F.c[2]), cross(F.c[2], F.c[0]), cross(F.c[0], F.c[1]) } };
+    m3 mHyd = mmul(H, Bt);
+
+    v3 d1 = mDev.c[0], d2 = mDev.c[1], d3 = mDev.c[2];
+    v3 d0 = neg(add(add(d1, d2), d3));
+    v3 h1 = mHyd.c[0], h2 = mHyd.c[1], h3 = mHyd.c[2];
+    v3 h0 = neg(add(add(h1, h2), h3));
+
+    real gx = ((w0 * dot(d0, d0) + w1 * dot(d1, d1)) + w2 * dot(d2, d2)) + w3 * dot(d3, d3);
+    real gy = ((w0 * dot(d0, h0) + w1 * dot(d1, h1)) + w2 * dot(d2, h2)) + w3 * dot(d3, h3);
+    real gz = ((w0 * dot(h0, h0) + w1 * dot(h1, h1)) + w2 * dot(h2, h2)) + w3 * dot(h3, h3);
+
+    v3 x0 = sub(p[i0], prev[i0]);
+    v3 x1 = sub(p[i1], prev[i1]);
+    v3 x2 = sub(p[i2], prev[i2]);
+    v3 x3 = sub(p[i3], prev[i3]);
+    real rD = ((dot(d0, x0) + dot(d1, x1)) + dot(d2, x2)) + dot(d3, x3);
+    real rH = ((dot(h0, x0) + dot(h1, x1)) + dot(h2, x2)) + dot(h3, x3);
+
+    real lD, lH;
+    xpbd_couple(cDev, cHyd, aD, aH, relax, rD, rH, gx, gy, gz, &lD, &lH);
+    p[i0] = add(p[i0], scale(add(scale(d0, lD), scale(h0, lH)), w0));
+    p[i1] = add(p[i1], scale(add(scale(d1, lD), scale(h1, lH)), w1));
+    p[i2] = add(p[i2], scale(add(scale(d2, lD), scale(h2, lH)), w2));
+    p[i3] = add(p[i3], scale(add(scale(d3, lD), scale(h3, lH)), w3));
+}
+
+/* Known-answer entry for one tetrahedron, the shape of `projectCoupled` in the
+ * reference's elastic.test.js:111-127 (B column-major; alphas already over dt^2). */
+void o_project_tet(const REAL *p12, const REAL *prev12, const REAL *w4, const REAL *B9,
+                   double alpha_dev, double alpha_hyd, double gamma, double relax, REAL *out12)
+{
+    build_tables();
+    v3 p[4], prev[4];
+    real w[4];
+    for (int i = 0; i < 4; i++) {
+        p[i] = mk(p12[i * 3], p12[i * 3 + 1], p12[i * 3 + 2]);
+        prev[i] = mk(prev12[i * 3], prev12[i * 3 + 1], prev12[i * 3 + 2]);
+        w[i] = w4[i];
+    }
+    m3 B;
+    for (int c = 0; c < 3; c++) B.c[c] = mk(B9[c * 3], B9[c * 3 + 1], B9[c * 3 + 2]);
+    static const int order[4] = { 0, 1, 2, 3 };
+    project_tet(p, prev, w, order, B, (real)gamma, (real)alpha_dev, (real)alpha_hyd, (real)relax);
+    for (int i = 0; i < 4; i++) {
+        out12[i * 3] = p[i].x;
+        out12[i * 3 + 1] = p[i].y;
+        out12[i * 3 + 2] = p[i].z;
+    }
+}
+
+static inline v3 load3(const real *x, const real *y, const real *z, uint32_t n) { return mk(x[n], y[n], z[n]); }
+
+/* project.wgsl:35-184 for one cell; cellCoords/cornerNode from cells.wgsl:39-53. */
+static void project_cell(o_scene *sc, const o_body *cp, uint32_t lin, real dt, int limit_only)
+{
+    uint32_t cx = lin % cp->cell_dims[0];
+    uint32_t cy = (lin / cp->cell_dims[0]) % cp->cell_dims[1];
+    uint32_t cz = lin / (cp->cell_dims[0] * cp->cell_dims[1]);
+    uint32_t ids[8];
+    v3 p[8], prev[8];
+    real w[8];
+    for (int c = 0; c < 8; c++) {
+        uint32_t nx = cx + (uint32_t)(c & 1), ny = cy + (uint32_t)((c >> 1) & 1), nz = cz + (uint32_t)((c >> 2) & 1);
+        uint32_t n = cp->node_offset + nx + cp->node_dims[0] * (ny + cp->node_dims[1] * nz);
+        ids[c] = n;
+        p[c] = load3(sc->px, sc->py, sc->pz, n);
+        prev[c] = load3(sc->qx, sc->qy, sc->qz, n);
+        w[c] = sc->w[n];
+    }
+    real maxStretch = (real)cp->max_stretch;
+    real h = (real)cp->h;
+    if (!limit_only) {
+        real invDt2 = (real)1 / (dt * dt);
+        real aD = (real)cp->alpha_dev * invDt2;
+        real aH = (real)cp->alpha_hyd * invDt2;
+        real relax = (real)cp->relaxation / dt;
+        real restScale = (real)1 / h;
+        real gamma = (real)cp->gamma;
+        for (int t = 0; t < 6; t++) {
+            m3 B = mscale(KUHN_INV[t], restScale);
+            project_tet(p, prev, w, KUHN_TETS[t], B, gamma, aD, aH, relax);
+        }
+    }
+    if (maxStretch > (real)1) {
+        for (int sweep = 0; sweep < 2; sweep++) {
+            for (int e = 0; e < 28; e++) {
+                int a = STRETCH_A[e], b = STRETCH_B[e];
+                v3 delta = sub(p[b], p[a]);
+                real len = length3(delta);
+                real maximum = (STRETCH_LEN[e] * h) * maxStretch;
+                real weight = w[a] + w[b];
+                if (len > maximum && weight > (real)0) {
+                    v3 corr = scale(delta, (len - maximum) / (len * weight));
+                    p[a] = add(p[a], scale(corr, w[a]));
+                    p[b] = sub(p[b], scale(corr, w[b]));
+                }
+            }
+        }
+    }
+    for (int c = 0; c < 8; c++) {
+        sc->px[ids[c]] = p[c].x;
+        sc->py[ids[c]] = p[c].y;
+        sc->pz[ids[c]] = p[c].z;
+    }
+}
+
+/* One colour pass (one `sim-elastic-<c>` dispatch, elasticCells.js:181-196). */
+void o_elastic_color(o_scene *sc, int color, int limit_only)
+{
+    build_tables();
+    real dt = (real)sc->dt;
+    int start = sc->color_start[color], count = sc->color_count[color];
+    for (int s = 0; s < count; s++) {
+        uint32_t body = sc->cells[(size_t)(start + s) * 2];
+        uint32_t lin = sc->cells[(size_t)(start + s) * 2 + 1];
+        const o_body *cp = &sc->bodies[body];
+        if (limit_only && !((real)cp->max_stretch > (real)1)) continue;
+        project_cell(sc, cp, lin, dt, limit_only);
+    }
+}
+
+/* predict.wgsl:19-36 */
+void o_predict(o_scene *sc)
+{
+    real dt = (real)sc->dt;
+    real gx = (real)sc->gravity[0], gy = (real)sc->gravity[1], gz = (real)sc->gravity[2];
+    for (int n = 0; n < sc->n_nodes; n++) {
+        if (sc->w[n] == (real)0) continue;
+        real vx = sc->vx[n] + gx * dt, vy = sc->vy[n] + gy * dt, vz = sc->vz[n] + gz * dt;
+        real px = sc->px[n], py = sc->py[n], pz = sc->pz[n];
+        sc->qx[n] = px;
+        sc->qy[n] = py;
+        sc->qz[n] = pz;
+        sc->px[n] = px + vx * dt;
+        sc->py[n] = py + vy * dt;
+        sc->pz[n] = pz + vz * dt;
+        sc->vx[n] = vx;
+        sc->vy[n] = vy;
+        sc->vz[n] = vz;
+    }
+}
+
+/* driveHard.wgsl:15-35 and driveCompliant.wgsl:24-41 */
+void o_drive(o_scene *sc)
+{
+    real dt = (real)sc->dt;
+    v3 target = mk((real)sc->target[0], (real)sc->target[1], (real)sc->target[2]);
+    v3 tvel = mk((real)sc->target_vel[0], (real)sc->target_vel[1], (real)sc->target_vel[2]);
+    for (int i = 0; i < sc->n_driven; i++) {
+        uint32_t n = sc->d_node[i];
+        v3 off = mk(sc->d_off[i * 3], sc->d_off[i * 3 + 1], sc->d_off[i * 3 + 2]);
+        v3 goal = add(target, off);
+        v3 here = load3(sc->px, sc->py, sc->pz, n);
+        if (sc->drive_law == 0) {
+            v3 step = scale(tvel, dt);
+            v3 toGoal = sub(goal, here);
+            v3 next = add(here, dot(step, step) >= dot(toGoal, toGoal) ? toGoal : step);
+            sc->qx[n] = here.x;
+            sc->qy[n] = here.y;
+            sc->qz[n] = here.z;
+            sc->px[n] = next.x;
+            sc->py[n] = next.y;
+            sc->pz[n] = next.z;
+            v3 v = sub(next, here);
+            sc->vx[n] = v.x / dt;
+            sc->vy[n] = v.y / dt;
+            sc->vz[n] = v.z / dt;
+        } else {
+            real dt2 = dt * dt;
+            real rt = (real)sc->response_time;
+            real tau2 = rt * rt;
+            real frac = dt2 / (dt2 + tau2);
+            v3 np = add(here, scale(sub(goal, here), frac));
+            sc->px[n] = np.x;
+            sc->py[n] = np.y;
+            sc->pz[n] = np.z;
+        }
+    }
+}
+
+/* gridCoord / bucketAt, contacts.wgsl:84-91 */
+static void grid_coord(const o_scene *sc, v3 p, uint32_t out[3])
+{
+    real o[3] = { (real)sc->grid_origin[0], (real)sc->grid_origin[1], (real)sc->grid_origin[2] };
+    real size = (real)sc->bucket_size;
+    real q[3] = { p.x, p.y, p.z };
+    for (int a = 0; a < 3; a++) {
+        real last = (real)sc->grid_dims[a] - (real)1;
+        real g = FLOOR((q[a] - o[a]) / size);
+        real c = g < (real)0 ? (real)0 : g; /* clamp = min(max(g, 0), last) */
+        c = c > last ? last : c;
+        out[a] = (uint32_t)c;
+    }
+}
+
+static uint32_t bucket_at(const o_scene *sc, const uint32_t c[3])
+{
+    return c[0] + sc->grid_dims[0] * (c[1] + sc->grid_dims[1] * c[2]);
+}
+
+/* The four contact passes (bodyContact.js:137-148): bin.wgsl:9-23 into
+ * the canonical bucket order sort.wgsl:14-53 establishes (ascending record
+ * index, the lowest `bucket_slots` kept), collide.wgsl:16-125, settle.wgsl:20-36. */
+void o_contact(o_scene *sc)
+{
+    int S = sc->n_surf;
+    if (S == 0) return;
+    uint32_t slots = sc->bucket_slots;
+    uint32_t B = sc->grid_dims[0] * sc->grid_dims[1] * sc->grid_dims[2];
+    uint32_t *count = (uint32_t *)calloc(B, sizeof(uint32_t));
+    uint32_t *slot = (uint32_t *)malloc((size_t)B * slots * sizeof(uint32_t));
+    real *delta = (real *)malloc((size_t)S * 4 * sizeof(real));
+
+    for (int s = 0; s < S; s++) {
+        uint32_t node = sc->s_node[s];
+        uint32_t c[3];
+        grid_coord(sc, load3(sc->px, sc->py, sc->pz, node), c);
+        uint32_t b = bucket_at(sc, c);
+        if (sc->key_out) sc->key_out[s] = b;
+        if (count[b] < slots) slot[(size_t)b * slots + count[b]] = (uint32_t)s;
+        count[b]++;
+    }
+
+    for (int s = 0; s < S; s++) {
+        uint32_t node = sc->s_node[s];
+        uint32_t body = sc->s_body[s];
+        real radius = sc->s_radius[s];
+        real wi = sc->w[node];
+        v3 pi = load3(sc->px, sc->py, sc->pz, node);
+        v3 movedI = sub(pi, load3(sc->qx, sc->qy, sc->qz, node));
+        real frictionI = (real)sc->bodies[body].friction;
+        v3 sum = mk(0, 0, 0);
+        real hits = 0;
+        int32_t npairs = 0;
+        uint32_t cc[3];
+        grid_coord(sc, pi, cc);
+        int cxi = (int)cc[0], cyi = (int)cc[1], czi = (int)cc[2];
+        int dims[3] = { (int)sc->grid_dims[0], (int)sc->grid_dims[1], (int)sc->grid_dims[2] };
+        for (int dz = -1; dz <= 1; dz++) {
+            int z = czi + dz;
+            if (z < 0 || z >= dims[2]) continue;
+            for (int dy = -1; dy <= 1; dy++) {
+                int y = cyi + dy;
+                if (y < 0 || y >= dims[1]) continue;
+                for (int dx = -1; dx <= 1; dx++) {
+                    int x = cxi + dx;
+                    if (x < 0 || x >= dims[0]) continue;
+                    uint32_t bc[3] = { (uint32_t)x, (uint32_t)y, (uint32_t)z };
+                    uint32_t b = bucket_at(sc, bc);
+                    uint32_t held = count[b] < slots ? count[b] : slots;
+                    for (uint32_t k = 0; k < held; k++) {
+                        uint32_t other = slot[(size_t)b * slots + k];
+                        uint32_t mateBody = sc->s_body[other];
+                        if (mateBody == body) continue;
+                        uint32_t mate = sc->s_node[other];
+                        v3 pj = load3(sc->px, sc->py, sc->pz, mate);
+                        v3 d = sub(pi, pj);
+                        real dist2 = dot(d, d);
+                        real reach = radius + sc->s_radius[other];
+                        if (dist2 >= reach * reach) continue;
+                        if (sc->pairs && npairs < sc->pair_cap) sc->pairs[(size_t)s * sc->pair_cap + npairs] = other;
+                        npairs++;
+                        real wj = sc->w[mate];
+                        real total = wi + wj;
+                        if (total <= (real)0) continue;
+                        real share = wi / total;
+                        real dist = SQRT(dist2);
+                        v3 normal = mk(0, node < mate ? (real)1 : (real)-1, 0);
+                        if (dist > (real)(float)1e-9) normal = mk(d.x / dist, d.y / dist, d.z / dist);
+                        real depth = reach - dist;
+                        v3 corr = scale(normal, depth * share);
+                        v3 sliding = sub(movedI, sub(pj, load3(sc->qx, sc->qy, sc->qz, mate)));
+                        v3 tangent = sub(sliding, scale(normal, dot(sliding, normal)));
+                        real slid = length3(tangent);
+                        if (slid > (real)(float)1e-9) {
+                            real fj = (real)sc->bodies[mateBody].friction;
+                            real mu = frictionI < fj ? frictionI : fj;
+                            real lim = (mu * depth) / slid;
+                            corr = sub(corr, scale(tangent, share * (lim < (real)1 ? lim : (real)1)));
+                        }
+                        sum = add(sum, corr);
+                        hits += (real)1;
+                    }
+                }
+            }
+        }
+        if (sc->pair_count) sc->pair_count[s] = npairs;
+        delta[s * 4] = sum.x;
+        delta[s * 4 + 1] = sum.y;
+        delta[s * 4 + 2] = sum.z;
+        delta[s * 4 + 3] = hits;
+    }
+
+    real gain = (real)sc->gain;
+    for (int s = 0; s < S; s++) {
+        real h = delta[s * 4 + 3];
+        if (h > (real)0) {
+            uint32_t node = sc->s_node[s];
+            real f = gain / h;
+            sc->px[node] = sc->px[node] + delta[s * 4] * f;
+            sc->py[node] = sc->py[node] + delta[s * 4 + 1] * f;
+            sc->pz[node] = sc->pz[node] + delta[s * 4 + 2] * f;
+        }
+    }
+    free(count);
+    free(slot);
+    free(delta);
+}
+
+/* resolvePlane, finalize.wgsl:29-42 */
+static v3 resolve_plane(v3 p, v3 prev, v3 normal, real depth, real friction)
+{
+    if (depth <= (real)0) return p;
+    v3 q = add(p, scale(normal, depth));
+    v3 delta = sub(q, prev);
+    v3 tangent = sub(delta, scale(normal, dot(delta, normal)));
+    real len = length3(tangent);
+    if (len > (real)(float)1e-9) {
+        real lim = (friction * depth) / len;
+        q = sub(q, scale(tangent, lim < (real)1 ? lim : (real)1));
+    }
+    return q;
+}
+
+/* bodyOfNode, state.wgsl:39-47 */
+static uint32_t body_of_node(const o_scene *sc, uint32_t node)
+{
+    for (int b = 0; b + 1 < sc->n_bodies; b++) {
+        if (node < sc->bodies[b].node_offset + sc->bodies[b].node_count) return (uint32_t)b;
+    }
+    return (uint32_t)(sc->n_bodies - 1);
+}
+
+/* The damping factor exp(-damping * dt) of finalize.wgsl:72: the f32 product
+ * of the two uniforms, exponentiated once in double and rounded. */
+static real damp_factor(double damping, double dtd)
+{
+    real a = -((real)damping * (real)dtd);
+    return (real)exp((double)a);
+}
+
+/* finalize.wgsl:44-74 */
+void o_finalize(o_scene *sc)
+{
+    real dt = (real)sc->dt;
+    real tmin[3] = { (real)sc->tank_min[0], (real)sc->tank_min[1], (real)sc->tank_min[2] };
+    real tmax[3] = { (real)sc->tank_max[0], (real)sc->tank_max[1], (real)sc->tank_max[2] };
+    for (int n = 0; n < sc->n_nodes; n++) {
+        if (sc->w[n] == (real)0) continue;
+        const o_body *bd = &sc->bodies[body_of_node(sc, (uint32_t)n)];
+        real fr = (real)bd->friction;
+        v3 prev = load3(sc->qx, sc->qy, sc->qz, (uint32_t)n);
+        v3 p = load3(sc->px, sc->py, sc->pz, (uint32_t)n);
+        p = resolve_plane(p, prev, mk(0, 1, 0), tmin[1] - p.y, fr);
+        p = resolve_plane(p, prev, mk(1, 0, 0), tmin[0] - p.x, fr);
+        p = resolve_plane(p, prev, mk(-1, 0, 0), p.x - tmax[0], fr);
+        p = resolve_plane(p, prev, mk(0, 0, 1), tmin[2] - p.z, fr);
+        p = resolve_plane(p, prev, mk(0, 0, -1), p.z - tmax[2], fr);
+        sc->px[n] = p.x;
+        sc->py[n] = p.y;
+        sc->pz[n] = p.z;
+        real damp = damp_factor(bd->damping, sc->dt);
+        v3 v = sub(p, prev);
+        sc->vx[n] = (v.x / dt) * damp;
+        sc->vy[n] = (v.y / dt) * damp;
+        sc->vz[n] = (v.z / dt) * damp;
+    }
+}
+
+/* Stepper.sequence (stepper.js:101-108) in the solve order of simulator.js:83:
+ * predict, [drive], elastic colours (+3 stretch sweeps when any body limits
+ * stretch, elasticCells.js:197-214), [contact], finalize. */
+void o_substep(o_scene *sc)
+{
+    build_tables();
+    o_predict(sc);
+    if (sc->n_driven > 0) o_drive(sc);
+    int limited = 0;
+    for (int b = 0; b < sc->n_bodies; b++) limited |= sc->bodies[b].max_stretch > 1.0;
+    for (int c = 0; c < 8; c++) {
+        if (sc->color_count[c] > 0) o_elastic_color(sc, c, 0);
+    }
+    if (limited) {
+        for (int sweep = 0; sweep < 3; sweep++) {
+            for (int c = 0; c < 8; c++) {
+                if (sc->color_count[c] > 0) o_elastic_color(sc, c, 1);
+            }
+        }
+    }
+    if (sc->contact && sc->n_surf > 0) o_contact(sc);
+    o_finalize(sc);
+}
+
+void o_substeps(o_scene *sc, int count)
+{
+    for (int i = 0; i < count; i++) o_substep(sc);
+}
+
+/* deformationAt (sample.wgsl:40-80) followed by deformedCovariance
+ * (covariance.wgsl:30-42) for every splat. */
+void o_deform_splats(const o_scene *sc, const o_splats *sp)
+{
+    for (int i = 0; i < sp->n; i++) {
+        const o_body *bd = &sc->bodies[sp->body[i]];
+        real h = (real)bd->h;
+        real inv = (real)1 / h;
+        real org[3] = { (real)bd->origin[0], (real)bd->origin[1], (real)bd->origin[2] };
+        real rest[3] = { sp->mx[i], sp->my[i], sp->mz[i] };
+        real local[3];
+        uint32_t base[3];
+        for (int a = 0; a < 3; a++) {
+            real grid = (rest[a] - org[a]) * inv;
+            real lastc = (real)bd->cell_dims[a] - (real)1;
+            real cell = FLOOR(grid);
+            cell = cell < (real)0 ? (real)0 : cell;
+            cell = cell > lastc ? lastc : cell;
+            local[a] = grid - cell;
+            base[a] = (uint32_t)cell;
+        }
+        v3 pos = mk(0, 0, 0), col0 = mk(0, 0, 0), col1 = mk(0, 0, 0), col2 = mk(0, 0, 0);
+        for (int c = 0; c < 8; c++) {
+            int far[3] = { c & 1, (c >> 1) & 1, (c >> 2) & 1 };
+            real f[3], dir[3];
+            for (int a = 0; a < 3; a++) {
+                f[a] = far[a] ? local[a] : (real)1 - local[a];
+                dir[a] = far[a] ? (real)1 : (real)-1;
+            }
+            uint32_t ix = base[0] + (uint32_t)far[0], iy = base[1] + (uint32_t)far[1], iz = base[2] + (uint32_t)far[2];
+            uint32_t n = bd->node_offset + ix + bd->node_dims[0] * (iy + bd->node_dims[1] * iz);
+            v3 p = load3(sc->px, sc->py, sc->pz, n);
+            pos = add(pos, scale(p, (f[0] * f[1]) * f[2]));
+            v3 g = scale(mk((dir[0] * f[1]) * f[2], (dir[1] * f[0]) * f[2], (dir[2] * f[0]) * f[1]), inv);
+            col0 = add(col0, scale(p, g.x));
+            col1 = add(col1, scale(p, g.y));
+            col2 = add(col2, scale(p, g.z));
+        }
+        sp->ox[i] = pos.x;
+        sp->oy[i] = pos.y;
+        sp->oz[i] = pos.z;
+
+        real x = sp->quat[i * 4], y = sp->quat[i * 4 + 1], z = sp->quat[i * 4 + 2], w = sp->quat[i * 4 + 3];
+        v3 r0 = mk((real)1 - (real)2 * (y * y + z * z), (real)2 * (x * y + z * w), (real)2 * (x * z - y * w));
+        v3 r1 = mk((real)2 * (x * y - z * w), (real)1 - (real)2 * (x * x + z * z), (real)2 * (y * z + x * w));
+        v3 r2 = mk((real)2 * (x * z + y * w), (real)2 * (y * z - x * w), (real)1 - (real)2 * (x * x + y * y));
+        /* gradient * basis[k] = col0*b.x + col1*b.y + col2*b.z, then * scale_k */
+        v3 a0 = scale(add(add(scale(col0, r0.x), scale(col1, r0.y)), scale(col2, r0.z)), sp->sx[i]);
+        v3 a1 = scale(add(add(scale(col0, r1.x), scale(col1, r1.y)), scale(col2, r1.z)), sp->sy[i]);
+        v3 a2 = scale(add(add(scale(col0, r2.x), scale(col1, r2.y)), scale(col2, r2.z)), sp->sz[i]);
+        /* a * transpose(a): (r, c) = a0[r]a0[c] + a1[r]a1[c] + a2[r]a2[c] */
+        size_t n = (size_t)sp->n;
+        sp->cov[0 * n + i] = (a0.x * a0.x + a1.x * a1.x) + a2.x * a2.x;
+        sp->cov[1 * n + i] = (a0.x * a0.y + a1.x * a1.y) + a2.x * a2.y;
+        sp->cov[2 * n + i] = (a0.x * a0.z + a1.x * a1.z) + a2.x * a2.z;
+        sp->cov[3 * n + i] = (a0.y * a0.y + a1.y * a1.y) + a2.y * a2.y;
+        sp->cov[4 * n + i] = (a0.y * a0.z + a1.y * a1.z) + a2.y * a2.z;
+        sp->cov[5 * n + i] = (a0.z * a0.z + a1.z * a1.z) + a2.z * a2.z;
+    }
+}
+
+/* Residuals of one cell's six tetrahedra at the current positions, for the
+ * error report: out[12] = (C_D, C_H) per tet. */
+void o_cell_residuals(const o_scene *sc, uint32_t body, uint32_t lin, REAL *out12)
+{
+    build_tables();
+    const o_body *cp = &sc->bodies[body];
+    uint32_t cx = lin % cp->cell_dims[0];
+    uint32_t cy = (lin / cp->cell_dims[0]) % cp->cell_dims[1];
+    uint32_t cz = lin / (cp->cell_dims[0] * cp->cell_dims[1]);
+    v3 p[8];
+    for (int c = 0; c < 8; c++) {
+        uint32_t n = cp->node_offset + (cx + (uint32_t)(c & 1)) +
+                     cp->node_dims[0] * ((cy + (uint32_t)((c >> 1) & 1)) + cp->node_dims[1] * (cz + (uint32_t)((c >> 2) & 1)));
+        p[c] = load3(sc->px, sc->py, sc->pz, n);
+    }
+    real rs = (real)1 / (real)cp->h;
+    for (int t = 0; t < 6; t++) {
+        const int *tet = KUHN_TETS[t];
+        m3 B = mscale(KUHN_INV[t], rs);
+        m3 Ds = { { sub(p[tet[1]], p[tet[0]]), sub(p[tet[2]], p[tet[0]]), sub(p[tet[3]], p[tet[0]]) } };
+        m3 F = mmul(Ds, B);
+        out12[t * 2] = SQRT((dot(F.c[0], F.c[0]) + dot(F.c[1], F.c[1])) + dot(F.c[2], F.c[2]));
+        out12[t * 2 + 1] = det3(F) - (real)cp->gamma;
+    }
+}
+
+int o_real_size(void) { return (int)sizeof(REAL); }

@@ -1,0 +1,250 @@
+"""ctypes binding of the CPU oracle (oracle.c), f32 and f64 builds.
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline. The
+solver never calls it.
+
+`OracleScene` runs a `vsim.scenes.Scene` through the restatement with the same
+scene-local arrays the GPU receives (node order, colour-grouped cells, surface
+records body by body).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+
+class OBody(C.Structure):
+    _fields_ = [
+        ("cell_dims", C.c_uint32 * 3), ("node_offset", C.c_uint32),
+        ("node_dims", C.c_uint32 * 3), ("node_count", C.c_uint32),
+        ("origin", C.c_double * 3),
+        ("h", C.c_double), ("alpha_dev", C.c_double), ("alpha_hyd", C.c_double), ("gamma", C.c_double),
+        ("relaxation", C.c_double), ("max_stretch", C.c_double), ("friction", C.c_double), ("damping", C.c_double),
+    ]
+
+
+class OScene(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int32), ("n_bodies", C.c_int32),
+        ("px", C.c_void_p), ("py", C.c_void_p), ("pz", C.c_void_p),
+        ("qx", C.c_void_p), ("qy", C.c_void_p), ("qz", C.c_void_p),
+        ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
+        ("w", C.c_void_p), ("bodies", C.c_void_p), ("cells", C.c_void_p),
+        ("color_start", C.c_int32 * 8), ("color_count", C.c_int32 * 8),
+        ("dt", C.c_double), ("gravity", C.c_double * 3), ("tank_min", C.c_double * 3), ("tank_max", C.c_double * 3),
+        ("n_surf", C.c_int32), ("contact", C.c_int32),
+        ("s_node", C.c_void_p), ("s_body", C.c_void_p), ("s_radius", C.c_void_p),
+        ("grid_origin", C.c_double * 3), ("bucket_size", C.c_double), ("gain", C.c_double),
+        ("grid_dims", C.c_uint32 * 3), ("bucket_slots", C.c_uint32),
+        ("n_driven", C.c_int32), ("drive_law", C.c_int32),
+        ("d_node", C.c_void_p), ("d_off", C.c_void_p),
+        ("target", C.c_double * 3), ("target_vel", C.c_double * 3), ("response_time", C.c_double),
+        ("key_out", C.c_void_p), ("pair_count", C.c_void_p), ("pairs", C.c_void_p),
+        ("pair_cap", C.c_int32), ("pad0", C.c_int32),
+    ]
+
+
+class OSplats(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("pad0", C.c_int32), ("body", C.c_void_p),
+        ("mx", C.c_void_p), ("my", C.c_void_p), ("mz", C.c_void_p), ("quat", C.c_void_p),
+        ("sx", C.c_void_p), ("sy", C.c_void_p), ("sz", C.c_void_p),
+        ("ox", C.c_void_p), ("oy", C.c_void_p), ("oz", C.c_void_p), ("cov", C.c_void_p),
+    ]
+
+
+def load(precision="f32"):
+    if precision in _libs:
+        return _libs[precision]
+    path = os.path.join(HERE, f"liboracle_{precision}.so")
+    if not os.path.exists(path):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    lib = C.CDLL(path)
+    sp = C.POINTER(OScene)
+    for name in ("o_predict", "o_drive", "o_contact", "o_finalize", "o_substep"):
+        getattr(lib, name).argtypes = [sp]
+        getattr(lib, name).restype = None
+    lib.o_substeps.argtypes = [sp, C.c_int]
+    lib.o_elastic_color.argtypes = [sp, C.c_int, C.c_int]
+    lib.o_deform_splats.argtypes = [sp, C.POINTER(OSplats)]
+    lib.o_project_tet.argtypes = [C.c_void_p] * 4 + [C.c_double] * 4 + [C.c_void_p]
+    lib.o_cell_residuals.argtypes = [sp, C.c_uint32, C.c_uint32, C.c_void_p]
+    lib.o_real_size.restype = C.c_int
+    assert lib.o_real_size() == (4 if precision == "f32" else 8)
+    _libs[precision] = lib
+    return lib
+
+
+def project_tet(p, prev, w, B, alpha_dev, alpha_hyd, gamma, relax=0.0, precision="f64"):
+    """One tetrahedron of the 2x2 block projection (project.wgsl:75-153);
+    p, prev (4, 3); B column-major 3x3 as 9 values; alphas over dt^2."""
+    dt = np.float32 if precision == "f32" else np.float64
+    lib = load(precision)
+    p = np.ascontiguousarray(p, dt).reshape(12)
+    prev = np.ascontiguousarray(prev, dt).reshape(12)
+    w = np.ascontiguousarray(w, dt)
+    B = np.ascontiguousarray(B, dt).reshape(9)
+    out = np.empty(12, dt)
+    lib.o_project_tet(p.ctypes.data, prev.ctypes.data, w.ctypes.data, B.ctypes.data, alpha_dev, alpha_hyd, gamma,
+                      relax, out.ctypes.data)
+    return out.reshape(4, 3)
+
+
+class OracleScene:
+    """A Scene's state and tables in the oracle's layout, in f32 or f64."""
+
+    def __init__(self, scene, precision="f32", pair_cap=0):
+        self.scene = scene
+        self.precision = precision
+        self.dtype = np.float32 if precision == "f32" else np.float64
+        self.lib = load(precision)
+        a = scene.arrays()
+        self.a = a
+        dt = self.dtype
+        n = len(a["w"])
+        self.p = np.ascontiguousarray(a["pos"].T, dt)         # (3, N)
+        self.q = self.p.copy()
+        self.v = np.ascontiguousarray(a["vel"].T, dt)
+        self.w = np.ascontiguousarray(a["w"], dt)
+        self.cells = np.ascontiguousarray(a["cells"], np.uint32)
+        bodies = (OBody * len(scene.bodies))()
+        for i, (b, off) in enumerate(zip(scene.bodies, a["node_offsets"])):
+            lat = b.lattice
+            dev, hyd, gamma = b.constants()
+            ob = bodies[i]
+            ob.cell_dims[:] = lat.cell_dims
+            ob.node_offset = int(off)
+            ob.node_dims[:] = lat.node_dims
+            ob.node_count = lat.node_count
+            ob.origin[:] = [float(np.float32(v)) for v in lat.origin]
+            ob.h, ob.alpha_dev, ob.alpha_hyd, ob.gamma = lat.h, dev, hyd, gamma
+            ob.relaxation, ob.max_stretch = b.material.relaxation, b.material.max_stretch
+            ob.friction, ob.damping = b.material.friction, b.material.damping
+        self.bodies = bodies
+        self.s_node = a["s_node"]
+        self.s_body = a["s_body"]
+        self.s_radius = np.ascontiguousarray(a["s_radius"], dt)
+        S = len(self.s_node)
+        self.keys = np.zeros(S, np.uint32)
+        self.pair_cap = pair_cap
+        self.pair_count = np.zeros(S, np.int32)
+        self.pairs = np.zeros((S, max(pair_cap, 1)), np.uint32)
+        sc = OScene()
+        sc.n_nodes, sc.n_bodies = n, len(scene.bodies)
+        sc.px, sc.py, sc.pz = (self.p[k].ctypes.data for k in range(3))
+        sc.qx, sc.qy, sc.qz = (self.q[k].ctypes.data for k in range(3))
+        sc.vx, sc.vy, sc.vz = (self.v[k].ctypes.data for k in range(3))
+        sc.w = self.w.ctypes.data
+        sc.bodies = C.cast(bodies, C.c_void_p)
+        sc.cells = self.cells.ctypes.data
+        sc.color_start[:] = [int(v) for v in a["color_start"]]
+        sc.color_count[:] = [int(v) for v in a["color_count"]]
+        sc.dt = scene.dt
+        sc.gravity[:] = scene.gravity
+        sc.tank_min[:] = scene.tank_min
+        sc.tank_max[:] = scene.tank_max
+        sc.n_surf = S if scene.contact else 0
+        sc.contact = 1 if scene.contact_active else 0
+        sc.s_node, sc.s_body, sc.s_radius = self.s_node.ctypes.data, self.s_body.ctypes.data, self.s_radius.ctypes.data
+        origin, edge, dims = scene.grid()
+        sc.grid_origin[:] = origin
+        sc.bucket_size = edge
+        sc.gain = 1.0
+        sc.grid_dims[:] = dims
+        sc.bucket_slots = 32
+        sc.key_out = self.keys.ctypes.data
+        if pair_cap:
+            sc.pair_count = self.pair_count.ctypes.data
+            sc.pairs = self.pairs.ctypes.data
+            sc.pair_cap = pair_cap
+        drv = scene.drive
+        if drv is not None:
+            self.d_node = np.ascontiguousarray(drv.nodes, np.uint32)
+            self.d_off = np.ascontiguousarray(drv.offsets, dt).reshape(-1)
+            sc.n_driven = len(self.d_node)
+            sc.drive_law = drv.law
+            sc.d_node, sc.d_off = self.d_node.ctypes.data, self.d_off.ctypes.data
+            sc.response_time = drv.response_time
+        self.sc = sc
+        self.frame_index = 0
+        # splats
+        means, quats, scales, owner = [], [], [], []
+        for i, b in enumerate(scene.bodies):
+            if b.n_splats:
+                means.append(b.splat_mean)
+                quats.append(b.splat_quat)
+                scales.append(b.splat_scale)
+                owner.append(np.full(b.n_splats, i, np.uint32))
+        G = scene.n_splats
+        self.G = G
+        if G:
+            m = np.concatenate(means).T
+            s = np.concatenate(scales).T
+            self.sm = np.ascontiguousarray(m, dt)
+            self.ss = np.ascontiguousarray(s, dt)
+            self.sq = np.ascontiguousarray(np.concatenate(quats), dt).reshape(-1)
+            self.sb = np.concatenate(owner)
+            self.out_mean = np.zeros((3, G), dt)
+            self.out_cov = np.zeros((6, G), dt)
+            sp = OSplats()
+            sp.n = G
+            sp.body = self.sb.ctypes.data
+            sp.mx, sp.my, sp.mz = (self.sm[k].ctypes.data for k in range(3))
+            sp.quat = self.sq.ctypes.data
+            sp.sx, sp.sy, sp.sz = (self.ss[k].ctypes.data for k in range(3))
+            sp.ox, sp.oy, sp.oz = (self.out_mean[k].ctypes.data for k in range(3))
+            sp.cov = self.out_cov.ctypes.data
+            self.sp = sp
+
+    # passes
+    def set_target(self, target, velocity):
+        self.sc.target[:] = [float(v) for v in target]
+        self.sc.target_vel[:] = [float(v) for v in velocity]
+
+    def predict(self):
+        self.lib.o_predict(C.byref(self.sc))
+
+    def drive(self):
+        self.lib.o_drive(C.byref(self.sc))
+
+    def elastic(self, color, limit_only=0):
+        self.lib.o_elastic_color(C.byref(self.sc), color, limit_only)
+
+    def contact(self):
+        self.lib.o_contact(C.byref(self.sc))
+
+    def finalize(self):
+        self.lib.o_finalize(C.byref(self.sc))
+
+    def substep(self):
+        self.lib.o_substep(C.byref(self.sc))
+
+    def substeps(self, count):
+        self.lib.o_substeps(C.byref(self.sc), count)
+
+    def frame(self, substeps=None, splats=False):
+        """One frame as Simulator.dispatch runs it (simulator.js:292-318)."""
+        n = substeps or self.scene.substeps
+        drv = self.scene.drive
+        if drv is not None:
+            t, v = drv.frame_target(self.frame_index, n * self.scene.dt)
+            self.set_target(t, v)
+        self.substeps(n)
+        self.frame_index += 1
+        if splats:
+            self.deform_splats()
+
+    def deform_splats(self):
+        if self.G:
+            self.lib.o_deform_splats(C.byref(self.sc), C.byref(self.sp))
+        return getattr(self, "out_mean", None), getattr(self, "out_cov", None)
+
+    def residuals(self, body, linear):
+        out = np.empty(12, self.dtype)
+        self.lib.o_cell_residuals(C.byref(self.sc), body, linear, out.ctypes.data)
+        return out.reshape(6, 2)

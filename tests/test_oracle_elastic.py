@@ -1,0 +1,251 @@
+"""Pins the oracle's elastic projection (project.wgsl:75-153 + xpbdCouple.wgsl)
+against the known answers of the reference's elastic.test.js, which the
+closed forms of the stable Neo-Hookean block fix: rest is a fixed point,
+momentum is conserved, an inert corner does not move, Kelvin-Voigt damping
+ignores rigid motion and removes energy, and the collapse barrier sits where
+collapseScale says."""
+import math
+
+import numpy as np
+import pytest
+
+from vsim import scenes
+
+H = 0.03125             # elastic.test.js:186 (h and 1/h exact in f32)
+DT = 1.0 / (60 * 20)    # elastic.test.js:189
+RATIOS = [0.25, 0.3, 0.35, 0.45, 0.49]
+KUHN_TETS = [[0, 1, 3, 7], [0, 5, 1, 7], [0, 3, 2, 7], [0, 2, 6, 7], [0, 4, 5, 7], [0, 6, 4, 7]]
+CORNERS = np.array([[c & 1, (c >> 1) & 1, (c >> 2) & 1] for c in range(8)], float)
+
+
+def rest_inverse(t, h=H):
+    """Inverse rest shape of Kuhn tet t, column-major, from numpy's inverse."""
+    i = KUHN_TETS[t]
+    Dm = np.stack([(CORNERS[i[k]] - CORNERS[i[0]]) * h for k in (1, 2, 3)], axis=1)
+    return np.linalg.inv(Dm).T.reshape(9)  # column-major = row-major of the transpose
+
+
+def rest_tet(nu, t=0):
+    m = scenes.Material(poisson_ratio=nu)
+    vol = scenes.tet_rest_volume(H)
+    dev, hyd, gamma = scenes.element_compliance(m, vol)
+    node_mass = m.density * vol / 4
+    mu, lam = scenes.lame_parameters(m)
+    p = CORNERS[KUHN_TETS[t]] * H
+    return dict(p=p, B=rest_inverse(t), w=np.full(4, 1 / node_mass), c=(dev / DT ** 2, hyd / DT ** 2, gamma),
+                mu=mu, lam=lam, vol=vol)
+
+
+def project(o, s, p, prev=None, relax=0.0, w=None):
+    return o.project_tet(p, p if prev is None else prev, s["w"] if w is None else w, s["B"], *s["c"], relax)
+
+
+def moved(a, b):
+    return float(np.max(np.linalg.norm(np.asarray(b) - np.asarray(a), axis=1)))
+
+
+def volume(p):
+    return np.linalg.det(np.stack([p[1] - p[0], p[2] - p[0], p[3] - p[0]], axis=1)) / 6
+
+
+def run(o, s, p, substeps, relax=0.0):
+    pos = np.array(p, float)
+    vel = np.zeros_like(pos)
+    for _ in range(substeps):
+        prev = pos.copy()
+        pos = pos + vel * DT
+        pos = project(o, s, pos, prev, relax)
+        vel = (pos - prev) / DT
+    return pos
+
+
+def constraints(s, p):
+    F = np.stack([p[1] - p[0], p[2] - p[0], p[3] - p[0]], axis=1) @ s["B"].reshape(3, 3).T
+    return np.linalg.norm(F), np.linalg.det(F) - s["c"][2]
+
+
+def stored(s, p):
+    def at(q):
+        cd, ch = constraints(s, q)
+        return 0.5 * s["mu"] * s["vol"] * cd * cd + 0.5 * s["lam"] * s["vol"] * ch * ch
+    return at(p) - at(s["p"])
+
+
+def test_rest_is_a_fixed_point_at_every_ratio(oracle):
+    for nu in RATIOS:
+        s = rest_tet(nu)
+        assert moved(s["p"], project(oracle, s, s["p"])) < 1e-9, nu
+
+
+def test_every_kuhn_tet_holds_rest(oracle):
+    for t in range(6):
+        s = rest_tet(0.25, t)
+        assert moved(s["p"], project(oracle, s, s["p"])) < 1e-9, t
+
+
+def test_f32_rest_fixed_point(oracle):
+    # h = 1/32 keeps B exact in f32, so the f32 build is at rest to f32 noise.
+    for nu in RATIOS:
+        s = rest_tet(nu)
+        out = oracle.project_tet(s["p"], s["p"], s["w"], s["B"], *s["c"], 0.0, precision="f32")
+        assert moved(s["p"], out) < 1e-7, nu
+
+
+def test_free_tet_keeps_its_volume_for_a_second(oracle):
+    s = rest_tet(0.25)
+    assert volume(run(oracle, s, s["p"], 1200)) / volume(s["p"]) == pytest.approx(1, abs=5e-7)
+
+
+def test_squashed_tet_rings_about_rest_volume(oracle):
+    s = rest_tet(0.25)
+    ratio = volume(run(oracle, s, s["p"] * 0.9, 1200)) / volume(s["p"])
+    assert 0.6 < ratio < 1.4
+
+
+def test_momentum_is_conserved(oracle):
+    s = rest_tet(0.3)
+    sq = s["p"] * np.array([1, 0.7, 1])
+    out = project(oracle, s, sq)
+    assert np.all(np.abs((out - sq).sum(axis=0)) < 1e-12)
+
+
+def test_collapsed_tet_stays_finite(oracle):
+    s = rest_tet(0.35)
+    out = project(oracle, s, np.zeros((4, 3)))
+    assert np.all(np.isfinite(out))
+
+
+def test_inert_corner_does_not_move_and_changes_the_others(oracle):
+    s = rest_tet(0.3)
+    sq = s["p"] * np.array([1, 0.85, 1])
+    driven = s["w"].copy()
+    driven[0] = 0
+    out = project(oracle, s, sq, w=driven)
+    assert moved(sq[:1], out[:1]) == 0
+    free = project(oracle, s, sq)
+    for i in range(1, 4):
+        assert moved(free[i:i + 1], out[i:i + 1]) > 1e-9
+
+
+RELAX = 0.04 / DT
+
+
+def test_kelvin_voigt_is_a_no_op_at_zero_relaxation(oracle):
+    s = rest_tet(0.35)
+    sq = s["p"] * np.array([1, 0.8, 1])
+    bare = project(oracle, s, sq)
+    damped = project(oracle, s, sq, prev=s["p"], relax=0.0)
+    assert moved(bare, damped) == 0
+
+
+def test_kelvin_voigt_keeps_rest_and_ignores_translation(oracle):
+    for nu in RATIOS:
+        s = rest_tet(nu)
+        assert moved(s["p"], project(oracle, s, s["p"], relax=RELAX)) < 1e-9
+    s = rest_tet(0.35)
+    shifted = s["p"] + np.array([0.004, -0.011, 0.002])
+    assert moved(shifted, project(oracle, s, shifted, prev=s["p"], relax=RELAX)) < 1e-9
+
+
+def test_kelvin_voigt_ignores_spin_to_second_order(oracle):
+    s = rest_tet(0.35)
+    a = 0.002
+    R = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+    spun = s["p"] @ R.T
+    sheared = s["p"] + np.outer(s["p"][:, 1] * a, [1, 0, 0])
+    shear = moved(sheared, project(oracle, s, sheared, prev=s["p"], relax=RELAX))
+    assert moved(spun, project(oracle, s, spun, prev=s["p"], relax=RELAX)) < shear / 100
+
+
+def test_kelvin_voigt_lowers_the_peak_deformation(oracle):
+    s = rest_tet(0.35)
+
+    def peak(relax):
+        pos = s["p"].copy()
+        vel = np.stack([s["p"][:, 1] * 2, np.zeros(4), np.zeros(4)], axis=1)
+        worst = 0.0
+        for _ in range(200):
+            prev = pos.copy()
+            pos = pos + vel * DT
+            pos = project(oracle, s, pos, prev, relax)
+            vel = (pos - prev) / DT
+            worst = max(worst, stored(s, pos))
+        return worst
+
+    assert peak(0.012 / DT) < peak(0) * 0.98
+
+
+def collapse_scale(gamma):
+    """Root of s^3 + s^2 + s = gamma - 1 by bisection (compliance.js:78-90)."""
+    lo, hi = 0.0, 4.0
+    for _ in range(40):
+        m = (lo + hi) / 2
+        if m ** 3 + m ** 2 + m < gamma - 1:
+            lo = m
+        else:
+            hi = m
+    return (lo + hi) / 2
+
+
+def test_collapse_scale_known_values():
+    assert collapse_scale(4) == pytest.approx(1, abs=1e-6)
+    assert collapse_scale(2) == pytest.approx(0.5437, abs=5e-4)
+    assert collapse_scale(1) == pytest.approx(0, abs=1e-6)
+    assert collapse_scale(5) > 1
+
+
+def test_barrier_is_where_collapse_scale_says(oracle):
+    # elastic.test.js:539-567: bisect for the largest uniform squash a free
+    # tetrahedron survives and compare with the closed-form barrier.
+    for nu in (0.25, 0.3, 0.35):
+        s = rest_tet(nu)
+        rest = volume(s["p"])
+
+        def survives(f):
+            r = volume(run(oracle, s, s["p"] * f, 3000)) / rest
+            return 0.5 < r < 2
+
+        ok, lost = 1.0, 0.05
+        for _ in range(18):
+            mid = (ok + lost) / 2
+            if survives(mid):
+                ok = mid
+            else:
+                lost = mid
+        assert abs(ok - collapse_scale(s["c"][2])) < 0.5 * 10 ** -2, nu
+
+
+def test_compliance_constants_reduce_to_lame():
+    m = scenes.Material(poisson_ratio=0.3)
+    mu, lam = scenes.lame_parameters(m)
+    dev, hyd, _ = scenes.element_compliance(m, scenes.tet_rest_volume(H))
+    assert (dev * mu) / (hyd * lam) == pytest.approx(1, abs=1e-12)
+    assert dev * mu == pytest.approx(1 / scenes.tet_rest_volume(H), rel=1e-12)
+
+
+def test_lattice_block_at_rest_without_gravity_stays_put(oracle):
+    # The cell path (gather, 6 tets in order, scatter) on a free 3^3 block:
+    # no gravity, no damping -> nothing moves beyond f32 rounding of B.
+    sc = scenes.jelly_block(3, h=H, pinned=False, tank_margin=1.0)
+    sc.gravity = (0.0, 0.0, 0.0)
+    o = oracle.OracleScene(sc, "f64")
+    before = o.p.copy()
+    o.substeps(20)
+    assert np.max(np.abs(o.p - before)) < 1e-12
+
+
+def test_lattice_block_conserves_momentum(oracle):
+    # A sheared free block: one colour sweep moves nodes but not their
+    # mass-weighted centre (equal masses in a colour's cells).
+    sc = scenes.jelly_block(4, h=H, pinned=False, tank_margin=1.0)
+    sc.gravity = (0.0, 0.0, 0.0)
+    o = oracle.OracleScene(sc, "f64")
+    o.p[0] += 0.2 * (o.p[1] - o.p[1].min())
+    o.q[:] = o.p
+    mass = np.where(o.w > 0, 1 / np.where(o.w > 0, o.w, 1), 0)
+    before = (o.p * mass).sum(axis=1)
+    for c in range(8):
+        o.elastic(c)
+    after = (o.p * mass).sum(axis=1)
+    assert np.all(np.abs(after - before) < 1e-12)
+    assert np.max(np.abs(o.p - o.q)) > 1e-4

@@ -1,0 +1,92 @@
+// kernels.h -- argument views and host launchers of the gfx950 kernels.
+//
+// Every pass runs over a tile buffer: a 16-byte header whose first word is the
+// tile count, then vs_tile records. A workgroup takes whole tiles, so a tile's
+// body and scene are uniform across it and their constants come in through
+// scalar loads. The count lives in device memory, so a launch captured in a
+// hipGraph stays valid when the host rewrites the tables between replays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/vsim.h"
+
+namespace vsim {
+
+struct NodeView {
+    float *px, *py, *pz;  // position planes
+    float *qx, *qy, *qz;  // position at substep start
+    float *vx, *vy, *vz;  // velocity planes
+    const float *w;       // inverse mass, zero where prescribed or inactive
+};
+
+struct TileView {
+    const uint32_t *count;  // header word 0
+    const vs_tile *tiles;   // header + 16 bytes
+};
+
+// Contact surface records as planes (struct Surface, contacts.wgsl:36-46).
+struct SurfaceView {
+    const uint32_t *node;
+    const uint32_t *body;
+    const float *radius;
+    uint32_t *key;   // bucket of the record this substep
+    uint32_t *rank;  // arrival slot returned by the bucket counter
+    float *dx, *dy, *dz, *hits;
+};
+
+// contactGrid (bodyContact.js:125-127): counters, then slots per bucket.
+struct GridView {
+    uint32_t *count;
+    uint32_t *slots;
+};
+
+// Optional log of the pairs collide accepts (record, mate), `cap` per record.
+struct PairLog {
+    int32_t *count;
+    uint32_t *mates;
+    uint32_t cap;
+};
+
+struct DriveView {
+    const uint32_t *node;
+    const float *ox, *oy, *oz;
+};
+
+struct SplatView {
+    const float *mx, *my, *mz;
+    const float4 *quat;
+    const float *sx, *sy, *sz;
+    float *ox, *oy, *oz;
+    float *cov;        // 6 planes of `plane` floats: xx xy xz yy yz zz
+    uint32_t plane;
+};
+
+struct Shape {
+    uint32_t grid;     // workgroups
+    uint32_t block;    // 64, 128, 256 or 512
+    uint32_t xcd;      // 1: contiguous tile runs per XCD
+};
+
+// Host launchers; each enqueues one kernel on `stream`. They return the HIP
+// launch status.
+hipError_t launch_predict(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, TileView t);
+hipError_t launch_finalize(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
+                           const vs_scene_params *scenes, TileView t);
+hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
+                          const vs_scene_params *scenes, const uint32_t *cells, TileView t, uint32_t limit_only);
+hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes,
+                              SurfaceView sv, GridView g, TileView t);
+hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv,
+                               GridView g, TileView t);
+hipError_t launch_contact_collide(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
+                                  const vs_scene_params *scenes, SurfaceView sv, GridView g, TileView t,
+                                  PairLog log);
+hipError_t launch_contact_settle(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes,
+                                 SurfaceView sv, GridView g, TileView t);
+hipError_t launch_drive(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, DriveView d,
+                        TileView t, int compliant);
+hipError_t launch_splats(hipStream_t s, Shape sh, const float *px, const float *py, const float *pz,
+                         const vs_body *bodies, SplatView sp, TileView t);
+
+}  // namespace vsim

@@ -1,0 +1,711 @@
+// kernels.hip -- the per-substep hot path as hand-written gfx950 kernels.
+//
+// Arithmetic follows the evaluation order the oracle defines (left-to-right
+// sums, `dot(a,a) = (ax*ax + ay*ay) + az*az`, WGSL matrix products column by
+// column, the mat3.js determinant), with -ffp-contract=off and IEEE divide and
+// square root, so a substep is bit-for-bit the oracle's f32 substep. Where a
+// reference formula multiplies by a constant 0 or +-1 (the Kuhn rest-shape
+// inverses, the tank plane normals) the kernels drop the zero terms and fold
+// the signs; in IEEE arithmetic that changes no finite result.
+//
+// Layout: nodes as SoA planes, one wavefront lane per cell / node / record,
+// workgroups over tiles (kernels.h). No MFMA: the work is 3x3 algebra per
+// cell and the passes are bound by HBM, not by math.
+#include "kernels.h"
+
+namespace vsim {
+namespace {
+
+struct V3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ V3 mk(float x, float y, float z) { return V3{ x, y, z }; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 mul(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b)
+{
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// Kuhn tetrahedra (kuhn.js:28-35) and the adjugates of their unit-cube edge
+// matrices (every det is 1), [tet][col][row], computed at compile time.
+struct Tet {
+    int i[4];
+    int inv[3][3];
+};
+
+constexpr int corner_axis(int c, int a) { return (c >> a) & 1; }
+
+constexpr Tet make_tet(int a, int b, int c, int d)
+{
+    Tet t{ { a, b, c, d }, {} };
+    int m[9] = {};  // column-major edge matrix, m[col*3+row]
+    const int k[3] = { b, c, d };
+    for (int col = 0; col < 3; col++)
+        for (int row = 0; row < 3; row++) m[col * 3 + row] = corner_axis(k[col], row) - corner_axis(a, row);
+    const int inv[9] = { m[4] * m[8] - m[7] * m[5], m[7] * m[2] - m[1] * m[8], m[1] * m[5] - m[4] * m[2],
+                         m[6] * m[5] - m[3] * m[8], m[0] * m[8] - m[6] * m[2], m[3] * m[2] - m[0] * m[5],
+                         m[3] * m[7] - m[6] * m[4], m[6] * m[1] - m[0] * m[7], m[0] * m[4] - m[3] * m[1] };
+    for (int col = 0; col < 3; col++)
+        for (int row = 0; row < 3; row++) t.inv[col][row] = inv[col * 3 + row];
+    return t;
+}
+
+constexpr Tet TETS[6] = { make_tet(0, 1, 3, 7), make_tet(0, 5, 1, 7), make_tet(0, 3, 2, 7),
+                          make_tet(0, 2, 6, 7), make_tet(0, 4, 5, 7), make_tet(0, 6, 4, 7) };
+
+// sum_k sign_k * (x_k * s), left to right, the first non-zero term opening
+// the sum: the WGSL product with the zero entries (exact zeros) dropped.
+__device__ __forceinline__ V3 combine(const V3 *x, int s0, int s1, int s2, float s)
+{
+    V3 acc = mk(0.f, 0.f, 0.f);
+    bool open = false;
+    const int sg[3] = { s0, s1, s2 };
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (sg[k] == 0) continue;
+        const V3 t = mul(x[k], s);
+        if (!open) {
+            acc = sg[k] > 0 ? t : neg(t);
+            open = true;
+        } else {
+            acc = sg[k] > 0 ? add(acc, t) : sub(acc, t);
+        }
+    }
+    return acc;
+}
+
+// xpbdCouple, xpbdCouple.wgsl:58-71.
+__device__ __forceinline__ void couple(float cD, float cH, float aD, float aH, float relax, float rD, float rH,
+                                       float gx, float gy, float gz, float &lD, float &lH)
+{
+    const float stiffened = 1.0f + relax;
+    const float a00 = gx * stiffened + aD;
+    const float a01 = gy * stiffened;
+    const float a11 = gz * stiffened + aH;
+    const float det = a00 * a11 - a01 * a01;
+    if (det < 1e-20f) {
+        lD = 0.f;
+        lH = 0.f;
+        return;
+    }
+    const float bD = -(cD + rD * relax);
+    const float bH = -(cH + rH * relax);
+    lD = (a11 * bD - a01 * bH) / det;
+    lH = (a00 * bH - a01 * bD) / det;
+}
+
+// One tetrahedron of project.wgsl:75-153.
+template <int T>
+__device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const float (&w)[8], float rs,
+                                            float gamma, float aD, float aH, float relax)
+{
+    constexpr Tet tet = TETS[T];
+    constexpr int i0 = tet.i[0], i1 = tet.i[1], i2 = tet.i[2], i3 = tet.i[3];
+    const float w0 = w[i0], w1 = w[i1], w2 = w[i2], w3 = w[i3];
+    if (((w0 + w1) + w2) + w3 == 0.0f) return;
+
+    const V3 e[3] = { sub(p[i1], p[i0]), sub(p[i2], p[i0]), sub(p[i3], p[i0]) };
+    // F[:,j] = sum_k e_k * B[j][k], B = inv * (1/h)
+    V3 F[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) F[j] = combine(e, tet.inv[j][0], tet.inv[j][1], tet.inv[j][2], rs);
+
+    const float cDev = sqrtf((dot(F[0], F[0]) + dot(F[1], F[1])) + dot(F[2], F[2]));
+    // mat3Determinant (mat3.js:30-36) on the columns of F
+    const float cHyd = (F[0].x * (F[1].y * F[2].z - F[2].y * F[1].z) - F[1].x * (F[0].y * F[2].z - F[2].y * F[0].z) +
+                        F[2].x * (F[0].y * F[1].z - F[1].y * F[0].z)) -
+                       gamma;
+
+    const float k = cDev > 1e-9f ? 1.0f / cDev : 0.0f;
+    const V3 X[3] = { mul(F[0], k), mul(F[1], k), mul(F[2], k) };
+    const V3 H[3] = { cross(F[1], F[2]), cross(F[2], F[0]), cross(F[0], F[1]) };
+    // (X * B^T)[:,j] = sum_i X_i * B[i][j]
+    V3 d[4], h[4];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        d[j + 1] = combine(X, tet.inv[0][j], tet.inv[1][j], tet.inv[2][j], rs);
+        h[j + 1] = combine(H, tet.inv[0][j], tet.inv[1][j], tet.inv[2][j], rs);
+    }
+    d[0] = neg(add(add(d[1], d[2]), d[3]));
+    h[0] = neg(add(add(h[1], h[2]), h[3]));
+
+    const float gx = ((w0 * dot(d[0], d[0]) + w1 * dot(d[1], d[1])) + w2 * dot(d[2], d[2])) + w3 * dot(d[3], d[3]);
+    const float gy = ((w0 * dot(d[0], h[0]) + w1 * dot(d[1], h[1])) + w2 * dot(d[2], h[2])) + w3 * dot(d[3], h[3]);
+    const float gz = ((w0 * dot(h[0], h[0]) + w1 * dot(h[1], h[1])) + w2 * dot(h[2], h[2])) + w3 * dot(h[3], h[3]);
+
+    const V3 x0 = sub(p[i0], q[i0]), x1 = sub(p[i1], q[i1]), x2 = sub(p[i2], q[i2]), x3 = sub(p[i3], q[i3]);
+    const float rD = ((dot(d[0], x0) + dot(d[1], x1)) + dot(d[2], x2)) + dot(d[3], x3);
+    const float rH = ((dot(h[0], x0) + dot(h[1], x1)) + dot(h[2], x2)) + dot(h[3], x3);
+
+    float lD, lH;
+    couple(cDev, cHyd, aD, aH, relax, rD, rH, gx, gy, gz, lD, lH);
+    p[i0] = add(p[i0], mul(add(mul(d[0], lD), mul(h[0], lH)), w0));
+    p[i1] = add(p[i1], mul(add(mul(d[1], lD), mul(h[1], lH)), w1));
+    p[i2] = add(p[i2], mul(add(mul(d[2], lD), mul(h[2], lH)), w2));
+    p[i3] = add(p[i3], mul(add(mul(d[3], lD), mul(h[3], lH)), w3));
+}
+
+// Tile scheduling. Interleaved: workgroup b takes tiles b, b+G, ... .
+// Chunked: the 8 XCDs each take one contiguous run of tiles (hardware deals
+// workgroup b to XCD b % 8), so every pass of a substep leaves a region's
+// nodes in the same XCD's L2.
+struct Range {
+    uint32_t first, end, stride;
+};
+
+__device__ __forceinline__ Range tile_range(uint32_t count, uint32_t xcd)
+{
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    if (xcd && G % 8u == 0u) {
+        const uint32_t lb = (b % 8u) * (G / 8u) + b / 8u;
+        return Range{ (uint32_t)(((uint64_t)lb * count) / G), (uint32_t)(((uint64_t)(lb + 1) * count) / G), 1u };
+    }
+    return Range{ b, count, G };
+}
+
+// STRETCH_EDGES (stretch.js:5-7) with f32 lengths 1, sqrt(2), sqrt(3).
+__device__ __forceinline__ float edge_len(int a, int b)
+{
+    const int d = ((a ^ b) & 1) + (((a ^ b) >> 1) & 1) + (((a ^ b) >> 2) & 1);
+    return d == 1 ? 1.0f : d == 2 ? 1.41421353816986083984375f : 1.73205077648162841796875f;
+}
+
+__device__ __forceinline__ V3 ld3(const float *x, const float *y, const float *z, uint32_t n)
+{
+    return mk(x[n], y[n], z[n]);
+}
+
+// ------------------------------------------------------------- elastic
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__restrict__ bodies,
+                                                   const vs_scene_params *__restrict__ scenes,
+                                                   const uint32_t *__restrict__ cells, TileView tv,
+                                                   uint32_t limit_only, uint32_t xcd)
+{
+    const Range r = tile_range(*tv.count, xcd);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_body bd = bodies[tile.body];
+        const float dt = scenes[bd.scene].dt;
+        const float maxStretch = bd.max_stretch;
+        if (limit_only && !(maxStretch > 1.0f)) continue;
+        const float invDt2 = 1.0f / (dt * dt);
+        const float aD = bd.alpha_dev * invDt2;
+        const float aH = bd.alpha_hyd * invDt2;
+        const float relax = bd.relaxation / dt;
+        const float rs = 1.0f / bd.h;
+        const uint32_t cdx = bd.cell_dims[0], cdy = bd.cell_dims[1];
+        const uint32_t ndx = bd.node_dims[0], ndxy = bd.node_dims[0] * bd.node_dims[1];
+
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t lin = cells[tile.begin + i];
+            const uint32_t cx = lin % cdx, cy = (lin / cdx) % cdy, cz = lin / (cdx * cdy);
+            const uint32_t n0 = bd.node_offset + cx + ndx * cy + ndxy * cz;
+            uint32_t ids[8];
+            V3 p[8], q[8];
+            float w[8];
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                ids[c] = n0 + (uint32_t)(c & 1) + ((c >> 1) & 1 ? ndx : 0u) + ((c >> 2) & 1 ? ndxy : 0u);
+                p[c] = ld3(n.px, n.py, n.pz, ids[c]);
+                q[c] = ld3(n.qx, n.qy, n.qz, ids[c]);
+                w[c] = n.w[ids[c]];
+            }
+            if (!limit_only) {
+                project_tet<0>(p, q, w, rs, bd.gamma, aD, aH, relax);
+                project_tet<1>(p, q, w, rs, bd.gamma, aD, aH, relax);
+                project_tet<2>(p, q, w, rs, bd.gamma, aD, aH, relax);
+                project_tet<3>(p, q, w, rs, bd.gamma, aD, aH, relax);
+                project_tet<4>(p, q, w, rs, bd.gamma, aD, aH, relax);
+                project_tet<5>(p, q, w, rs, bd.gamma, aD, aH, relax);
+            }
+            if (maxStretch > 1.0f) {
+                // project.wgsl:162-179
+                for (int sweep = 0; sweep < 2; sweep++) {
+#pragma unroll
+                    for (int a = 0; a < 8; a++) {
+#pragma unroll
+                        for (int b = a + 1; b < 8; b++) {
+                            const V3 delta = sub(p[b], p[a]);
+                            const float len = sqrtf(dot(delta, delta));
+                            const float maximum = (edge_len(a, b) * bd.h) * maxStretch;
+                            const float weight = w[a] + w[b];
+                            if (len > maximum && weight > 0.0f) {
+                                const V3 corr = mul(delta, (len - maximum) / (len * weight));
+                                p[a] = add(p[a], mul(corr, w[a]));
+                                p[b] = sub(p[b], mul(corr, w[b]));
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                n.px[ids[c]] = p[c].x;
+                n.py[ids[c]] = p[c].y;
+                n.pz[ids[c]] = p[c].z;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------ predict/finalize
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_predict(NodeView n, const vs_scene_params *__restrict__ scenes, TileView tv)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_scene_params &sc = scenes[tile.scene];
+        const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t k = tile.begin + i;
+            if (n.w[k] == 0.0f) continue;
+            const float vx = n.vx[k] + gx * dt, vy = n.vy[k] + gy * dt, vz = n.vz[k] + gz * dt;
+            const float px = n.px[k], py = n.py[k], pz = n.pz[k];
+            n.qx[k] = px;
+            n.qy[k] = py;
+            n.qz[k] = pz;
+            n.px[k] = px + vx * dt;
+            n.py[k] = py + vy * dt;
+            n.pz[k] = pz + vz * dt;
+            n.vx[k] = vx;
+            n.vy[k] = vy;
+            n.vz[k] = vz;
+        }
+    }
+}
+
+// resolvePlane (finalize.wgsl:29-42) for an axis-aligned normal +-e_A.
+template <int A, int SIGN>
+__device__ __forceinline__ V3 plane(V3 p, V3 prev, float depth, float friction)
+{
+    if (!(depth > 0.0f)) return p;
+    float c[3] = { p.x, p.y, p.z };
+    c[A] = SIGN > 0 ? c[A] + depth : c[A] - depth;
+    V3 q = mk(c[0], c[1], c[2]);
+    const V3 delta = sub(q, prev);
+    // tangent = delta - n * dot(delta, n): the A component cancels exactly.
+    float tg[3] = { delta.x, delta.y, delta.z };
+    tg[A] = 0.0f;
+    const V3 tangent = mk(tg[0], tg[1], tg[2]);
+    const float len = sqrtf(dot(tangent, tangent));
+    if (len > 1e-9f) {
+        const float lim = (friction * depth) / len;
+        q = sub(q, mul(tangent, lim < 1.0f ? lim : 1.0f));
+    }
+    return q;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_finalize(NodeView n, const vs_body *__restrict__ bodies,
+                                                    const vs_scene_params *__restrict__ scenes, TileView tv)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_scene_params &sc = scenes[tile.scene];
+        const float dt = sc.dt;
+        const float fr = bodies[tile.body].friction, damp = bodies[tile.body].damp;
+        const float lx = sc.tank_min[0], ly = sc.tank_min[1], lz = sc.tank_min[2];
+        const float hx = sc.tank_max[0], hz = sc.tank_max[2];
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t k = tile.begin + i;
+            if (n.w[k] == 0.0f) continue;
+            const V3 prev = ld3(n.qx, n.qy, n.qz, k);
+            V3 p = ld3(n.px, n.py, n.pz, k);
+            p = plane<1, 1>(p, prev, ly - p.y, fr);
+            p = plane<0, 1>(p, prev, lx - p.x, fr);
+            p = plane<0, -1>(p, prev, p.x - hx, fr);
+            p = plane<2, 1>(p, prev, lz - p.z, fr);
+            p = plane<2, -1>(p, prev, p.z - hz, fr);
+            n.px[k] = p.x;
+            n.py[k] = p.y;
+            n.pz[k] = p.z;
+            n.vx[k] = ((p.x - prev.x) / dt) * damp;
+            n.vy[k] = ((p.y - prev.y) / dt) * damp;
+            n.vz[k] = ((p.z - prev.z) / dt) * damp;
+        }
+    }
+}
+
+// -------------------------------------------------------------- contact
+// gridCoord (contacts.wgsl:84-87)
+__device__ __forceinline__ void grid_coord(const vs_scene_params &sc, V3 p, uint32_t &x, uint32_t &y, uint32_t &z)
+{
+    const float c[3] = { p.x, p.y, p.z };
+    uint32_t o[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float last = (float)sc.grid_dims[a] - 1.0f;
+        float g = floorf((c[a] - sc.grid_origin[a]) / sc.bucket_size);
+        g = g < 0.0f ? 0.0f : g;
+        g = g > last ? last : g;
+        o[a] = (uint32_t)g;
+    }
+    x = o[0];
+    y = o[1];
+    z = o[2];
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bin(NodeView n, const vs_scene_params *__restrict__ scenes, SurfaceView sv,
+                                               GridView g, TileView tv)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_scene_params &sc = scenes[tile.scene];
+        const uint32_t slots = sc.bucket_slots, dx = sc.grid_dims[0], dy = sc.grid_dims[1], gb = sc.grid_base;
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t s = tile.begin + i;
+            const uint32_t node = sv.node[s];
+            uint32_t x, y, z;
+            grid_coord(sc, ld3(n.px, n.py, n.pz, node), x, y, z);
+            const uint32_t b = x + dx * (y + dy * z);
+            sv.key[s] = b;
+            const uint32_t slot = atomicAdd(&g.count[gb + b], 1u);
+            sv.rank[s] = slot;
+            if (slot < slots) g.slots[(size_t)(gb + b) * slots + slot] = s;
+        }
+    }
+}
+
+// sort.wgsl:14-53, run by the record that arrived first in each bucket.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restrict__ scenes, SurfaceView sv, GridView g,
+                                                TileView tv)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_scene_params &sc = scenes[tile.scene];
+        const uint32_t slots = sc.bucket_slots, gb = sc.grid_base;
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t s = tile.begin + i;
+            if (sv.rank[s] != 0u) continue;
+            const uint32_t b = sv.key[s];
+            const uint32_t held = g.count[gb + b];
+            uint32_t *base = g.slots + (size_t)(gb + b) * slots;
+            if (held > slots) {
+                // Overflow: keep the lowest `slots` record ids of the bucket.
+                uint32_t kept = 0;
+                const uint32_t end = sc.surf_base + sc.surf_count;
+                for (uint32_t o = sc.surf_base; o < end && kept < slots; o++) {
+                    if (sv.key[o] == b) base[kept++] = o;
+                }
+                continue;
+            }
+            for (uint32_t a = 1; a < held; a++) {
+                const uint32_t value = base[a];
+                uint32_t at = a;
+                while (at > 0u) {
+                    const uint32_t previous = base[at - 1u];
+                    if (previous <= value) break;
+                    base[at] = previous;
+                    at--;
+                }
+                base[at] = value;
+            }
+        }
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_collide(NodeView n, const vs_body *__restrict__ bodies,
+                                                   const vs_scene_params *__restrict__ scenes, SurfaceView sv,
+                                                   GridView g, TileView tv, PairLog log)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_scene_params &sc = scenes[tile.scene];
+        const uint32_t slots = sc.bucket_slots, gb = sc.grid_base;
+        const int dx = (int)sc.grid_dims[0], dy = (int)sc.grid_dims[1], dz = (int)sc.grid_dims[2];
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t s = tile.begin + i;
+            const uint32_t node = sv.node[s], body = sv.body[s];
+            const float radius = sv.radius[s];
+            const float wi = n.w[node];
+            const V3 pi = ld3(n.px, n.py, n.pz, node);
+            const V3 movedI = sub(pi, ld3(n.qx, n.qy, n.qz, node));
+            const float frictionI = bodies[body].friction;
+            const uint32_t key = sv.key[s];
+            const int cx = (int)(key % (uint32_t)dx), cy = (int)((key / (uint32_t)dx) % (uint32_t)dy),
+                      cz = (int)(key / (uint32_t)(dx * dy));
+            V3 sum = mk(0.f, 0.f, 0.f);
+            float hits = 0.f;
+            int32_t logged = 0;
+            for (int oz = -1; oz <= 1; oz++) {
+                const int z = cz + oz;
+                if (z < 0 || z >= dz) continue;
+                for (int oy = -1; oy <= 1; oy++) {
+                    const int y = cy + oy;
+                    if (y < 0 || y >= dy) continue;
+                    for (int ox = -1; ox <= 1; ox++) {
+                        const int x = cx + ox;
+                        if (x < 0 || x >= dx) continue;
+                        const uint32_t bk = gb + (uint32_t)(x + dx * (y + dy * z));
+                        const uint32_t cnt = g.count[bk];
+                        const uint32_t held = cnt < slots ? cnt : slots;
+                        const uint32_t *base = g.slots + (size_t)bk * slots;
+                        for (uint32_t k = 0; k < held; k++) {
+                            const uint32_t other = base[k];
+                            const uint32_t mateBody = sv.body[other];
+                            if (mateBody == body) continue;
+                            const uint32_t mate = sv.node[other];
+                            const V3 pj = ld3(n.px, n.py, n.pz, mate);
+                            const V3 d = sub(pi, pj);
+                            const float dist2 = dot(d, d);
+                            const float reach = radius + sv.radius[other];
+                            if (dist2 >= reach * reach) continue;
+                            if (log.cap) {
+                                if ((uint32_t)logged < log.cap) log.mates[(size_t)s * log.cap + logged] = other;
+                                logged++;
+                            }
+                            const float wj = n.w[mate];
+                            const float total = wi + wj;
+                            if (total <= 0.0f) continue;
+                            const float share = wi / total;
+                            const float dist = sqrtf(dist2);
+                            V3 normal = mk(0.f, node < mate ? 1.0f : -1.0f, 0.f);
+                            if (dist > 1e-9f) normal = mk(d.x / dist, d.y / dist, d.z / dist);
+                            const float depth = reach - dist;
+                            V3 corr = mul(normal, depth * share);
+                            const V3 sliding = sub(movedI, sub(pj, ld3(n.qx, n.qy, n.qz, mate)));
+                            const V3 tangent = sub(sliding, mul(normal, dot(sliding, normal)));
+                            const float slid = sqrtf(dot(tangent, tangent));
+                            if (slid > 1e-9f) {
+                                const float fj = bodies[mateBody].friction;
+                                const float mu = frictionI < fj ? frictionI : fj;
+                                const float lim = (mu * depth) / slid;
+                                corr = sub(corr, mul(tangent, share * (lim < 1.0f ? lim : 1.0f)));
+                            }
+                            sum = add(sum, corr);
+                            hits += 1.0f;
+                        }
+                    }
+                }
+            }
+            if (log.cap) log.count[s] = logged;
+            sv.dx[s] = sum.x;
+            sv.dy[s] = sum.y;
+            sv.dz[s] = sum.z;
+            sv.hits[s] = hits;
+        }
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_settle(NodeView n, const vs_scene_params *__restrict__ scenes,
+                                                  SurfaceView sv, GridView g, TileView tv)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_scene_params &sc = scenes[tile.scene];
+        const float gain = sc.gain;
+        const uint32_t gb = sc.grid_base;
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t s = tile.begin + i;
+            if (sv.rank[s] == 0u) g.count[gb + sv.key[s]] = 0u;
+            const float h = sv.hits[s];
+            if (h > 0.0f) {
+                const uint32_t node = sv.node[s];
+                const float f = gain / h;
+                n.px[node] = n.px[node] + sv.dx[s] * f;
+                n.py[node] = n.py[node] + sv.dy[s] * f;
+                n.pz[node] = n.pz[node] + sv.dz[s] * f;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- drive
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_drive(NodeView n, const vs_scene_params *__restrict__ scenes, DriveView d,
+                                                 TileView tv, int law)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_scene_params &sc = scenes[tile.scene];
+        const float dt = sc.dt;
+        const V3 target = mk(sc.target[0], sc.target[1], sc.target[2]);
+        const V3 tvel = mk(sc.target_vel[0], sc.target_vel[1], sc.target_vel[2]);
+        const int compliant = law < 0 ? (int)sc.drive_law : law;
+        const float rt = sc.response_time;
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t k = tile.begin + i;
+            const uint32_t node = d.node[k];
+            const V3 goal = add(target, mk(d.ox[k], d.oy[k], d.oz[k]));
+            const V3 here = ld3(n.px, n.py, n.pz, node);
+            if (!compliant) {
+                const V3 step = mul(tvel, dt);
+                const V3 toGoal = sub(goal, here);
+                const V3 next = add(here, dot(step, step) >= dot(toGoal, toGoal) ? toGoal : step);
+                n.qx[node] = here.x;
+                n.qy[node] = here.y;
+                n.qz[node] = here.z;
+                n.px[node] = next.x;
+                n.py[node] = next.y;
+                n.pz[node] = next.z;
+                const V3 v = sub(next, here);
+                n.vx[node] = v.x / dt;
+                n.vy[node] = v.y / dt;
+                n.vz[node] = v.z / dt;
+            } else {
+                const float dt2 = dt * dt;
+                const float tau2 = rt * rt;
+                const float frac = dt2 / (dt2 + tau2);
+                const V3 np = add(here, mul(sub(goal, here), frac));
+                n.px[node] = np.x;
+                n.py[node] = np.y;
+                n.pz[node] = np.z;
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------- splats
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_splats(const float *__restrict__ px, const float *__restrict__ py,
+                                                  const float *__restrict__ pz, const vs_body *__restrict__ bodies,
+                                                  SplatView sp, TileView tv)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_body bd = bodies[tile.body];
+        const float inv = 1.0f / bd.h;
+        const float last[3] = { (float)bd.cell_dims[0] - 1.0f, (float)bd.cell_dims[1] - 1.0f,
+                                (float)bd.cell_dims[2] - 1.0f };
+        const uint32_t ndx = bd.node_dims[0], ndxy = bd.node_dims[0] * bd.node_dims[1];
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t k = tile.begin + i;
+            const float rest[3] = { sp.mx[k], sp.my[k], sp.mz[k] };
+            float local[3];
+            uint32_t base[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const float grid = (rest[a] - bd.origin[a]) * inv;
+                float cell = floorf(grid);
+                cell = cell < 0.0f ? 0.0f : cell;
+                cell = cell > last[a] ? last[a] : cell;
+                local[a] = grid - cell;
+                base[a] = (uint32_t)cell;
+            }
+            const uint32_t n0 = bd.node_offset + base[0] + ndx * base[1] + ndxy * base[2];
+            V3 pos = mk(0.f, 0.f, 0.f), c0 = pos, c1 = pos, c2 = pos;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int fx = c & 1, fy = (c >> 1) & 1, fz = (c >> 2) & 1;
+                const float f0 = fx ? local[0] : 1.0f - local[0];
+                const float f1 = fy ? local[1] : 1.0f - local[1];
+                const float f2 = fz ? local[2] : 1.0f - local[2];
+                const uint32_t node = n0 + (uint32_t)fx + (fy ? ndx : 0u) + (fz ? ndxy : 0u);
+                const V3 p = ld3(px, py, pz, node);
+                pos = add(pos, mul(p, (f0 * f1) * f2));
+                // dir * f * f with dir = +-1: the sign is exact.
+                const float g0 = (fx ? f1 * f2 : -(f1 * f2)) * inv;
+                const float g1 = (fy ? f0 * f2 : -(f0 * f2)) * inv;
+                const float g2 = (fz ? f0 * f1 : -(f0 * f1)) * inv;
+                c0 = add(c0, mul(p, g0));
+                c1 = add(c1, mul(p, g1));
+                c2 = add(c2, mul(p, g2));
+            }
+            sp.ox[k] = pos.x;
+            sp.oy[k] = pos.y;
+            sp.oz[k] = pos.z;
+            const float4 qv = sp.quat[k];
+            const float x = qv.x, y = qv.y, z = qv.z, w = qv.w;
+            const V3 r0 = mk(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + z * w), 2.0f * (x * z - y * w));
+            const V3 r1 = mk(2.0f * (x * y - z * w), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + x * w));
+            const V3 r2 = mk(2.0f * (x * z + y * w), 2.0f * (y * z - x * w), 1.0f - 2.0f * (x * x + y * y));
+            const V3 a0 = mul(add(add(mul(c0, r0.x), mul(c1, r0.y)), mul(c2, r0.z)), sp.sx[k]);
+            const V3 a1 = mul(add(add(mul(c0, r1.x), mul(c1, r1.y)), mul(c2, r1.z)), sp.sy[k]);
+            const V3 a2 = mul(add(add(mul(c0, r2.x), mul(c1, r2.y)), mul(c2, r2.z)), sp.sz[k]);
+            const size_t P = sp.plane;
+            sp.cov[0 * P + k] = (a0.x * a0.x + a1.x * a1.x) + a2.x * a2.x;
+            sp.cov[1 * P + k] = (a0.x * a0.y + a1.x * a1.y) + a2.x * a2.y;
+            sp.cov[2 * P + k] = (a0.x * a0.z + a1.x * a1.z) + a2.x * a2.z;
+            sp.cov[3 * P + k] = (a0.y * a0.y + a1.y * a1.y) + a2.y * a2.y;
+            sp.cov[4 * P + k] = (a0.y * a0.z + a1.y * a1.z) + a2.y * a2.z;
+            sp.cov[5 * P + k] = (a0.z * a0.z + a1.z * a1.z) + a2.z * a2.z;
+        }
+    }
+}
+
+}  // namespace
+
+#define VSIM_DISPATCH_BLOCK(KERNEL, SH, STREAM, ...)                                                           \
+    do {                                                                                                       \
+        switch ((SH).block) {                                                                                  \
+        case 64: hipLaunchKernelGGL(KERNEL<64>, dim3((SH).grid), dim3(64), 0, STREAM, __VA_ARGS__); break;    \
+        case 128: hipLaunchKernelGGL(KERNEL<128>, dim3((SH).grid), dim3(128), 0, STREAM, __VA_ARGS__); break; \
+        case 256: hipLaunchKernelGGL(KERNEL<256>, dim3((SH).grid), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+        case 512: hipLaunchKernelGGL(KERNEL<512>, dim3((SH).grid), dim3(512), 0, STREAM, __VA_ARGS__); break; \
+        default: return hipErrorInvalidConfiguration;                                                          \
+        }                                                                                                      \
+        return hipGetLastError();                                                                              \
+    } while (0)
+
+hipError_t launch_predict(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, TileView t)
+{
+    VSIM_DISPATCH_BLOCK(k_predict, sh, s, n, scenes, t);
+}
+
+hipError_t launch_finalize(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies, const vs_scene_params *scenes,
+                           TileView t)
+{
+    VSIM_DISPATCH_BLOCK(k_finalize, sh, s, n, bodies, scenes, t);
+}
+
+hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies, const vs_scene_params *scenes,
+                          const uint32_t *cells, TileView t, uint32_t limit_only)
+{
+    VSIM_DISPATCH_BLOCK(k_elastic, sh, s, n, bodies, scenes, cells, t, limit_only, sh.xcd);
+}
+
+hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,
+                              GridView g, TileView t)
+{
+    VSIM_DISPATCH_BLOCK(k_bin, sh, s, n, scenes, sv, g, t);
+}
+
+hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv, GridView g,
+                               TileView t)
+{
+    VSIM_DISPATCH_BLOCK(k_sort, sh, s, scenes, sv, g, t);
+}
+
+hipError_t launch_contact_collide(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
+                                  const vs_scene_params *scenes, SurfaceView sv, GridView g, TileView t, PairLog log)
+{
+    VSIM_DISPATCH_BLOCK(k_collide, sh, s, n, bodies, scenes, sv, g, t, log);
+}
+
+hipError_t launch_contact_settle(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,
+                                 GridView g, TileView t)
+{
+    VSIM_DISPATCH_BLOCK(k_settle, sh, s, n, scenes, sv, g, t);
+}
+
+hipError_t launch_drive(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, DriveView d, TileView t,
+                        int law)
+{
+    VSIM_DISPATCH_BLOCK(k_drive, sh, s, n, scenes, d, t, law);
+}
+
+hipError_t launch_splats(hipStream_t s, Shape sh, const float *px, const float *py, const float *pz,
+                         const vs_body *bodies, SplatView sp, TileView t)
+{
+    VSIM_DISPATCH_BLOCK(k_splats, sh, s, px, py, pz, bodies, sp, t);
+}
+
+}  // namespace vsim

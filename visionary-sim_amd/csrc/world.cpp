@@ -1,0 +1,1099 @@
+// world.cpp -- layer 2 of vsim.h: many scenes resident on one device, stepped
+// as one batch per pass.
+//
+// Device memory is a set of pools sized once from the HBM budget (node planes,
+// cells, bodies, scene uniforms, contact records and buckets, driven records,
+// splats). A scene owns one range in each pool. Tile tables list the work of
+// every admitted scene per pass; a pass is one launch over its table whatever
+// the number of scenes, and a frame is captured once into a hipGraph and
+// replayed. The plain path launches the same kernels over one scene's slice of
+// each table, the way the reference dispatches one scene (stepper.js:101-108).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace {
+
+using vsim::GridView;
+using vsim::NodeView;
+using vsim::PairLog;
+using vsim::Shape;
+using vsim::SplatView;
+using vsim::SurfaceView;
+using vsim::TileView;
+
+constexpr uint32_t BUCKET_SLOTS = 32;  // contacts.wgsl:75, bodyContact.js:35
+constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, NPASS = 12;
+constexpr int NODE_PLANES = 13;        // p xyz, prev xyz, vel xyz, w, rest xyz
+constexpr int SURF_PLANES = 9;
+constexpr int DRIVE_PLANES = 4;
+constexpr uint32_t EVENT_POOL = 8192;
+
+// First-fit range allocator with coalescing.
+struct Pool {
+    uint64_t cap = 0, used = 0;
+    std::map<uint64_t, uint64_t> free_;
+    void init(uint64_t c)
+    {
+        cap = c;
+        used = 0;
+        free_.clear();
+        if (c) free_[0] = c;
+    }
+    bool alloc(uint64_t n, uint64_t &out)
+    {
+        out = 0;
+        if (n == 0) return true;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            out = it->first;
+            const uint64_t rest = it->second - n, next = it->first + n;
+            free_.erase(it);
+            if (rest) free_[next] = rest;
+            used += n;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t b, uint64_t n)
+    {
+        if (!n) return;
+        used -= n;
+        auto next = free_.lower_bound(b);
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == b) {
+                b = prev->first;
+                n += prev->second;
+                free_.erase(prev);
+            }
+        }
+        if (next != free_.end() && b + n == next->first) {
+            n += next->second;
+            free_.erase(next);
+        }
+        free_[b] = n;
+    }
+};
+
+struct BodyRec {
+    uint64_t node_begin = 0, node_count = 0;
+    uint64_t color_begin[8] = {}, color_count[8] = {};
+    uint64_t splat_begin = 0, splat_count = 0;
+    float damping = 0;
+};
+
+struct SceneRec {
+    bool live = false;
+    vs_scene_params params{};
+    uint64_t body_begin = 0, body_count = 0;
+    uint64_t node_begin = 0, node_count = 0;
+    uint64_t cell_begin = 0, cell_count = 0;
+    uint64_t surf_begin = 0, surf_count = 0;
+    uint64_t bucket_begin = 0, bucket_count = 0;
+    uint64_t drive_begin = 0, drive_count = 0;
+    uint64_t splat_begin = 0, splat_count = 0;
+    bool contact = false, stretch = false;
+    std::vector<BodyRec> bodies;
+};
+
+template <typename T>
+struct HostPinned {
+    T *p = nullptr;
+    size_t n = 0;
+    ~HostPinned()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+}  // namespace
+
+struct vs_world {
+    vs_world_config cfg{};
+    std::string error;
+    hipStream_t stream = nullptr, upload = nullptr;
+    hipEvent_t upload_done = nullptr, tables_done = nullptr, params_done = nullptr;
+    std::vector<hipEvent_t> events;
+    hipEvent_t tune[2] = {};
+    uint32_t next_event = 0;
+
+    Pool nodes, cells, bodies, scenes, surface, buckets, driven, splats;
+    float *d_nodes = nullptr;
+    uint32_t *d_cells = nullptr;
+    vs_body *d_bodies = nullptr;
+    vs_scene_params *d_scenes = nullptr;
+    uint32_t *d_surf = nullptr;
+    uint32_t *d_grid = nullptr;
+    uint32_t *d_drive = nullptr;
+    float *d_splat_in = nullptr, *d_splat_out = nullptr;
+    int32_t *d_pairlog = nullptr;
+    uint32_t pair_cap = 0;
+
+    // Tile tables: per pass a 16-byte header then tiles; per-scene counts for
+    // the plain path.
+    uint32_t *d_tiles[NPASS] = {};
+    uint64_t tile_cap[NPASS] = {};
+    uint32_t *d_scene_counts = nullptr;  // [NPASS][max_scenes]
+    HostPinned<uint32_t> h_tiles[NPASS];
+    HostPinned<uint32_t> h_scene_counts;
+    std::vector<uint32_t> tile_count;                 // [NPASS]
+    std::vector<std::vector<uint32_t>> scene_first;   // [NPASS][max_scenes]
+    std::vector<std::vector<uint32_t>> scene_ntiles;  // [NPASS][max_scenes]
+    HostPinned<vs_scene_params> h_scenes;
+    std::vector<vs_body> h_bodies;
+
+    std::vector<SceneRec> recs;
+    bool tables_dirty = true, params_dirty = true, any_stretch = false, any_contact = false;
+    std::vector<std::vector<char>> pending;  // host copies kept alive until uploaded
+
+    // Captured frame.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint32_t> graph_key;
+    uint32_t captures = 0;
+
+    uint32_t block_node = 256, block_cells = 256, block_surf = 256, block_drive = 64, block_splat = 256;
+    uint32_t tile_nodes = 256, tile_cells = 256, tile_surf = 256, tile_drive = 64, tile_splat = 256;
+};
+
+namespace {
+
+int fail(vs_world *w, int code, const std::string &m)
+{
+    if (w) w->error = m;
+    return code;
+}
+
+#define HIPW(w, call)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) return fail((w), VS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+float *plane(vs_world *w, int k) { return w->d_nodes + (size_t)k * w->cfg.max_nodes; }
+
+NodeView node_view(vs_world *w)
+{
+    return NodeView{ plane(w, 0), plane(w, 1), plane(w, 2), plane(w, 3), plane(w, 4),
+                     plane(w, 5), plane(w, 6), plane(w, 7), plane(w, 8), plane(w, 9) };
+}
+
+SurfaceView surf_view(vs_world *w)
+{
+    const size_t S = w->cfg.max_surface;
+    uint32_t *s = w->d_surf;
+    return SurfaceView{ s,
+                        s + S,
+                        reinterpret_cast<float *>(s + 2 * S),
+                        s + 3 * S,
+                        s + 4 * S,
+                        reinterpret_cast<float *>(s + 5 * S),
+                        reinterpret_cast<float *>(s + 6 * S),
+                        reinterpret_cast<float *>(s + 7 * S),
+                        reinterpret_cast<float *>(s + 8 * S) };
+}
+
+GridView grid_view(vs_world *w) { return GridView{ w->d_grid, w->d_grid + w->cfg.max_buckets }; }
+
+vsim::DriveView drive_view(vs_world *w)
+{
+    const size_t D = w->cfg.max_driven;
+    uint32_t *d = w->d_drive;
+    return vsim::DriveView{ d, reinterpret_cast<float *>(d + D), reinterpret_cast<float *>(d + 2 * D),
+                            reinterpret_cast<float *>(d + 3 * D) };
+}
+
+SplatView splat_view(vs_world *w)
+{
+    const size_t G = w->cfg.max_splats;
+    float *in = w->d_splat_in, *out = w->d_splat_out;
+    return SplatView{ in + 4 * G,  in + 5 * G,  in + 6 * G,  reinterpret_cast<const float4 *>(in),
+                      in + 7 * G,  in + 8 * G,  in + 9 * G,  out, out + G, out + 2 * G, out + 3 * G,
+                      (uint32_t)G };
+}
+
+uint32_t pass_block(const vs_world *w, int p)
+{
+    if (p == P_NODE) return w->block_node;
+    if (p >= P_CELL && p < P_CELL + 8) return w->block_cells;
+    if (p == P_SURF) return w->block_surf;
+    if (p == P_DRIVE) return w->block_drive;
+    return w->block_splat;
+}
+
+uint32_t pass_tile(const vs_world *w, int p)
+{
+    if (p == P_NODE) return w->tile_nodes;
+    if (p >= P_CELL && p < P_CELL + 8) return w->tile_cells;
+    if (p == P_SURF) return w->tile_surf;
+    if (p == P_DRIVE) return w->tile_drive;
+    return w->tile_splat;
+}
+
+// Rebuilds every tile table from the admitted scenes, in scene-id order.
+int rebuild_tables(vs_world *w)
+{
+    HIPW(w, hipEventSynchronize(w->tables_done));
+    const uint32_t S = w->cfg.max_scenes;
+    w->any_stretch = false;
+    w->any_contact = false;
+    for (int p = 0; p < NPASS; p++) {
+        vs_tile *tiles = reinterpret_cast<vs_tile *>(w->h_tiles[p].p + 4);
+        uint64_t n = 0;
+        const uint32_t T = pass_tile(w, p);
+        for (uint32_t s = 0; s < S; s++) {
+            SceneRec &r = w->recs[s];
+            w->scene_first[p][s] = (uint32_t)n;
+            const uint64_t start = n;
+            if (r.live) {
+                auto emit = [&](uint64_t begin, uint64_t count, uint32_t body) {
+                    for (uint64_t o = 0; o < count; o += T) {
+                        if (n >= w->tile_cap[p]) return false;
+                        tiles[n++] = vs_tile{ (uint32_t)(begin + o), (uint32_t)std::min<uint64_t>(T, count - o), body,
+                                              s };
+                    }
+                    return true;
+                };
+                bool ok = true;
+                if (p == P_NODE) {
+                    for (size_t b = 0; b < r.bodies.size(); b++)
+                        ok &= emit(r.bodies[b].node_begin, r.bodies[b].node_count, (uint32_t)(r.body_begin + b));
+                } else if (p < P_CELL + 8) {
+                    const int c = p - P_CELL;
+                    for (size_t b = 0; b < r.bodies.size(); b++)
+                        ok &= emit(r.bodies[b].color_begin[c], r.bodies[b].color_count[c], (uint32_t)(r.body_begin + b));
+                } else if (p == P_SURF) {
+                    if (r.contact) ok &= emit(r.surf_begin, r.surf_count, (uint32_t)r.body_begin);
+                } else if (p == P_DRIVE) {
+                    ok &= emit(r.drive_begin, r.drive_count, (uint32_t)r.body_begin);
+                } else {
+                    for (size_t b = 0; b < r.bodies.size(); b++)
+                        ok &= emit(r.bodies[b].splat_begin, r.bodies[b].splat_count, (uint32_t)(r.body_begin + b));
+                }
+                if (!ok) return fail(w, VS_E_NOMEM, "tile table overflow");
+                w->any_stretch |= r.stretch;
+                w->any_contact |= r.contact;
+            }
+            w->scene_ntiles[p][s] = (uint32_t)(n - start);
+            w->h_scene_counts.p[(size_t)p * S + s] = (uint32_t)(n - start);
+        }
+        w->tile_count[p] = (uint32_t)n;
+        w->h_tiles[p].p[0] = (uint32_t)n;
+        w->h_tiles[p].p[1] = w->h_tiles[p].p[2] = w->h_tiles[p].p[3] = 0;
+        HIPW(w, hipMemcpyAsync(w->d_tiles[p], w->h_tiles[p].p, 16 + n * sizeof(vs_tile), hipMemcpyHostToDevice,
+                               w->stream));
+    }
+    HIPW(w, hipMemcpyAsync(w->d_scene_counts, w->h_scene_counts.p, sizeof(uint32_t) * NPASS * S,
+                           hipMemcpyHostToDevice, w->stream));
+    HIPW(w, hipEventRecord(w->tables_done, w->stream));
+    w->tables_dirty = false;
+    return VS_OK;
+}
+
+int upload_params(vs_world *w)
+{
+    HIPW(w, hipEventSynchronize(w->params_done));
+    for (uint32_t s = 0; s < w->cfg.max_scenes; s++) {
+        if (w->recs[s].live) w->h_scenes.p[s] = w->recs[s].params;
+    }
+    HIPW(w, hipMemcpyAsync(w->d_scenes, w->h_scenes.p, sizeof(vs_scene_params) * w->cfg.max_scenes,
+                           hipMemcpyHostToDevice, w->stream));
+    HIPW(w, hipEventRecord(w->params_done, w->stream));
+    w->params_dirty = false;
+    return VS_OK;
+}
+
+int prepare(vs_world *w)
+{
+    // Work admitted on the upload stream must land before the next frame.
+    HIPW(w, hipStreamWaitEvent(w->stream, w->upload_done, 0));
+    if (w->tables_dirty) {
+        if (int rc = rebuild_tables(w)) return rc;
+    }
+    if (w->params_dirty) {
+        if (int rc = upload_params(w)) return rc;
+    }
+    return VS_OK;
+}
+
+// A tile view over the whole table (scene < 0) or one scene's slice of it.
+TileView tile_view(vs_world *w, int p, int scene)
+{
+    const vs_tile *tiles = reinterpret_cast<const vs_tile *>(w->d_tiles[p] + 4);
+    if (scene < 0) return TileView{ w->d_tiles[p], tiles };
+    return TileView{ w->d_scene_counts + (size_t)p * w->cfg.max_scenes + scene, tiles + w->scene_first[p][scene] };
+}
+
+uint32_t grid_for(vs_world *w, int p, int scene)
+{
+    const uint32_t n = scene < 0 ? w->tile_count[p] : w->scene_ntiles[p][scene];
+    if (n == 0) return 0;
+    if (scene < 0 && w->cfg.persistent && p >= P_CELL && p < P_CELL + 8) return std::min(n, w->cfg.persistent);
+    return n;
+}
+
+// Events around launches, for vs_world_profile.
+struct Profile {
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, size_t>> marks;  // kind, index of the start event
+    size_t used = 0;
+    hipEvent_t next()
+    {
+        if (used == pool.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    ~Profile()
+    {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// Enqueues one frame: `substeps` substeps in Stepper.sequence order
+// (stepper.js:101-108, solve order simulator.js:83), then the splats.
+int enqueue_frame(vs_world *w, uint32_t substeps, bool splats, int scene, Profile *prof = nullptr)
+{
+    using namespace vsim;
+    NodeView n = node_view(w);
+    SurfaceView sv = surf_view(w);
+    GridView g = grid_view(w);
+    PairLog log{ w->d_pairlog, w->d_pairlog ? reinterpret_cast<uint32_t *>(w->d_pairlog + w->cfg.max_surface) : nullptr,
+                 w->pair_cap };
+    bool stretch = w->any_stretch, contact = w->any_contact;
+    if (scene >= 0) {
+        stretch = w->recs[scene].stretch;
+        contact = w->recs[scene].contact;
+    }
+    auto shape = [&](int p) { return Shape{ grid_for(w, p, scene), pass_block(w, p), scene < 0 ? w->cfg.xcd_chunked : 0u }; };
+    hipStream_t st = w->stream;
+#define TIMED(KIND, CALL)                                                   \
+    do {                                                                    \
+        if (prof) {                                                         \
+            prof->marks.emplace_back((KIND), prof->used);                   \
+            HIPW(w, hipEventRecord(prof->next(), st));                      \
+        }                                                                   \
+        HIPW(w, CALL);                                                      \
+        if (prof) HIPW(w, hipEventRecord(prof->next(), st));                \
+    } while (0)
+    for (uint32_t s = 0; s < substeps; s++) {
+        Shape sh = shape(P_NODE);
+        if (sh.grid) TIMED(VS_P_PREDICT, launch_predict(st, sh, n, w->d_scenes, tile_view(w, P_NODE, scene)));
+        sh = shape(P_DRIVE);
+        if (sh.grid)
+            TIMED(VS_P_DRIVE, launch_drive(st, sh, n, w->d_scenes, drive_view(w), tile_view(w, P_DRIVE, scene), -1));
+        for (int sweep = 0; sweep < (stretch ? 4 : 1); sweep++) {
+            for (int c = 0; c < 8; c++) {
+                sh = shape(P_CELL + c);
+                if (sh.grid)
+                    TIMED(VS_P_ELASTIC, launch_elastic(st, sh, n, w->d_bodies, w->d_scenes, w->d_cells,
+                                                       tile_view(w, P_CELL + c, scene), sweep > 0 ? 1u : 0u));
+            }
+        }
+        sh = shape(P_SURF);
+        if (contact && sh.grid) {
+            TileView t = tile_view(w, P_SURF, scene);
+            TIMED(VS_P_BIN, launch_contact_bin(st, sh, n, w->d_scenes, sv, g, t));
+            TIMED(VS_P_SORT, launch_contact_sort(st, sh, w->d_scenes, sv, g, t));
+            TIMED(VS_P_COLLIDE, launch_contact_collide(st, sh, n, w->d_bodies, w->d_scenes, sv, g, t, log));
+            TIMED(VS_P_SETTLE, launch_contact_settle(st, sh, n, w->d_scenes, sv, g, t));
+        }
+        sh = shape(P_NODE);
+        if (sh.grid)
+            TIMED(VS_P_FINALIZE, launch_finalize(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_NODE, scene)));
+    }
+    if (splats) {
+        Shape sh = shape(P_SPLAT);
+        if (sh.grid)
+            TIMED(VS_P_SPLAT, launch_splats(st, sh, n.px, n.py, n.pz, w->d_bodies, splat_view(w),
+                                            tile_view(w, P_SPLAT, scene)));
+    }
+#undef TIMED
+    return VS_OK;
+}
+
+void release_ranges(vs_world *w, const SceneRec &r)
+{
+    w->bodies.release(r.body_begin, r.body_count);
+    w->nodes.release(r.node_begin, r.node_count);
+    w->cells.release(r.cell_begin, r.cell_count);
+    w->surface.release(r.surf_begin, r.surf_count);
+    w->buckets.release(r.bucket_begin, r.bucket_count);
+    w->driven.release(r.drive_begin, r.drive_count);
+    w->splats.release(r.splat_begin, r.splat_count);
+}
+
+template <typename T>
+const T *keep(vs_world *w, std::vector<T> &&v)
+{
+    std::vector<char> bytes(v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(bytes.data(), v.data(), bytes.size());
+    w->pending.push_back(std::move(bytes));
+    return reinterpret_cast<const T *>(w->pending.back().data());
+}
+
+float damp_factor(float damping, float dt)
+{
+    // exp(-damping * dt) of finalize.wgsl:72: the f32 product of the two
+    // uniforms, exponentiated in double and rounded once.
+    const float a = -(damping * dt);
+    return (float)std::exp((double)a);
+}
+
+}  // namespace
+
+extern "C" {
+
+static int world_init(vs_world *W);
+
+int vs_world_create(const vs_world_config *cfg, vs_world **out)
+{
+    if (!cfg || !out) return VS_E_INVALID;
+    *out = nullptr;
+    vs_world *w = new vs_world();
+    w->cfg = *cfg;
+    const int rc = world_init(w);
+    if (rc) {
+        vs_world_destroy(w);
+        return rc;
+    }
+    *out = w;
+    return VS_OK;
+}
+
+static int world_init(vs_world *w)
+{
+    vs_world_config &c = w->cfg;
+    if (c.max_scenes == 0 || c.max_bodies == 0 || c.max_nodes == 0) return VS_E_INVALID;
+    if (c.max_cells == 0) c.max_cells = 1;
+    if (c.max_surface == 0) c.max_surface = 1;
+    if (c.max_buckets == 0) c.max_buckets = 1;
+    if (c.max_driven == 0) c.max_driven = 1;
+    if (c.max_splats == 0) c.max_splats = 1;
+    if (c.tile_nodes) w->tile_nodes = c.tile_nodes;
+    if (c.tile_cells) w->tile_cells = c.tile_cells;
+    if (c.block_cells) w->block_cells = c.block_cells;
+    if (w->block_cells != 64 && w->block_cells != 128 && w->block_cells != 256 && w->block_cells != 512) return VS_E_INVALID;
+    hipError_t e = hipSetDevice(c.device);
+    if (e != hipSuccess) return VS_E_DEVICE;
+    vs_world *W = w;
+#define ALLOC(ptr, bytes) HIPW(W, hipMalloc(reinterpret_cast<void **>(&(ptr)), (bytes)))
+    ALLOC(W->d_nodes, sizeof(float) * NODE_PLANES * c.max_nodes);
+    ALLOC(W->d_cells, sizeof(uint32_t) * c.max_cells);
+    ALLOC(W->d_bodies, sizeof(vs_body) * c.max_bodies);
+    ALLOC(W->d_scenes, sizeof(vs_scene_params) * c.max_scenes);
+    ALLOC(W->d_surf, sizeof(uint32_t) * SURF_PLANES * c.max_surface);
+    ALLOC(W->d_grid, sizeof(uint32_t) * (1 + BUCKET_SLOTS) * c.max_buckets);
+    ALLOC(W->d_drive, sizeof(uint32_t) * DRIVE_PLANES * c.max_driven);
+    ALLOC(W->d_splat_in, sizeof(float) * 10 * c.max_splats);
+    ALLOC(W->d_splat_out, sizeof(float) * 9 * c.max_splats);
+    ALLOC(W->d_scene_counts, sizeof(uint32_t) * NPASS * c.max_scenes);
+    HIPW(W, hipMemset(W->d_nodes, 0, sizeof(float) * NODE_PLANES * c.max_nodes));
+    HIPW(W, hipMemset(W->d_grid, 0, sizeof(uint32_t) * c.max_buckets));
+    HIPW(W, hipMemset(W->d_scenes, 0, sizeof(vs_scene_params) * c.max_scenes));
+    W->nodes.init(c.max_nodes);
+    W->cells.init(c.max_cells);
+    W->bodies.init(c.max_bodies);
+    W->scenes.init(c.max_scenes);
+    W->surface.init(c.max_surface);
+    W->buckets.init(c.max_buckets);
+    W->driven.init(c.max_driven);
+    W->splats.init(c.max_splats);
+    // A table needs a tile per `tile` items plus one partial tile per body.
+    W->tile_cap[P_NODE] = c.max_nodes / W->tile_nodes + c.max_bodies + 1;
+    W->tile_cap[P_SURF] = c.max_surface / W->tile_surf + c.max_scenes + 1;
+    W->tile_cap[P_DRIVE] = c.max_driven / W->tile_drive + c.max_scenes + 1;
+    W->tile_cap[P_SPLAT] = c.max_splats / W->tile_splat + c.max_bodies + 1;
+    // Colour populations are uneven for small or odd lattices: size every
+    // colour table for all cells (tiles are 16 bytes, cheap to over-provision).
+    for (int k = 0; k < 8; k++) W->tile_cap[P_CELL + k] = c.max_cells / 64 + c.max_bodies + 64;
+    for (int p = 0; p < NPASS; p++) {
+        const size_t bytes = 16 + W->tile_cap[p] * sizeof(vs_tile);
+        ALLOC(W->d_tiles[p], bytes);
+        HIPW(W, hipHostMalloc(reinterpret_cast<void **>(&W->h_tiles[p].p), bytes, hipHostMallocDefault));
+        std::memset(W->h_tiles[p].p, 0, 16);
+        HIPW(W, hipMemset(W->d_tiles[p], 0, 16));
+    }
+    HIPW(W, hipHostMalloc(reinterpret_cast<void **>(&W->h_scene_counts.p), sizeof(uint32_t) * NPASS * c.max_scenes,
+                          hipHostMallocDefault));
+    HIPW(W, hipHostMalloc(reinterpret_cast<void **>(&W->h_scenes.p), sizeof(vs_scene_params) * c.max_scenes,
+                          hipHostMallocDefault));
+    std::memset(W->h_scenes.p, 0, sizeof(vs_scene_params) * c.max_scenes);
+#undef ALLOC
+    W->tile_count.assign(NPASS, 0);
+    W->scene_first.assign(NPASS, std::vector<uint32_t>(c.max_scenes, 0));
+    W->scene_ntiles.assign(NPASS, std::vector<uint32_t>(c.max_scenes, 0));
+    W->recs.resize(c.max_scenes);
+    W->h_bodies.resize(c.max_bodies);
+    HIPW(W, hipStreamCreateWithFlags(&W->stream, hipStreamNonBlocking));
+    HIPW(W, hipStreamCreateWithFlags(&W->upload, hipStreamNonBlocking));
+    HIPW(W, hipEventCreateWithFlags(&W->upload_done, hipEventDisableTiming));
+    HIPW(W, hipEventCreateWithFlags(&W->tables_done, hipEventDisableTiming));
+    HIPW(W, hipEventCreateWithFlags(&W->params_done, hipEventDisableTiming));
+    HIPW(W, hipEventRecord(W->upload_done, W->upload));
+    HIPW(W, hipEventRecord(W->tables_done, W->stream));
+    HIPW(W, hipEventRecord(W->params_done, W->stream));
+    W->events.resize(EVENT_POOL);
+    for (auto &ev : W->events) HIPW(W, hipEventCreate(&ev));
+    for (auto &ev : W->tune) HIPW(W, hipEventCreate(&ev));
+    HIPW(W, hipDeviceSynchronize());
+    return VS_OK;
+}
+
+void vs_world_destroy(vs_world *w)
+{
+    if (!w) return;
+    (void)hipSetDevice(w->cfg.device);
+    if (w->stream) (void)hipStreamSynchronize(w->stream);
+    if (w->upload) (void)hipStreamSynchronize(w->upload);
+    if (w->exec) (void)hipGraphExecDestroy(w->exec);
+    if (w->graph) (void)hipGraphDestroy(w->graph);
+    for (auto &ev : w->events)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : { w->upload_done, w->tables_done, w->params_done, w->tune[0], w->tune[1] })
+        if (ev) (void)hipEventDestroy(ev);
+    void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf,
+                     w->d_grid,  w->d_drive, w->d_splat_in, w->d_splat_out, w->d_pairlog, w->d_scene_counts };
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int p = 0; p < NPASS; p++)
+        if (w->d_tiles[p]) (void)hipFree(w->d_tiles[p]);
+    if (w->stream) (void)hipStreamDestroy(w->stream);
+    if (w->upload) (void)hipStreamDestroy(w->upload);
+    delete w;
+}
+
+const char *vs_world_error(const vs_world *w) { return w ? w->error.c_str() : "no world"; }
+
+int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
+{
+    if (!w || !d || !scene_id || (d->n_bodies && !d->bodies)) return VS_E_INVALID;
+    if (d->n_bodies == 0) return fail(w, VS_E_INVALID, "a scene needs at least one body");
+    if (!(d->dt > 0.0f)) return fail(w, VS_E_INVALID, "stepDuration must be a finite positive number");
+    // Sizes.
+    SceneRec r;
+    uint64_t nodes = 0, ncells = 0, nsurf = 0, nsplats = 0;
+    for (uint32_t b = 0; b < d->n_bodies; b++) {
+        const vs_body_desc &bd = d->bodies[b];
+        if (!bd.rest || !bd.inv_mass || (bd.n_cells && !bd.cells)) return fail(w, VS_E_INVALID, "body arrays missing");
+        uint64_t colored = 0;
+        for (int c = 0; c < 8; c++) colored += bd.color_count[c];
+        if (colored != bd.n_cells) return fail(w, VS_E_INVALID, "colour counts do not sum to the cell count");
+        if (!(bd.h > 0.0f)) return fail(w, VS_E_INVALID, "cell size must be positive");
+        const uint64_t cells_total = (uint64_t)bd.cell_dims[0] * bd.cell_dims[1] * bd.cell_dims[2];
+        for (uint32_t i = 0; i < bd.n_cells; i++) {
+            if (bd.cells[i] >= cells_total) return fail(w, VS_E_INVALID, "cell index outside the lattice");
+        }
+        const uint64_t nn = (uint64_t)(bd.cell_dims[0] + 1) * (bd.cell_dims[1] + 1) * (bd.cell_dims[2] + 1);
+        for (uint32_t i = 0; i < bd.n_surface; i++) {
+            if (bd.surface[i] >= nn) return fail(w, VS_E_INVALID, "surface node outside the lattice");
+        }
+        nodes += nn;
+        ncells += bd.n_cells;
+        if (d->contact) nsurf += bd.n_surface;
+        nsplats += bd.n_splats;
+        r.stretch |= bd.max_stretch > 1.0f;
+    }
+    for (uint32_t i = 0; i < d->n_driven; i++) {
+        if (d->driven[i] >= nodes) return fail(w, VS_E_INVALID, "driven node outside the scene");
+    }
+    r.contact = d->contact == 2 || (d->contact && d->n_bodies >= 2);
+    // Contact grid (broadphase.js:46-58): one bucket per contact diameter
+    // over the tank, a ring of margin buckets, origin one bucket below min.
+    double edge = d->bucket_size;
+    uint32_t dims[3] = { 1, 1, 1 };
+    double origin[3] = { 0, 0, 0 };
+    uint64_t nbuckets = 0;
+    if (r.contact && d->grid_dims[0] && d->grid_dims[1] && d->grid_dims[2]) {
+        for (int a = 0; a < 3; a++) {
+            dims[a] = d->grid_dims[a];
+            origin[a] = d->grid_origin[a];
+        }
+        nbuckets = (uint64_t)dims[0] * dims[1] * dims[2];
+    } else if (r.contact) {
+        if (!(edge > 0)) {
+            edge = 0;
+            for (int a = 0; a < 3; a++) edge = std::max(edge, (double)d->tank_max[a] - (double)d->tank_min[a]);
+        }
+        for (int a = 0; a < 3; a++) {
+            const double extent = (double)d->tank_max[a] - (double)d->tank_min[a];
+            dims[a] = (uint32_t)std::max(1.0, std::ceil(extent / edge)) + 2u;
+            origin[a] = (double)d->tank_min[a] - edge;
+        }
+        nbuckets = (uint64_t)dims[0] * dims[1] * dims[2];
+    }
+    // Ranges.
+    uint32_t id = 0;
+    for (; id < w->cfg.max_scenes && w->recs[id].live; id++) {
+    }
+    if (id == w->cfg.max_scenes) return fail(w, VS_E_NOMEM, "no free scene slot");
+    const bool ok = w->bodies.alloc(d->n_bodies, r.body_begin) && (r.body_count = d->n_bodies, true) &&
+                    w->nodes.alloc(nodes, r.node_begin) && (r.node_count = nodes, true) &&
+                    w->cells.alloc(ncells, r.cell_begin) && (r.cell_count = ncells, true) &&
+                    w->surface.alloc(nsurf, r.surf_begin) && (r.surf_count = nsurf, true) &&
+                    w->buckets.alloc(nbuckets, r.bucket_begin) && (r.bucket_count = nbuckets, true) &&
+                    w->driven.alloc(d->n_driven, r.drive_begin) && (r.drive_count = d->n_driven, true) &&
+                    w->splats.alloc(nsplats, r.splat_begin) && (r.splat_count = nsplats, true);
+    if (!ok) {
+        release_ranges(w, r);
+        return fail(w, VS_E_NOMEM, "scene does not fit the HBM budget");
+    }
+    // Retire host copies whose uploads have landed.
+    if (hipEventQuery(w->upload_done) == hipSuccess) w->pending.clear();
+
+    vs_scene_params &P = r.params;
+    P = vs_scene_params{};
+    P.dt = d->dt;
+    for (int a = 0; a < 3; a++) {
+        P.gravity[a] = d->gravity[a];
+        P.tank_min[a] = d->tank_min[a];
+        P.tank_max[a] = d->tank_max[a];
+        P.grid_origin[a] = (float)origin[a];
+        P.grid_dims[a] = dims[a];
+    }
+    P.gain = 1.0f;  // GAIN, bodyContact.js:48
+    P.bucket_size = (float)edge;
+    P.bucket_count = (uint32_t)nbuckets;
+    P.grid_base = (uint32_t)r.bucket_begin;
+    P.response_time = d->response_time > 0 ? d->response_time : 0.05f;  // RESPONSE_TIME, kinematicDrive.js:65
+    P.drive_law = d->drive_law;
+    P.surf_base = (uint32_t)r.surf_begin;
+    P.surf_count = (uint32_t)nsurf;
+    P.bucket_slots = BUCKET_SLOTS;
+
+    hipStream_t up = w->upload;
+    const size_t N = w->cfg.max_nodes;
+    uint64_t node_at = r.node_begin, cell_at = r.cell_begin, splat_at = r.splat_begin;
+    std::vector<uint32_t> s_node, s_body;
+    std::vector<float> s_rad;
+    r.bodies.resize(d->n_bodies);
+    for (uint32_t b = 0; b < d->n_bodies; b++) {
+        const vs_body_desc &bd = d->bodies[b];
+        BodyRec &br = r.bodies[b];
+        const uint32_t nd[3] = { bd.cell_dims[0] + 1, bd.cell_dims[1] + 1, bd.cell_dims[2] + 1 };
+        const uint64_t nn = (uint64_t)nd[0] * nd[1] * nd[2];
+        br.node_begin = node_at;
+        br.node_count = nn;
+        br.damping = bd.damping;
+        vs_body &vb = w->h_bodies[r.body_begin + b];
+        vb = vs_body{};
+        for (int a = 0; a < 3; a++) {
+            vb.cell_dims[a] = bd.cell_dims[a];
+            vb.node_dims[a] = nd[a];
+            vb.origin[a] = bd.origin[a];
+        }
+        vb.node_offset = (uint32_t)node_at;
+        vb.node_count = (uint32_t)nn;
+        vb.scene = id;
+        vb.h = bd.h;
+        vb.alpha_dev = bd.alpha_dev;
+        vb.alpha_hyd = bd.alpha_hyd;
+        vb.gamma = bd.gamma;
+        vb.relaxation = bd.relaxation;
+        vb.max_stretch = bd.max_stretch;
+        vb.friction = bd.friction;
+        vb.damp = damp_factor(bd.damping, d->dt);
+        // Node planes: p = prev = rest, vel = 0, w, rest.
+        std::vector<float> xyz(3 * nn);
+        for (uint64_t i = 0; i < nn; i++) {
+            xyz[i] = bd.rest[3 * i];
+            xyz[nn + i] = bd.rest[3 * i + 1];
+            xyz[2 * nn + i] = bd.rest[3 * i + 2];
+        }
+        const float *hx = keep(w, std::move(xyz));
+        for (int k : { 0, 3, 10 }) {
+            for (int a = 0; a < 3; a++)
+                HIPW(w, hipMemcpyAsync(w->d_nodes + (k + a) * N + node_at, hx + a * nn, nn * 4, hipMemcpyHostToDevice, up));
+        }
+        for (int a = 0; a < 3; a++) {
+            if (bd.velocity[a] == 0.0f) {
+                HIPW(w, hipMemsetAsync(w->d_nodes + (6 + a) * N + node_at, 0, nn * 4, up));
+            } else {
+                const float *hv = keep(w, std::vector<float>(nn, bd.velocity[a]));
+                HIPW(w, hipMemcpyAsync(w->d_nodes + (6 + a) * N + node_at, hv, nn * 4, hipMemcpyHostToDevice, up));
+            }
+        }
+        const float *hw = keep(w, std::vector<float>(bd.inv_mass, bd.inv_mass + nn));
+        HIPW(w, hipMemcpyAsync(w->d_nodes + 9 * N + node_at, hw, nn * 4, hipMemcpyHostToDevice, up));
+        // Cells, colour by colour.
+        uint64_t src = 0;
+        for (int c = 0; c < 8; c++) {
+            br.color_begin[c] = cell_at + src;
+            br.color_count[c] = bd.color_count[c];
+            src += bd.color_count[c];
+        }
+        if (bd.n_cells) {
+            const uint32_t *hc = keep(w, std::vector<uint32_t>(bd.cells, bd.cells + bd.n_cells));
+            HIPW(w, hipMemcpyAsync(w->d_cells + cell_at, hc, (size_t)bd.n_cells * 4, hipMemcpyHostToDevice, up));
+        }
+        cell_at += bd.n_cells;
+        // Surface records, body by body (bodyContact.js:72-85).
+        if (d->contact) {
+            for (uint32_t i = 0; i < bd.n_surface; i++) {
+                s_node.push_back((uint32_t)(node_at + bd.surface[i]));
+                s_body.push_back((uint32_t)(r.body_begin + b));
+                s_rad.push_back(bd.contact_radius);
+            }
+
+        }
+        // Splats.
+        br.splat_begin = splat_at;
+        br.splat_count = bd.n_splats;
+        if (bd.n_splats) {
+            const size_t G = w->cfg.max_splats;
+            const uint32_t ns = bd.n_splats;
+            std::vector<float> planes(6 * (size_t)ns);
+            for (uint32_t i = 0; i < ns; i++) {
+                for (int a = 0; a < 3; a++) {
+                    planes[a * ns + i] = bd.splat_mean[3 * i + a];
+                    planes[(3 + a) * ns + i] = bd.splat_scale[3 * i + a];
+                }
+            }
+            const float *hp = keep(w, std::move(planes));
+            const float *hq = keep(w, std::vector<float>(bd.splat_quat, bd.splat_quat + 4 * (size_t)ns));
+            HIPW(w, hipMemcpyAsync(w->d_splat_in + 4 * splat_at, hq, (size_t)ns * 16, hipMemcpyHostToDevice, up));
+            for (int k = 0; k < 6; k++)
+                HIPW(w, hipMemcpyAsync(w->d_splat_in + (4 + k) * G + splat_at, hp + (size_t)k * ns, (size_t)ns * 4,
+                                       hipMemcpyHostToDevice, up));
+        }
+        splat_at += bd.n_splats;
+        node_at += nn;
+    }
+    const vs_body *hb_rec = keep(w, std::vector<vs_body>(w->h_bodies.begin() + r.body_begin,
+                                                         w->h_bodies.begin() + r.body_begin + d->n_bodies));
+    HIPW(w, hipMemcpyAsync(w->d_bodies + r.body_begin, hb_rec, sizeof(vs_body) * d->n_bodies, hipMemcpyHostToDevice, up));
+    if (nsurf) {
+        const size_t S = w->cfg.max_surface;
+        const uint32_t *hn = keep(w, std::move(s_node));
+        const uint32_t *hb = keep(w, std::move(s_body));
+        const float *hr = keep(w, std::move(s_rad));
+        HIPW(w, hipMemcpyAsync(w->d_surf + r.surf_begin, hn, nsurf * 4, hipMemcpyHostToDevice, up));
+        HIPW(w, hipMemcpyAsync(w->d_surf + S + r.surf_begin, hb, nsurf * 4, hipMemcpyHostToDevice, up));
+        HIPW(w, hipMemcpyAsync(w->d_surf + 2 * S + r.surf_begin, hr, nsurf * 4, hipMemcpyHostToDevice, up));
+    }
+    if (nbuckets) HIPW(w, hipMemsetAsync(w->d_grid + r.bucket_begin, 0, nbuckets * 4, up));
+    if (d->n_driven) {
+        const size_t D = w->cfg.max_driven;
+        std::vector<uint32_t> dn(d->n_driven);
+        std::vector<float> off(3 * (size_t)d->n_driven);
+        for (uint32_t i = 0; i < d->n_driven; i++) {
+            dn[i] = (uint32_t)(r.node_begin + d->driven[i]);
+            for (int a = 0; a < 3; a++) off[a * d->n_driven + i] = d->driven_offset[3 * i + a];
+        }
+        const uint32_t *hd = keep(w, std::move(dn));
+        const float *ho = keep(w, std::move(off));
+        HIPW(w, hipMemcpyAsync(w->d_drive + r.drive_begin, hd, (size_t)d->n_driven * 4, hipMemcpyHostToDevice, up));
+        for (int a = 0; a < 3; a++)
+            HIPW(w, hipMemcpyAsync(w->d_drive + (1 + a) * D + r.drive_begin, ho + (size_t)a * d->n_driven,
+                                   (size_t)d->n_driven * 4, hipMemcpyHostToDevice, up));
+    }
+    HIPW(w, hipEventRecord(w->upload_done, up));
+    r.live = true;
+    w->recs[id] = std::move(r);
+    w->tables_dirty = true;
+    w->params_dirty = true;
+    *scene_id = id;
+    return VS_OK;
+}
+
+int vs_world_remove_scene(vs_world *w, uint32_t id)
+{
+    if (!w || id >= w->cfg.max_scenes || !w->recs[id].live) return VS_E_INVALID;
+    // Frames already enqueued may still read the scene's ranges, and the next
+    // admission may hand them out again on the upload stream: drain first.
+    HIPW(w, hipStreamSynchronize(w->stream));
+    release_ranges(w, w->recs[id]);
+    w->recs[id] = SceneRec{};
+    w->tables_dirty = true;
+    return VS_OK;
+}
+
+int vs_world_reset_scene(vs_world *w, uint32_t id)
+{
+    if (!w || id >= w->cfg.max_scenes || !w->recs[id].live) return VS_E_INVALID;
+    const SceneRec &r = w->recs[id];
+    const size_t N = w->cfg.max_nodes;
+    HIPW(w, hipStreamWaitEvent(w->stream, w->upload_done, 0));
+    for (int a = 0; a < 3; a++) {
+        HIPW(w, hipMemcpyAsync(w->d_nodes + a * N + r.node_begin, w->d_nodes + (10 + a) * N + r.node_begin,
+                               r.node_count * 4, hipMemcpyDeviceToDevice, w->stream));
+        HIPW(w, hipMemcpyAsync(w->d_nodes + (3 + a) * N + r.node_begin, w->d_nodes + (10 + a) * N + r.node_begin,
+                               r.node_count * 4, hipMemcpyDeviceToDevice, w->stream));
+        HIPW(w, hipMemsetAsync(w->d_nodes + (6 + a) * N + r.node_begin, 0, r.node_count * 4, w->stream));
+    }
+    return VS_OK;
+}
+
+int vs_world_set_target(vs_world *w, uint32_t id, const float target[3], const float velocity[3])
+{
+    if (!w || id >= w->cfg.max_scenes || !w->recs[id].live || !target || !velocity) return VS_E_INVALID;
+    for (int a = 0; a < 3; a++) {
+        w->recs[id].params.target[a] = target[a];
+        w->recs[id].params.target_vel[a] = velocity[a];
+    }
+    w->params_dirty = true;
+    return VS_OK;
+}
+
+int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph)
+{
+    if (!w) return VS_E_INVALID;
+    HIPW(w, hipSetDevice(w->cfg.device));
+    if (int rc = prepare(w)) return rc;
+    if (!graph) return enqueue_frame(w, substeps, splats != 0, -1);
+    std::vector<uint32_t> key = { substeps, (uint32_t)(splats != 0), (uint32_t)w->any_stretch, (uint32_t)w->any_contact,
+                                  w->pair_cap, w->block_cells, w->cfg.xcd_chunked };
+    for (int p = 0; p < NPASS; p++) key.push_back(grid_for(w, p, -1));
+    if (!w->exec || key != w->graph_key) {
+        if (w->exec) HIPW(w, hipGraphExecDestroy(w->exec));
+        if (w->graph) HIPW(w, hipGraphDestroy(w->graph));
+        w->exec = nullptr;
+        w->graph = nullptr;
+        HIPW(w, hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_frame(w, substeps, splats != 0, -1);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(w->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(w, VS_E_DEVICE, std::string("frame capture: ") + hipGetErrorString(e));
+        w->graph = g;
+        HIPW(w, hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0));
+        w->graph_key = key;
+        w->captures++;
+    }
+    HIPW(w, hipGraphLaunch(w->exec, w->stream));
+    return VS_OK;
+}
+
+int vs_world_frame_scene(vs_world *w, uint32_t id, uint32_t substeps, int32_t splats)
+{
+    if (!w || id >= w->cfg.max_scenes || !w->recs[id].live) return VS_E_INVALID;
+    HIPW(w, hipSetDevice(w->cfg.device));
+    if (int rc = prepare(w)) return rc;
+    return enqueue_frame(w, substeps, splats != 0, (int)id);
+}
+
+int vs_world_synchronize(vs_world *w)
+{
+    if (!w) return VS_E_INVALID;
+    HIPW(w, hipStreamSynchronize(w->upload));
+    HIPW(w, hipStreamSynchronize(w->stream));
+    w->pending.clear();
+    return VS_OK;
+}
+
+int vs_world_record(vs_world *w, uint32_t *slot)
+{
+    if (!w || !slot) return VS_E_INVALID;
+    const uint32_t k = w->next_event++ % EVENT_POOL;
+    HIPW(w, hipEventRecord(w->events[k], w->stream));
+    *slot = k;
+    return VS_OK;
+}
+
+int vs_world_elapsed(vs_world *w, uint32_t from, uint32_t to, float *ms)
+{
+    if (!w || !ms || from >= EVENT_POOL || to >= EVENT_POOL) return VS_E_INVALID;
+    HIPW(w, hipEventSynchronize(w->events[to]));
+    HIPW(w, hipEventElapsedTime(ms, w->events[from], w->events[to]));
+    return VS_OK;
+}
+
+int vs_world_read_nodes(vs_world *w, uint32_t id, float *pos, float *vel)
+{
+    if (!w || id >= w->cfg.max_scenes || !w->recs[id].live) return VS_E_INVALID;
+    if (int rc = vs_world_synchronize(w)) return rc;
+    const SceneRec &r = w->recs[id];
+    const size_t N = w->cfg.max_nodes;
+    for (int a = 0; a < 3; a++) {
+        if (pos) HIPW(w, hipMemcpy(pos + a * r.node_count, w->d_nodes + a * N + r.node_begin, r.node_count * 4, hipMemcpyDeviceToHost));
+        if (vel) HIPW(w, hipMemcpy(vel + a * r.node_count, w->d_nodes + (6 + a) * N + r.node_begin, r.node_count * 4, hipMemcpyDeviceToHost));
+    }
+    return VS_OK;
+}
+
+int vs_world_read_splats(vs_world *w, uint32_t id, float *mean, float *cov)
+{
+    if (!w || id >= w->cfg.max_scenes || !w->recs[id].live) return VS_E_INVALID;
+    if (int rc = vs_world_synchronize(w)) return rc;
+    const SceneRec &r = w->recs[id];
+    const size_t G = w->cfg.max_splats;
+    for (int k = 0; k < 9; k++) {
+        float *dst = k < 3 ? (mean ? mean + k * r.splat_count : nullptr) : (cov ? cov + (k - 3) * r.splat_count : nullptr);
+        if (dst && r.splat_count)
+            HIPW(w, hipMemcpy(dst, w->d_splat_out + k * G + r.splat_begin, r.splat_count * 4, hipMemcpyDeviceToHost));
+    }
+    return VS_OK;
+}
+
+int vs_world_log_pairs(vs_world *w, uint32_t cap)
+{
+    if (!w) return VS_E_INVALID;
+    HIPW(w, hipStreamSynchronize(w->stream));
+    if (w->d_pairlog) HIPW(w, hipFree(w->d_pairlog));
+    w->d_pairlog = nullptr;
+    w->pair_cap = 0;
+    if (cap) {
+        HIPW(w, hipMalloc(reinterpret_cast<void **>(&w->d_pairlog), sizeof(uint32_t) * w->cfg.max_surface * (1 + (size_t)cap)));
+        HIPW(w, hipMemset(w->d_pairlog, 0, sizeof(uint32_t) * w->cfg.max_surface * (1 + (size_t)cap)));
+        w->pair_cap = cap;
+    }
+    return VS_OK;
+}
+
+int vs_world_read_contacts(vs_world *w, uint32_t id, uint32_t *keys, int32_t *counts, uint32_t *pairs)
+{
+    if (!w || id >= w->cfg.max_scenes || !w->recs[id].live) return VS_E_INVALID;
+    if (int rc = vs_world_synchronize(w)) return rc;
+    const SceneRec &r = w->recs[id];
+    if (!r.surf_count) return VS_OK;
+    const size_t S = w->cfg.max_surface;
+    if (keys) HIPW(w, hipMemcpy(keys, w->d_surf + 3 * S + r.surf_begin, r.surf_count * 4, hipMemcpyDeviceToHost));
+    if (w->d_pairlog) {
+        if (counts) HIPW(w, hipMemcpy(counts, w->d_pairlog + r.surf_begin, r.surf_count * 4, hipMemcpyDeviceToHost));
+        if (pairs)
+            HIPW(w, hipMemcpy(pairs, reinterpret_cast<uint32_t *>(w->d_pairlog + S) + r.surf_begin * w->pair_cap,
+                              r.surf_count * w->pair_cap * 4, hipMemcpyDeviceToHost));
+    }
+    return VS_OK;
+}
+
+int vs_world_stats_get(const vs_world *w, vs_world_stats *out)
+{
+    if (!w || !out) return VS_E_INVALID;
+    *out = vs_world_stats{};
+    for (const SceneRec &r : w->recs) {
+        if (!r.live) continue;
+        out->scenes++;
+        out->bodies += r.body_count;
+        out->nodes += r.node_count;
+        out->cells += r.cell_count;
+        out->surface += r.surf_count;
+        out->buckets += r.bucket_count;
+        out->splats += r.splat_count;
+        out->driven += r.drive_count;
+    }
+    const vs_world_config &c = w->cfg;
+    out->bytes_reserved = 4ull * NODE_PLANES * c.max_nodes + 4ull * c.max_cells + sizeof(vs_body) * c.max_bodies +
+                          sizeof(vs_scene_params) * c.max_scenes + 4ull * SURF_PLANES * c.max_surface +
+                          4ull * (1 + BUCKET_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
+                          76ull * c.max_splats;
+    out->graph_captures = w->captures;
+    return VS_OK;
+}
+
+int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, uint32_t *launches)
+{
+    if (!w || !ms || !launches) return VS_E_INVALID;
+    HIPW(w, hipSetDevice(w->cfg.device));
+    if (int rc = prepare(w)) return rc;
+    Profile prof;
+    if (int rc = enqueue_frame(w, substeps, splats != 0, -1, &prof)) return rc;
+    HIPW(w, hipStreamSynchronize(w->stream));
+    for (int k = 0; k < VS_P_COUNT; k++) {
+        ms[k] = 0.f;
+        launches[k] = 0;
+    }
+    for (const auto &m : prof.marks) {
+        float t = 0.f;
+        HIPW(w, hipEventElapsedTime(&t, prof.pool[m.second], prof.pool[m.second + 1]));
+        ms[m.first] += t;
+        launches[m.first]++;
+    }
+    return VS_OK;
+}
+
+int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t report_bytes)
+{
+    if (!w || substeps == 0) return VS_E_INVALID;
+    HIPW(w, hipSetDevice(w->cfg.device));
+    if (int rc = prepare(w)) return rc;
+    // The search steps the live scenes; snapshot the node planes and put them
+    // back afterwards so tuning leaves the simulation where it was.
+    const size_t bytes = sizeof(float) * 10 * w->cfg.max_nodes;
+    void *snap = nullptr;
+    HIPW(w, hipMalloc(&snap, bytes));
+    HIPW(w, hipMemcpyAsync(snap, w->d_nodes, bytes, hipMemcpyDeviceToDevice, w->stream));
+    struct Cand {
+        uint32_t block, tile, persistent, xcd;
+        float ms;
+    };
+    std::vector<Cand> cands;
+    for (uint32_t block : { 64u, 128u, 256u, 512u })
+        for (uint32_t mult : { 1u, 2u, 4u })
+            for (uint32_t persistent : { 0u, 2048u })
+                for (uint32_t xcd : { 0u, 1u }) {
+                    if (xcd && !persistent) continue;
+                    cands.push_back(Cand{ block, block * mult, persistent, xcd, 0.f });
+                }
+    const uint32_t saved_block = w->block_cells, saved_tile = w->tile_cells, saved_p = w->cfg.persistent,
+                   saved_x = w->cfg.xcd_chunked;
+    int rc = VS_OK;
+    for (Cand &c : cands) {
+        w->block_cells = c.block;
+        w->tile_cells = c.tile;
+        w->cfg.persistent = c.persistent;
+        w->cfg.xcd_chunked = c.xcd;
+        w->tables_dirty = true;
+        if ((rc = prepare(w))) break;
+        if ((rc = enqueue_frame(w, substeps, false, -1))) break;  // warm-up
+        hipEvent_t a = w->tune[0], b = w->tune[1];
+        HIPW(w, hipEventRecord(a, w->stream));
+        for (int rep = 0; rep < 3 && !rc; rep++) rc = enqueue_frame(w, substeps, false, -1);
+        if (rc) break;
+        HIPW(w, hipEventRecord(b, w->stream));
+        HIPW(w, hipEventSynchronize(b));
+        HIPW(w, hipEventElapsedTime(&c.ms, a, b));
+    }
+    HIPW(w, hipMemcpyAsync(w->d_nodes, snap, bytes, hipMemcpyDeviceToDevice, w->stream));
+    HIPW(w, hipStreamSynchronize(w->stream));
+    HIPW(w, hipFree(snap));
+    if (rc) {
+        w->block_cells = saved_block;
+        w->tile_cells = saved_tile;
+        w->cfg.persistent = saved_p;
+        w->cfg.xcd_chunked = saved_x;
+        w->tables_dirty = true;
+        return rc;
+    }
+    const Cand *best = &cands[0];
+    for (const Cand &c : cands)
+        if (c.ms < best->ms) best = &c;
+    w->block_cells = best->block;
+    w->tile_cells = best->tile;
+    w->cfg.persistent = best->persistent;
+    w->cfg.xcd_chunked = best->xcd;
+    w->tables_dirty = true;
+    if (report && report_bytes) {
+        std::string out = "{\"candidates\":[";
+        for (size_t i = 0; i < cands.size(); i++) {
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "%s{\"block\":%u,\"tile\":%u,\"persistent\":%u,\"xcd\":%u,\"ms\":%.4f}",
+                          i ? "," : "", cands[i].block, cands[i].tile, cands[i].persistent, cands[i].xcd, cands[i].ms);
+            out += buf;
+        }
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "],\"best\":{\"block\":%u,\"tile\":%u,\"persistent\":%u,\"xcd\":%u,\"ms\":%.4f}}",
+                      best->block, best->tile, best->persistent, best->xcd, best->ms);
+        out += buf;
+        std::snprintf(report, report_bytes, "%s", out.c_str());
+    }
+    return VS_OK;
+}
+
+}  // extern "C"
