@@ -99,7 +99,21 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a device-to-device copy of 1 GiB (5 times), the reference kernel "
+                         "for scaling FETCH_SIZE/WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = (1 << 30) // 4
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); scenes shard across GPUs with no "
               "communication, so the whole-job figure is N x the one-GPU figure by construction.", file=sys.stderr)
@@ -163,18 +177,20 @@ def main():
     log(f"[bench] profile: " + ", ".join(f"{k} {v[0]:.3f} ms/{v[1]}" for k, v in prof.items() if v[1]))
 
     # ---- plain per-scene launches (the reference's dispatch shape)
-    for sid in ids:
-        world.frame_scene(sid, substeps, splats=True)
-    sync(world)
-    t1 = time.perf_counter()
-    for f in range(args.plain_steps):
-        set_targets(world, ids, scs, f)
+    plain = None
+    if not args.no_plain and args.plain_steps > 0:
         for sid in ids:
             world.frame_scene(sid, substeps, splats=True)
-    sync(world)
-    plain_wall = time.perf_counter() - t1
-    plain = nodes * substeps * args.plain_steps / plain_wall
-    log(f"[bench] plain: {plain_wall * 1e3 / args.plain_steps:.3f} ms/frame, {plain:.4e} node-substeps/s")
+        sync(world)
+        t1 = time.perf_counter()
+        for f in range(args.plain_steps):
+            set_targets(world, ids, scs, f)
+            for sid in ids:
+                world.frame_scene(sid, substeps, splats=True)
+        sync(world)
+        plain_wall = time.perf_counter() - t1
+        plain = nodes * substeps * args.plain_steps / plain_wall
+        log(f"[bench] plain: {plain_wall * 1e3 / args.plain_steps:.3f} ms/frame, {plain:.4e} node-substeps/s")
     world.close()
 
     cpu = None
@@ -197,7 +213,7 @@ def main():
         "data": "synthetic (seeded lattices and Gaussians, resident in HBM)",
         "config": config,
         "plain": plain,
-        "speedup_vs_plain": value / plain,
+        "speedup_vs_plain": value / plain if plain else None,
         "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
         "nodes": nodes, "cells": cells, "splats": splats, "substeps_per_frame": substeps,
         "profile_ms": {k: {"total": v[0], "launches": v[1]} for k, v in prof.items() if v[1]},

@@ -15,9 +15,9 @@ class Raw:
         self.lib = O.load(precision)
         pos = np.asarray(pos, float)
         n = len(pos)
-        self.p = np.ascontiguousarray(pos.T, dt_)
-        self.q = np.ascontiguousarray((pos if prev is None else np.asarray(prev, float)).T, dt_)
-        self.v = np.ascontiguousarray(np.zeros((3, n)) if vel is None else np.asarray(vel, float).T, dt_)
+        self.p = np.array(pos.T, dt_, order="C", copy=True)
+        self.q = np.array((pos if prev is None else np.asarray(prev, float)).T, dt_, order="C", copy=True)
+        self.v = np.array(np.zeros((3, n)) if vel is None else np.asarray(vel, float).T, dt_, order="C", copy=True)
         self.w = np.ascontiguousarray(np.ones(n) if w is None else w, dt_)
         body_of = np.zeros(n, int) if body_of is None else np.asarray(body_of)
         ob = (O.OBody * len(bodies))()

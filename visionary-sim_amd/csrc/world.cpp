@@ -105,6 +105,7 @@ struct SceneRec {
     uint64_t splat_begin = 0, splat_count = 0;
     bool contact = false, stretch = false;
     std::vector<BodyRec> bodies;
+    std::vector<uint32_t> splat_perm;  // stored splat -> caller's splat index
 };
 
 template <typename T>
@@ -753,17 +754,37 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
         br.splat_begin = splat_at;
         br.splat_count = bd.n_splats;
         if (bd.n_splats) {
+            // Stored in cell order (x fastest) so that the lanes of a wave
+            // gather the same corner nodes; `splat_perm` maps the stored
+            // order back to the caller's for readback.
             const size_t G = w->cfg.max_splats;
             const uint32_t ns = bd.n_splats;
-            std::vector<float> planes(6 * (size_t)ns);
+            std::vector<uint64_t> key(ns);
+            const double inv = 1.0 / (double)bd.h;
             for (uint32_t i = 0; i < ns; i++) {
+                uint64_t c[3];
                 for (int a = 0; a < 3; a++) {
-                    planes[a * ns + i] = bd.splat_mean[3 * i + a];
-                    planes[(3 + a) * ns + i] = bd.splat_scale[3 * i + a];
+                    double g = std::floor(((double)bd.splat_mean[3 * i + a] - (double)bd.origin[a]) * inv);
+                    g = std::min(std::max(g, 0.0), (double)bd.cell_dims[a] - 1.0);
+                    c[a] = (uint64_t)g;
                 }
+                key[i] = c[0] + (uint64_t)bd.cell_dims[0] * (c[1] + (uint64_t)bd.cell_dims[1] * c[2]);
+            }
+            std::vector<uint32_t> order(ns);
+            for (uint32_t i = 0; i < ns; i++) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return key[x] < key[y]; });
+            std::vector<float> planes(6 * (size_t)ns), quat(4 * (size_t)ns);
+            for (uint32_t j = 0; j < ns; j++) {
+                const uint32_t i = order[j];
+                for (int a = 0; a < 3; a++) {
+                    planes[a * (size_t)ns + j] = bd.splat_mean[3 * (size_t)i + a];
+                    planes[(3 + a) * (size_t)ns + j] = bd.splat_scale[3 * (size_t)i + a];
+                }
+                for (int a = 0; a < 4; a++) quat[4 * (size_t)j + a] = bd.splat_quat[4 * (size_t)i + a];
+                r.splat_perm.push_back((uint32_t)(splat_at - r.splat_begin) + i);
             }
             const float *hp = keep(w, std::move(planes));
-            const float *hq = keep(w, std::vector<float>(bd.splat_quat, bd.splat_quat + 4 * (size_t)ns));
+            const float *hq = keep(w, std::move(quat));
             HIPW(w, hipMemcpyAsync(w->d_splat_in + 4 * splat_at, hq, (size_t)ns * 16, hipMemcpyHostToDevice, up));
             for (int k = 0; k < 6; k++)
                 HIPW(w, hipMemcpyAsync(w->d_splat_in + (4 + k) * G + splat_at, hp + (size_t)k * ns, (size_t)ns * 4,
@@ -932,11 +953,13 @@ int vs_world_read_splats(vs_world *w, uint32_t id, float *mean, float *cov)
     if (!w || id >= w->cfg.max_scenes || !w->recs[id].live) return VS_E_INVALID;
     if (int rc = vs_world_synchronize(w)) return rc;
     const SceneRec &r = w->recs[id];
-    const size_t G = w->cfg.max_splats;
+    const size_t G = w->cfg.max_splats, n = r.splat_count;
+    std::vector<float> tmp(n);
     for (int k = 0; k < 9; k++) {
-        float *dst = k < 3 ? (mean ? mean + k * r.splat_count : nullptr) : (cov ? cov + (k - 3) * r.splat_count : nullptr);
-        if (dst && r.splat_count)
-            HIPW(w, hipMemcpy(dst, w->d_splat_out + k * G + r.splat_begin, r.splat_count * 4, hipMemcpyDeviceToHost));
+        float *dst = k < 3 ? (mean ? mean + k * n : nullptr) : (cov ? cov + (k - 3) * n : nullptr);
+        if (!dst || !n) continue;
+        HIPW(w, hipMemcpy(tmp.data(), w->d_splat_out + k * G + r.splat_begin, n * 4, hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < n; j++) dst[r.splat_perm[j]] = tmp[j];
     }
     return VS_OK;
 }

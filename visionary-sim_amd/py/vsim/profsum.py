@@ -1,0 +1,64 @@
+"""Summarises rocprofv3 CSV output for the solver's kernels.
+
+    python -m vsim.profsum trace <dir>/..._kernel_trace.csv
+    python -m vsim.profsum pmc <fetch_counter_collection.csv> <write_counter_collection.csv> \
+        <copy_fetch.csv> <copy_write.csv>
+
+`trace` prints per (kernel, launch grid) call counts and average durations, so
+the batched launches (large grids) separate from the per-scene ones. `pmc`
+prints FETCH_SIZE / WRITE_SIZE per dispatch, scaled to bytes by what the same
+counters read for a device-to-device copy of known size.
+"""
+import collections
+import csv
+import sys
+
+
+def short(name):
+    name = name.replace("(anonymous namespace)::", "").replace("void ", "").strip('"')
+    return name.split("(")[0].replace("vsim::", "")
+
+
+def trace(path, top=20):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        grid = int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0)
+        d[(short(r["Kernel_Name"]), grid)].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    rows = sorted(d.items(), key=lambda kv: -sum(kv[1]))[:top]
+    out = ["| kernel | grid (threads) | calls | avg us | total ms |", "|---|---:|---:|---:|---:|"]
+    for (k, g), v in rows:
+        out.append(f"| {k} | {g} | {len(v)} | {sum(v) / len(v) / 1e3:.2f} | {sum(v) / 1e6:.2f} |")
+    return "\n".join(out)
+
+
+def _per_kernel(path):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        d[(short(r["Kernel_Name"]), int(r.get("Grid_Size") or 0))].append(float(r["Counter_Value"]))
+    return d
+
+
+def pmc(fetch, write, cfetch, cwrite, copy_bytes=1 << 30):
+    cf = max((sum(v) / len(v) for (k, _), v in _per_kernel(cfetch).items() if "copyBuffer" in k), default=None)
+    cw = max((sum(v) / len(v) for (k, _), v in _per_kernel(cwrite).items() if "copyBuffer" in k), default=None)
+    fs = copy_bytes / (cf * 1024) if cf else 1.0
+    ws = copy_bytes / (cw * 1024) if cw else 1.0
+    F, W = _per_kernel(fetch), _per_kernel(write)
+    out = [f"copy calibration: 1 GiB copy reads FETCH_SIZE {cf:.0f} KiB (scale {fs:.3f}), "
+           f"WRITE_SIZE {cw:.0f} KiB (scale {ws:.3f})", "",
+           "| kernel | grid (threads) | dispatches | FETCH MB (scaled) | WRITE MB (scaled) | total MB |",
+           "|---|---:|---:|---:|---:|---:|"]
+    for key in sorted(F, key=lambda k: -sum(F[k])):
+        if key not in W or "rocclr" in key[0]:
+            continue
+        f = sum(F[key]) / len(F[key]) * 1024 * fs / 1e6
+        w = sum(W[key]) / len(W[key]) * 1024 * ws / 1e6
+        out.append(f"| {key[0]} | {key[1]} | {len(F[key])} | {f:.1f} | {w:.1f} | {f + w:.1f} |")
+    return "\n".join(out)
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "trace":
+        print(trace(sys.argv[2]))
+    else:
+        print(pmc(*sys.argv[2:6]))
