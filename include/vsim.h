@@ -62,7 +62,7 @@ typedef struct vs_body {
 
 /* Per-scene uniforms: the Params blocks of predict.wgsl:1-5,
  * finalize.wgsl:1-6, contacts.wgsl:1-8 and drive.wgsl:1-7 in one record.
- * 112 bytes. */
+ * 128 bytes. */
 typedef struct vs_scene_params {
     float dt;
     float gravity[3];

@@ -72,6 +72,12 @@ def stable_step_duration(wave_rate):
     return MAX_STEP_DURATION if wave_rate == 0 else min(MAX_STEP_DURATION, TARGET_CFL / wave_rate)
 
 
+def worst_case_bucket_occupancy(bucket_size, spacing):
+    """Bound on one body's nodes in one bucket (broadphase.js:82-85): cubes of
+    edge spacing/sqrt(3) hold at most one node each."""
+    return math.ceil(math.sqrt(3) * bucket_size / spacing) ** 3
+
+
 def cell_color(i, j, k):
     return (i & 1) | ((j & 1) << 1) | ((k & 1) << 2)
 
