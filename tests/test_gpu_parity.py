@@ -63,3 +63,86 @@ def test_c1_ten_frames_batched_graph():
     print(f"10 frames: exact {np.mean(pos == o.p):.6f}, rel {rel_err(pos, o.p):.3e}")
     assert rel_err(pos, o.p) <= RTOL
     assert rel_err(vel, o.v) <= 1e-4
+
+
+def test_contact_keys_and_pairs_bit_exact():
+    from oracle.oracle import OracleScene
+    sc = scenes.contact_stack(4)
+    w = world_for(sc)
+    w.log_pairs(64)
+    sid = w.add(sc)
+    o = OracleScene(sc, "f32", pair_cap=64)
+    for f in range(3):
+        w.frame(sc.substeps)
+        o.frame(sc.substeps)
+        keys, counts, pairs = w.read_contacts(sid)
+        assert np.array_equal(keys, o.keys)
+        assert np.array_equal(counts, o.pair_count)
+        for s in np.nonzero(counts)[0]:
+            n = min(counts[s], 64)
+            assert np.array_equal(pairs[s, :n], o.pairs[s, :n])
+        pos, vel = w.read_nodes(sid)
+        assert rel_err(pos, o.p) <= RTOL
+    print(f"pairs in last substep: {int(counts.sum())}")
+    assert counts.sum() > 0
+
+
+def test_contact_many_bodies_one_substep():
+    from oracle.oracle import OracleScene
+    sc = scenes.c3_scene(bodies=64)
+    w, sid, o, pos, vel = step_both(sc, 1, substeps=1)
+    assert rel_err(pos, o.p) <= RTOL
+    assert rel_err(vel, o.v) <= RTOL
+
+
+@pytest.mark.parametrize("law", [0, 1])
+def test_drive_handle_frames(law):
+    sc = scenes.jelly_block(12, pinned=False, tank_margin=0.3, name="drag")
+    sc.tank_min = (sc.tank_min[0], 0.0, sc.tank_min[2])
+    sc.drive = scenes.handle_drive(sc, seed=3, law=law)
+    w, sid, o, pos, vel = step_both(sc, 6)
+    print(f"drive law {law}: exact {np.mean(pos == o.p):.6f}, rel {rel_err(pos, o.p):.3e}")
+    assert rel_err(pos, o.p) <= RTOL
+    held = sc.drive.nodes
+    if law == 0:
+        # hard law: held nodes sit exactly on their goals after each frame
+        goal = np.asarray(sc.drive.frame_target(5, sc.substeps * sc.dt)[0], np.float32)[:, None] + sc.drive.offsets.T
+        assert np.max(np.abs(pos[:, held] - goal)) < 1e-5
+
+
+def test_stretch_limit_sweeps():
+    sc = scenes.jelly_block(8, pinned=True, name="stretch")
+    sc.bodies[0].material.max_stretch = 1.02
+    sc.drive = scenes.handle_drive(sc, seed=5, law=0, amplitude=0.3, frequency=2.0)
+    w, sid, o, pos, vel = step_both(sc, 4)
+    assert rel_err(pos, o.p) <= RTOL
+
+
+def test_splats_after_frames():
+    sc = scenes.c1_scene(splats=50_000)
+    w, sid, o, pos, vel = step_both(sc, 3, splats=True)
+    mean, cov = w.read_splats(sid)
+    print(f"splats: mean exact {np.mean(mean == o.out_mean):.6f}, cov exact {np.mean(cov == o.out_cov):.6f}")
+    assert rel_err(mean, o.out_mean) <= RTOL
+    assert rel_err(cov, o.out_cov) <= RTOL
+
+
+def test_batched_equals_alone_and_graph_equals_eager():
+    mix = [scenes.jelly_block(s, splats=300, seed=s, name=f"m{s}") for s in (5, 9, 13)]
+    mix[1].drive = scenes.handle_drive(mix[1], seed=2)
+    w = world_for(*mix)
+    ids = [w.add(s) for s in mix]
+    for f in range(4):
+        for sid, sc in zip(ids, mix):
+            if sc.drive is not None:
+                w.set_target(sid, *sc.drive.frame_target(f, sc.substeps * sc.dt))
+        w.frame(10, splats=True, graph=(f % 2 == 0))
+    together = [w.read_nodes(i)[0] for i in ids]
+    for sc, got in zip(mix, together):
+        alone = world_for(sc)
+        a = alone.add(sc)
+        for f in range(4):
+            if sc.drive is not None:
+                alone.set_target(a, *sc.drive.frame_target(f, sc.substeps * sc.dt))
+            alone.frame_scene(a, 10, True)
+        assert np.array_equal(alone.read_nodes(a)[0], got)

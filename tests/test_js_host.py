@@ -1,0 +1,100 @@
+"""The JavaScript host (visionary-sim_amd/js) over the Node-API addon.
+
+CPU: the host's clock, lattice and backend validation (tests/js/host.test.mjs,
+addon faked). GPU: the Simulator runs a scene on the HIP backend and its
+lattice, positions and splats are compared with the Python host and the f32
+oracle."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from vsim import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NODE = shutil.which("node")
+
+
+def run_node(*args, timeout=300):
+    return subprocess.run([NODE, *args], cwd=ROOT, capture_output=True, text=True, timeout=timeout)
+
+
+@pytest.mark.skipif(NODE is None, reason="node is not installed")
+def test_js_host_cpu_suite():
+    r = run_node("tests/js/run.mjs", "tests/js/host.test.mjs")
+    print(r.stdout[-3000:], r.stderr[-3000:])
+    assert r.returncode == 0
+
+
+def spec_for(sc, frames, tmp, splats=True):
+    b = sc.bodies[0]
+    lat = b.lattice
+    spec = dict(cells=list(lat.cell_dims), origin=list(lat.origin), h=lat.h,
+                material=dict(youngsModulus=b.material.youngs_modulus, poissonRatio=b.material.poisson_ratio,
+                              density=b.material.density, friction=b.material.friction, damping=b.material.damping,
+                              relaxation=b.material.relaxation, maxStretch=b.material.max_stretch),
+                pinned=b.static_nodes is not None, tank=dict(min=list(sc.tank_min), max=list(sc.tank_max)),
+                dt=sc.dt, substeps=sc.substeps, frames=frames, splats=None, drive=None)
+    if splats and b.n_splats:
+        d = os.path.join(tmp, "splats")
+        os.makedirs(d, exist_ok=True)
+        b.splat_mean.astype(np.float32).tofile(os.path.join(d, "means.bin"))
+        b.splat_quat.astype(np.float32).tofile(os.path.join(d, "quats.bin"))
+        b.splat_scale.astype(np.float32).tofile(os.path.join(d, "scales.bin"))
+        spec["splats"] = d
+    if sc.drive is not None:
+        fdt = sc.substeps * sc.dt
+        spec["drive"] = dict(law=sc.drive.law, nodes=[int(n) for n in sc.drive.nodes],
+                             offsets=sc.drive.offsets.astype(float).tolist(),
+                             targets=[list(map(float, sc.drive.target(f * fdt))) for f in range(frames + 1)])
+    path = os.path.join(tmp, "spec.json")
+    with open(path, "w") as f:
+        json.dump(spec, f)
+    return path
+
+
+def load(out, name, dtype=np.float32):
+    return np.fromfile(os.path.join(out, name), dtype)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["block", "drag"])
+def test_js_simulator_matches_oracle(tmp_path, case):
+    from oracle.oracle import OracleScene
+    if case == "block":
+        sc = scenes.jelly_block(16, splats=20_000, seed=21, name="js")
+    else:
+        sc = scenes.jelly_block(12, splats=0, seed=22, pinned=False, tank_margin=0.3, name="jsdrag")
+        sc.tank_min = (sc.tank_min[0], 0.0, sc.tank_min[2])
+        sc.drive = scenes.handle_drive(sc, seed=4)
+    frames = 5
+    spec = spec_for(sc, frames, str(tmp_path))
+    out = str(tmp_path / "out")
+    r = run_node("tests/js/sim_gpu.mjs", spec, out)
+    print(r.stdout, r.stderr[-3000:])
+    assert r.returncode == 0
+    lat = sc.bodies[0].lattice
+    # The JS host builds exactly the Python host's lattice.
+    assert np.array_equal(load(out, "rest.bin").reshape(-1, 3), lat.rest)
+    assert np.array_equal(load(out, "cells.bin", np.uint32), lat.cells)
+    assert np.array_equal(load(out, "surface.bin", np.uint32), scenes.surface_nodes(lat))
+    arrays = sc.arrays()
+    assert np.array_equal(load(out, "w.bin"), arrays["w"])
+    o = OracleScene(sc, "f32")
+    for _ in range(frames):
+        o.frame(sc.substeps, splats=True)
+    pos = load(out, "pos.bin").reshape(3, -1)
+    vel = load(out, "vel.bin").reshape(3, -1)
+    scale = np.max(np.abs(o.p))
+    err = np.max(np.abs(pos - o.p)) / scale
+    print(f"{case}: exact {np.mean(pos == o.p):.6f} rel {err:.3e}")
+    assert err <= 1e-5
+    assert np.max(np.abs(vel - o.v)) / max(np.max(np.abs(o.v)), 1e-6) <= 1e-4
+    if sc.bodies[0].n_splats:
+        mean = load(out, "splat_means.bin").reshape(3, -1)
+        cov = load(out, "splat_cov.bin").reshape(6, -1)
+        assert np.max(np.abs(mean - o.out_mean)) / np.max(np.abs(o.out_mean)) <= 1e-5
+        assert np.max(np.abs(cov - o.out_cov)) / np.max(np.abs(o.out_cov)) <= 1e-5

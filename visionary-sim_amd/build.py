@@ -82,7 +82,7 @@ def build_addon():
     tmp = out + ".tmp"
     cmd = [shutil.which("g++") or "g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I" + inc,
            "-I" + os.path.join(ROOT, "include"), src, "-o", tmp, "-L" + LIB, "-lvsim",
-           "-Wl,-rpath,$ORIGIN", "-Wl,--unresolved-symbols=ignore-in-shared-libs"]
+           "-Wl,-rpath,$ORIGIN"]
     _run([cmd])
     os.replace(tmp, out)
     return out

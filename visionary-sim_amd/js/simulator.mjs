@@ -1,0 +1,224 @@
+// The simulation's public surface, kept from the kernel package
+// (packages/visionary-sim/src/sim/simulator.js:35-518): assembly in
+// dependency order, step(dt) through the fixed clock, dispatch(steps) as one
+// submission per frame, stats, material changes, reset, readback. The
+// compute backend underneath is the HIP one. Interactive picking and grip
+// strategies stay out of scope; a held node set is given directly (hold).
+import { ComputeBackend } from './compute/backend.mjs';
+import { proxyFor } from './discretize/lattice.mjs';
+import { Particles } from './state/particles.mjs';
+import { KinematicDrive } from './constraints/kinematicDrive.mjs';
+import { elasticConstraints } from './constraints/elasticCells.mjs';
+import { BodyContact } from './constraints/bodyContact.mjs';
+import { Stepper } from './solver/stepper.mjs';
+import { FixedTimestep, stableStepDuration } from './solver/timestep.mjs';
+import { collapseScale, gammaOf, waveRatio } from './constraints/compliance.mjs';
+import { DEFAULT_MATERIAL, validateMaxStretch } from './body/material.mjs';
+
+const DEFAULT_PARAMS = { gravity: [0, -9.81, 0], paused: false };
+
+/** A body as an author describes it (body/softBody.js:37-60). */
+export class SoftBody {
+    constructor({ name, source, material = {}, staticRegion = null, velocity = null }) {
+        this.name = name;
+        this.source = source;
+        this.material = Object.assign({}, DEFAULT_MATERIAL, material);
+        validateMaxStretch(this.material.maxStretch);
+        this.staticRegion = staticRegion;
+        this.velocity = velocity;
+    }
+}
+
+/** Body-local node ids whose rest position the region claims (staticPrescription.js:9-20). */
+export function staticNodesInRegion(restPositions, region = null) {
+    const nodes = [];
+    if (region === null) return nodes;
+    for (let n = 0; n < restPositions.length / 3; n++) {
+        if (region(restPositions[n * 3], restPositions[n * 3 + 1], restPositions[n * 3 + 2])) nodes.push(n);
+    }
+    return nodes;
+}
+
+export class Simulator {
+    /**
+     * @param {number|ComputeBackend} device - HIP device index, or a backend to borrow.
+     * @param {SoftBody[]} bodies - Bodies to simulate.
+     * @param {{ min: number[], max: number[] }} tank - Tank interior bounds.
+     * @param {{ bucketSize?: number }} [options] - Contact grid override.
+     */
+    constructor(device, bodies, tank, options = {}) {
+        const owned = !(device instanceof ComputeBackend);
+        const backend = owned ? new ComputeBackend(device) : device;
+        this.backend = backend;
+        this.ownsBackend = owned;
+        this.destroyed = false;
+        try {
+            this.tank = tank;
+            this.params = Object.assign({}, DEFAULT_PARAMS);
+            this.simBodies = bodies.map(body => ({ body, proxy: proxyFor(body) }));
+            this.uniforms = backend.uniforms();
+            this.particles = new Particles(backend, this.simBodies.map(({ body, proxy }) => Object.assign({}, proxy.dof, {
+                staticNodes: staticNodesInRegion(proxy.dof.restPositions, body.staticRegion),
+                velocity: body.velocity
+            })), this.uniforms);
+            this.drive = new KinematicDrive(backend, this.particles);
+            this.elasticSets = elasticConstraints(backend, this.particles,
+                this.simBodies.map(({ proxy }) => proxy), this.simBodies.map(({ body }) => body.material));
+            this.contact = new BodyContact(backend, this.particles, this.simBodies.map(({ proxy }) => proxy), tank, options);
+            // Solve order (simulator.js:76-83): drive, elasticity, contact.
+            this.stepper = new Stepper(backend, this.particles, [this.drive, ...this.elasticSets, this.contact]);
+            this.clock = new FixedTimestep(stableStepDuration(this.waveRate()));
+            this.simBodies.forEach(({ body }, b) => {
+                this.particles.setParams(b, { damping: body.material.damping, friction: body.material.friction });
+            });
+            this.grabTarget = { position: [0, 0, 0], previousPosition: [0, 0, 0] };
+            this.pendingStep = false;
+        } catch (error) {
+            if (owned) backend.destroy();
+            throw error;
+        }
+    }
+
+    worst() {
+        if (this.simBodies.length === 0) return null;
+        return this.simBodies.reduce((worst, entry) =>
+            (waveRatio(entry.body.material, entry.proxy.cflLength) > waveRatio(worst.body.material, worst.proxy.cflLength)
+                ? entry : worst));
+    }
+
+    waveRate() {
+        const limiting = this.worst();
+        return limiting ? waveRatio(limiting.body.material, limiting.proxy.cflLength) : 0;
+    }
+
+    cflNumber() { return this.waveRate() * this.clock.stepDuration; }
+
+    stats() {
+        const timing = this.clock.snapshot();
+        const worst = this.worst();
+        return Object.assign({
+            bodyCount: this.simBodies.length,
+            nodes: this.particles.nodeCount,
+            cells: this.elasticSets.reduce((sum, set) => sum + set.cellCount, 0),
+            contactNodes: this.contact.surfaceCount,
+            contact: this.contact.enabled,
+            cfl: this.cflNumber(),
+            limitedBy: worst ? worst.body.name : 'nothing',
+            perBody: this.simBodies.map(({ body }) => {
+                const gamma = gammaOf(body.material);
+                return { name: body.name, gamma, collapseScale: collapseScale(gamma) };
+            })
+        }, timing);
+    }
+
+    /** Advances by one render frame of `dt` seconds (simulator.js:239-278). */
+    step(dt) {
+        if (this.particles.nodeCount === 0) {
+            this.pendingStep = false;
+            this.clock.idle();
+            return;
+        }
+        const elapsed = Number.isFinite(dt) && dt >= 0 ? dt : 0;
+        const pendingStep = this.pendingStep;
+        this.pendingStep = false;
+        let timing;
+        if (this.params.paused) {
+            if (!pendingStep) {
+                this.clock.idle();
+                return;
+            }
+            timing = this.clock.singleStep();
+        } else {
+            timing = this.clock.consume(elapsed, { forceStep: pendingStep });
+        }
+        this.dispatch(timing.steps);
+    }
+
+    requestStep() { this.pendingStep = true; }
+
+    /** All `steps` substeps of a frame in one submission (simulator.js:292-318). */
+    dispatch(steps) {
+        if (steps < 1 || this.particles.nodeCount === 0) return;
+        this.stepper.setFrameParameters({
+            substepDt: this.clock.stepDuration,
+            gravity: this.params.gravity,
+            tankMin: this.tank.min,
+            tankMax: this.tank.max
+        });
+        if (this.drive.drivenCount > 0) {
+            const { position, previousPosition } = this.grabTarget;
+            const frameDt = steps * this.clock.stepDuration;
+            this.drive.setGrabTarget(position, [0, 1, 2].map(a => (position[a] - previousPosition[a]) / frameDt));
+            this.grabTarget.previousPosition = position;
+        }
+        this.backend.dispatch(this.stepper.sequence(steps), 'softbody-step');
+    }
+
+    /** Holds `nodes` (global ids) at `offsets` from an anchor at `position`. */
+    hold(nodes, offsets, position) {
+        this.drive.hold(nodes, offsets);
+        this.grabTarget.position = position.slice();
+        this.grabTarget.previousPosition = position.slice();
+    }
+
+    moveGrab(position) { this.grabTarget.position = position.slice(); }
+
+    releaseGrab() { if (!this.destroyed) this.drive.release(); }
+
+    setDriveLaw(law) { this.drive.law = law; }
+
+    setDriveResponseTime(seconds) { this.drive.responseTime = seconds; }
+
+    setContact(enabled) { this.contact.enabled = enabled; }
+
+    setMaterial(bodyIndex, changes) {
+        if ('maxStretch' in changes) validateMaxStretch(changes.maxStretch);
+        const { body, proxy } = this.simBodies[bodyIndex];
+        const rest = Object.assign({}, changes);
+        const density = rest.density;
+        delete rest.density;
+        Object.assign(body.material, rest);
+        if (density > 0 && density !== body.material.density) {
+            proxy.setDensity(body.material.density, density);
+            body.material.density = density;
+            this.particles.uploadMasses();
+        }
+        this.particles.setParams(bodyIndex, { damping: body.material.damping, friction: body.material.friction });
+        for (const elastic of this.elasticSets) elastic.uploadParams();
+        this.clock.setStepDuration(stableStepDuration(this.waveRate()));
+    }
+
+    /** A body's deformation field: the live node buffer and where the body sits in it. */
+    deformationOf(bodyIndex) {
+        const { proxy } = this.simBodies[bodyIndex];
+        const { origin, h, cellDims, nodeDims } = proxy.lattice;
+        return {
+            kernel: 'sim-splat-deform',
+            parameters: {
+                deformNodePos: this.particles.nodePos, deformOrigin: origin, deformH: h,
+                deformCellDims: cellDims, deformNodeDims: nodeDims, deformNodeOffset: this.particles.offsetOf(bodyIndex),
+                body: bodyIndex
+            }
+        };
+    }
+
+    reset() {
+        this.releaseGrab();
+        this.particles.reset();
+        this.clock.reset();
+        this.pendingStep = false;
+    }
+
+    readPositions() { return this.particles.readPositions(); }
+
+    destroy() {
+        if (this.destroyed) return;
+        this.destroyed = true;
+        this.stepper.destroy();
+        this.contact.destroy();
+        for (const elastic of this.elasticSets) elastic.destroy();
+        this.drive.destroy();
+        this.particles.destroy();
+        if (this.ownsBackend) this.backend.destroy();
+    }
+}

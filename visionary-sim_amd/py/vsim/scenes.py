@@ -487,6 +487,24 @@ def c3_scene(bodies=512, side=8, h=0.01, box=1.0, seed=3, bucket_voxels=1.0):
                  bucket_size=float(np.float32(bucket_voxels * h)) if bucket_voxels != 1.0 else 0.0, name="c3")
 
 
+def contact_stack(count=3, side=4, h=0.01, gap=-0.25, seed=9, friction=0.4):
+    """`count` blocks of side^3 cells stacked along y, each overlapping the one
+    below by -gap cells (gap < 0) and shifted sideways by a seeded fraction
+    of a cell, falling onto the tank floor: contact from the first substep."""
+    rng = np.random.default_rng(seed)
+    bodies = []
+    for k in range(count):
+        shift = rng.uniform(-0.5, 0.5, 2) * h
+        origin = (shift[0], k * (side + gap) * h + 0.001, shift[1])
+        mat = Material(youngs_modulus=5e4, poisson_ratio=0.3, density=1000.0, friction=friction)
+        bodies.append(Body(block_lattice((side, side, side), origin, h, mat.density), mat))
+    rate = max(wave_speed(b.material) / b.lattice.h for b in bodies)
+    ext = count * side * h
+    return Scene(bodies, dt=float(np.float32(stable_step_duration(rate))), substeps=8,
+                 tank_min=(-0.1, 0.0, -0.1), tank_max=(0.1 + side * h, ext + 0.1, 0.1 + side * h), contact=1,
+                 name="stack")
+
+
 def c4_scenes(count=256, seed=4, min_side=8, max_side=48, min_splats=10_000, max_splats=500_000):
     """`count` independent jelly blocks, side ~ U{8..48}, splats log-uniform."""
     rng = np.random.default_rng(seed)
