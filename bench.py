@@ -88,6 +88,31 @@ def cpu_baseline(scs, budget_s):
                        f"{elapsed:.1f} s) of the same workload, splats included")
 
 
+def serve_leg(count, budget_nodes):
+    """Poisson arrivals (50/s) of scenes with log-uniform 4k-500k nodes, each
+    running 120 frames with a seeded drag handle, admitted while a world of
+    `budget_nodes` node capacity holds them; continuous batching."""
+    from vsim import native, scenes
+    reqs = scenes.c5_requests(count)
+    cells = int(budget_nodes * 1.05)
+    world = native.World(max_scenes=256, max_bodies=256, max_nodes=budget_nodes, max_cells=cells,
+                         max_driven=256 * 27, max_splats=1024, persistent=2048)
+    srv = native.Server(world, substeps=10, graph=True)
+    for t, sc, frames in reqs:
+        srv.submit(sc, t, frames)
+    rep = srv.run(realtime=True)
+    lat = srv.latencies()
+    srv.close()
+    world.close()
+    return {"requests": count, "arrival_rate_per_s": 50, "frames_per_request": 120,
+            "budget_nodes": budget_nodes, "wall_s": rep["wall_s"],
+            "node_substeps_per_s": rep["node_substeps"] / rep["wall_s"],
+            "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                 "max": float(lat.max())},
+            "queue_s": {"mean": rep["mean_queue_s"], "max": rep["max_queue_s"]},
+            "max_active": rep["max_active"], "iterations": rep["iterations"], "completed": rep["completed"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +125,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=120,
+                    help="scene requests of the serving leg (Poisson 50/s, 4k-500k nodes, 120 frames); 0 skips it")
+    ap.add_argument("--serve-budget-nodes", type=float, default=6e6,
+                    help="node capacity of the serving world (its HBM budget for node state)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a device-to-device copy of 1 GiB (5 times), the reference kernel "
                          "for scaling FETCH_SIZE/WRITE_SIZE counters to bytes")
@@ -193,6 +222,11 @@ def main():
         log(f"[bench] plain: {plain_wall * 1e3 / args.plain_steps:.3f} ms/frame, {plain:.4e} node-substeps/s")
     world.close()
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_leg(args.serve_requests, int(args.serve_budget_nodes))
+        log(f"[bench] serving: {serving}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scs, args.cpu_budget)
@@ -221,6 +255,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_elastic (colour pass)", "bytes_per_launch": el_bytes,
                      "avg_launch_us": el_avg_s * 1e6},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     if tune:

@@ -293,6 +293,45 @@ int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, 
  * fastest (block size x cells per tile x persistent grid x XCD mapping). */
 int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t report_bytes);
 
+/* ------------------------------------------------------ the server */
+
+typedef struct vs_server vs_server;
+
+typedef struct vs_server_config {
+    uint32_t substeps;     /* substeps per frame                                 */
+    int32_t splats;        /* deform splats every frame                          */
+    int32_t graph;         /* replay frames from a captured hipGraph             */
+    int32_t keep_results;  /* keep each finished scene's final node positions     */
+} vs_server_config;
+
+typedef struct vs_server_report {
+    double wall_s;
+    uint64_t node_substeps, frames;
+    uint32_t requests, completed, iterations, max_active;
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* frame latency                  */
+    double mean_queue_s, max_queue_s;          /* arrival to admission           */
+} vs_server_report;
+
+/* A continuous-batching server over `world` (which it borrows). */
+int vs_server_create(vs_world *world, const vs_server_config *cfg, vs_server **out);
+void vs_server_destroy(vs_server *s);
+const char *vs_server_error(const vs_server *s);
+/* Queues a scene (copied) arriving `arrival_s` seconds after run() starts,
+ * to run `frames` frames; `targets` holds frames + 1 grab-target positions
+ * (xyz) when the scene has a drive, else NULL. Submit in arrival order. */
+int vs_server_submit(vs_server *s, const vs_scene_desc *desc, double arrival_s, uint32_t frames,
+                     const float *targets, uint32_t *request_id);
+/* Serves every request: admits arrivals while the world's pools hold them,
+ * steps all admitted scenes one batched frame per iteration, retires finished
+ * scenes. `realtime` = 0 ignores arrival times (all requests are ready). */
+int vs_server_run(vs_server *s, int32_t realtime, vs_server_report *report);
+/* Every frame latency (ms) of the last run, in completion order. */
+int vs_server_latencies(const vs_server *s, float *ms, uint64_t cap, uint64_t *count);
+/* Final node positions (3 planes) of a finished request (keep_results). */
+int vs_server_result(const vs_server *s, uint32_t request_id, float *pos);
+/* Seconds since run() began: arrival, admission and completion of a request. */
+int vs_server_times(const vs_server *s, uint32_t request_id, double *arrival, double *admitted, double *finished);
+
 #ifdef __cplusplus
 }
 #endif

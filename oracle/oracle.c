@@ -39,6 +39,19 @@ static inline v3 cross(v3 a, v3 b)
 }
 static inline real length3(v3 a) { return SQRT(dot(a, a)); }
 
+/* Fused multiply-add in the working precision (fmaf for the f32 build: one
+ * rounding, as the GPU's v_fma_f32). */
+static inline real FMA(real a, real b, real c)
+{
+    return sizeof(real) == 4 ? (real)fmaf((float)a, (float)b, (float)c) : (real)fma((double)a, (double)b, (double)c);
+}
+static inline real dotf(v3 a, v3 b) { return FMA(a.z, b.z, FMA(a.y, b.y, a.x * b.x)); }
+static inline real dotacc(v3 a, v3 b, real acc) { return FMA(a.z, b.z, FMA(a.y, b.y, FMA(a.x, b.x, acc))); }
+static inline v3 crossf(v3 a, v3 b)
+{
+    return mk(FMA(a.y, b.z, -(a.z * b.y)), FMA(a.z, b.x, -(a.x * b.z)), FMA(a.x, b.y, -(a.y * b.x)));
+}
+
 /* (m * n)[:, j] = m[:,0] * n[j].x + m[:,1] * n[j].y + m[:,2] * n[j].z */
 static m3 mmul(m3 m, m3 n)
 {
@@ -82,6 +95,10 @@ static const int KUHN_TETS[6][4] = {
  * edge matrices). Every Dm has det 1 and integer entries, so the inverse is
  * the adjugate and exact in any precision. Built once, lazily. */
 static m3 KUHN_INV[6];
+/* Every Kuhn tetrahedron is a path of three axis steps from corner 0 to
+ * corner 7: PATH[t] lists its corners in path order, AXIS[t][j] the axis of
+ * step j, POS[t][i] the path position of the tet's i-th vertex. */
+static int PATH[6][4], AXIS[6][3], POS[6][4];
 static real STRETCH_LEN[28];
 static int STRETCH_A[28], STRETCH_B[28];
 static int tables_ready;
@@ -111,6 +128,27 @@ static void build_tables(void)
         for (int col = 0; col < 3; col++) {
             KUHN_INV[t].c[col] = mk((real)inv[col * 3], (real)inv[col * 3 + 1], (real)inv[col * 3 + 2]);
         }
+    }
+    for (int t = 0; t < 6; t++) {
+        int at = 0;
+        PATH[t][0] = 0;
+        for (int j = 0; j < 3; j++) {
+            for (int a = 0; a < 3; a++) {
+                int next = at | (1 << a);
+                if (next == at) continue;
+                int member = 0;
+                for (int v = 0; v < 4; v++) member |= KUHN_TETS[t][v] == next;
+                if (member) {
+                    AXIS[t][j] = a;
+                    at = next;
+                    break;
+                }
+            }
+            PATH[t][j + 1] = at;
+        }
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                if (PATH[t][j] == KUHN_TETS[t][i]) POS[t][i] = j;
     }
     /* STRETCH_EDGES, stretch.js:5-7: every corner pair i<j with its unit-cell
      * length, stored as an f32 literal by the kernel generator. */
@@ -195,6 +233,111 @@ static void project_tet(v3 *p, const v3 *prev, const real *w, const int *tet, m3
     p[i3] = add(p[i3], scale(add(scale(d3, lD), scale(h3, lH)), w3));
 }
 
+/* The same projection in the evaluation order the GPU kernel uses. The
+ * mathematics is project.wgsl's: because every Kuhn tetrahedron is an axis
+ * path, F = Ds * Dm^-1 has the path's edge differences times 1/h as its
+ * columns (axis by axis), and (dC/dF) * B^T gives each path corner the
+ * difference of the per-axis gradients of the steps into and out of it
+ * (d0 = -(d1 + d2 + d3) holds by construction). Products feeding a sum are
+ * fused (one rounding), and the 2x2 block divides once. */
+static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real s, real gamma, real aD, real aH,
+                              real relax)
+{
+    const int *tv = KUHN_TETS[t];
+    real wt[4] = { w[tv[0]], w[tv[1]], w[tv[2]], w[tv[3]] };
+    if (((wt[0] + wt[1]) + wt[2]) + wt[3] == (real)0) return;
+    v3 F[3];
+    for (int j = 0; j < 3; j++) F[AXIS[t][j]] = scale(sub(p[PATH[t][j + 1]], p[PATH[t][j]]), s);
+    real n2 = F[0].x * F[0].x;
+    n2 = FMA(F[0].y, F[0].y, n2);
+    n2 = FMA(F[0].z, F[0].z, n2);
+    for (int c = 1; c < 3; c++) {
+        n2 = FMA(F[c].x, F[c].x, n2);
+        n2 = FMA(F[c].y, F[c].y, n2);
+        n2 = FMA(F[c].z, F[c].z, n2);
+    }
+    real cDev = SQRT(n2);
+    v3 H[3] = { crossf(F[1], F[2]), crossf(F[2], F[0]), crossf(F[0], F[1]) };
+    real cHyd = dotf(F[0], H[0]) - gamma;
+    real k = cDev > (real)(float)1e-9 ? (real)1 / cDev : (real)0;
+    real ks = k * s;
+    v3 gD[3], gH[3];
+    for (int a = 0; a < 3; a++) {
+        gD[a] = scale(F[a], ks);
+        gH[a] = scale(H[a], s);
+    }
+    v3 d[4], h[4], x[4];
+    for (int i = 0; i < 4; i++) {
+        int j = POS[t][i];
+        if (j == 0) {
+            d[i] = neg(gD[AXIS[t][0]]);
+            h[i] = neg(gH[AXIS[t][0]]);
+        } else if (j == 3) {
+            d[i] = gD[AXIS[t][2]];
+            h[i] = gH[AXIS[t][2]];
+        } else {
+            d[i] = sub(gD[AXIS[t][j - 1]], gD[AXIS[t][j]]);
+            h[i] = sub(gH[AXIS[t][j - 1]], gH[AXIS[t][j]]);
+        }
+        x[i] = sub(p[tv[i]], q[tv[i]]);
+    }
+    real gx = 0, gy = 0, gz = 0, rD = 0, rH = 0;
+    for (int i = 0; i < 4; i++) {
+        gx = FMA(wt[i], dotf(d[i], d[i]), gx);
+        gy = FMA(wt[i], dotf(d[i], h[i]), gy);
+        gz = FMA(wt[i], dotf(h[i], h[i]), gz);
+        rD = dotacc(d[i], x[i], rD);
+        rH = dotacc(h[i], x[i], rH);
+    }
+    real st = (real)1 + relax;
+    real a00 = FMA(gx, st, aD);
+    real a01 = gy * st;
+    real a11 = FMA(gz, st, aH);
+    real det = FMA(a00, a11, -(a01 * a01));
+    if (det < (real)(float)1e-20) return;
+    real bD = -FMA(rD, relax, cDev);
+    real bH = -FMA(rH, relax, cHyd);
+    real inv = (real)1 / det;
+    real lD = FMA(a11, bD, -(a01 * bH)) * inv;
+    real lH = FMA(a00, bH, -(a01 * bD)) * inv;
+    for (int i = 0; i < 4; i++) {
+        v3 u = scale(d[i], lD);
+        u = mk(FMA(h[i].x, lH, u.x), FMA(h[i].y, lH, u.y), FMA(h[i].z, lH, u.z));
+        v3 *pi = &p[tv[i]];
+        *pi = mk(FMA(u.x, wt[i], pi->x), FMA(u.y, wt[i], pi->y), FMA(u.z, wt[i], pi->z));
+    }
+}
+
+/* Kuhn tetrahedron t of a cell of edge h, corners given in KUHN_TETS order:
+ * either evaluation (literal = WGSL formulas of project.wgsl, else the fused
+ * GPU order). Alphas already over dt^2. */
+void o_project_kuhn_tet(const REAL *p12, const REAL *prev12, const REAL *w4, int t, double h, double alpha_dev,
+                        double alpha_hyd, double gamma, double relax, int literal, REAL *out12)
+{
+    build_tables();
+    v3 p[8], q[8];
+    real w[8] = { 0 };
+    for (int c = 0; c < 8; c++) p[c] = q[c] = mk(0, 0, 0);
+    for (int i = 0; i < 4; i++) {
+        int c = KUHN_TETS[t][i];
+        p[c] = mk(p12[i * 3], p12[i * 3 + 1], p12[i * 3 + 2]);
+        q[c] = mk(prev12[i * 3], prev12[i * 3 + 1], prev12[i * 3 + 2]);
+        w[c] = w4[i];
+    }
+    real s = (real)1 / (real)h;
+    if (literal)
+        project_tet(p, q, w, KUHN_TETS[t], mscale(KUHN_INV[t], s), (real)gamma, (real)alpha_dev, (real)alpha_hyd,
+                    (real)relax);
+    else
+        project_tet_fused(p, q, w, t, s, (real)gamma, (real)alpha_dev, (real)alpha_hyd, (real)relax);
+    for (int i = 0; i < 4; i++) {
+        int c = KUHN_TETS[t][i];
+        out12[i * 3] = p[c].x;
+        out12[i * 3 + 1] = p[c].y;
+        out12[i * 3 + 2] = p[c].z;
+    }
+}
+
 /* Known-answer entry for one tetrahedron, the shape of `projectCoupled` in the
  * reference's elastic.test.js:111-127 (B column-major; alphas already over dt^2). */
 void o_project_tet(const REAL *p12, const REAL *prev12, const REAL *w4, const REAL *B9,
@@ -222,7 +365,7 @@ void o_project_tet(const REAL *p12, const REAL *prev12, const REAL *w4, const RE
 static inline v3 load3(const real *x, const real *y, const real *z, uint32_t n) { return mk(x[n], y[n], z[n]); }
 
 /* project.wgsl:35-184 for one cell; cellCoords/cornerNode from cells.wgsl:39-53. */
-static void project_cell(o_scene *sc, const o_body *cp, uint32_t lin, real dt, int limit_only)
+static void project_cell(o_scene *sc, const o_body *cp, uint32_t lin, real dt, int limit_only, int literal)
 {
     uint32_t cx = lin % cp->cell_dims[0];
     uint32_t cy = (lin / cp->cell_dims[0]) % cp->cell_dims[1];
@@ -248,8 +391,12 @@ static void project_cell(o_scene *sc, const o_body *cp, uint32_t lin, real dt, i
         real restScale = (real)1 / h;
         real gamma = (real)cp->gamma;
         for (int t = 0; t < 6; t++) {
-            m3 B = mscale(KUHN_INV[t], restScale);
-            project_tet(p, prev, w, KUHN_TETS[t], B, gamma, aD, aH, relax);
+            if (literal) {
+                m3 B = mscale(KUHN_INV[t], restScale);
+                project_tet(p, prev, w, KUHN_TETS[t], B, gamma, aD, aH, relax);
+            } else {
+                project_tet_fused(p, prev, w, t, restScale, gamma, aD, aH, relax);
+            }
         }
     }
     if (maxStretch > (real)1) {
@@ -286,7 +433,7 @@ void o_elastic_color(o_scene *sc, int color, int limit_only)
         uint32_t lin = sc->cells[(size_t)(start + s) * 2 + 1];
         const o_body *cp = &sc->bodies[body];
         if (limit_only && !((real)cp->max_stretch > (real)1)) continue;
-        project_cell(sc, cp, lin, dt, limit_only);
+        project_cell(sc, cp, lin, dt, limit_only, sc->literal);
     }
 }
 

@@ -63,7 +63,9 @@ typedef struct {
     uint32_t *key_out;           /* bucket of each surface record */
     int32_t *pair_count;         /* pairs passing the reach test, per record */
     uint32_t *pairs;             /* mate record ids, pair_cap per record */
-    int32_t pair_cap, pad0;
+    int32_t pair_cap;
+    int32_t literal;             /* 1: elastic pass in project.wgsl's literal
+                                    formulas; 0: the fused order of the GPU  */
 } o_scene;
 
 /* Splats of one scene; each splat rides one body's lattice. Outputs are SoA:

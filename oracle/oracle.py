@@ -45,7 +45,7 @@ class OScene(C.Structure):
         ("d_node", C.c_void_p), ("d_off", C.c_void_p),
         ("target", C.c_double * 3), ("target_vel", C.c_double * 3), ("response_time", C.c_double),
         ("key_out", C.c_void_p), ("pair_count", C.c_void_p), ("pairs", C.c_void_p),
-        ("pair_cap", C.c_int32), ("pad0", C.c_int32),
+        ("pair_cap", C.c_int32), ("literal", C.c_int32),
     ]
 
 
@@ -73,6 +73,7 @@ def load(precision="f32"):
     lib.o_elastic_color.argtypes = [sp, C.c_int, C.c_int]
     lib.o_deform_splats.argtypes = [sp, C.POINTER(OSplats)]
     lib.o_project_tet.argtypes = [C.c_void_p] * 4 + [C.c_double] * 4 + [C.c_void_p]
+    lib.o_project_kuhn_tet.argtypes = [C.c_void_p] * 3 + [C.c_int] + [C.c_double] * 5 + [C.c_int, C.c_void_p]
     lib.o_cell_residuals.argtypes = [sp, C.c_uint32, C.c_uint32, C.c_void_p]
     lib.o_real_size.restype = C.c_int
     assert lib.o_real_size() == (4 if precision == "f32" else 8)
@@ -95,10 +96,24 @@ def project_tet(p, prev, w, B, alpha_dev, alpha_hyd, gamma, relax=0.0, precision
     return out.reshape(4, 3)
 
 
+def project_kuhn_tet(p, prev, w, t, h, alpha_dev, alpha_hyd, gamma, relax=0.0, literal=False, precision="f64"):
+    """Kuhn tetrahedron t (corners in KUHN_TETS order) in the literal WGSL
+    formulas or in the fused order the GPU kernel evaluates."""
+    dt = np.float32 if precision == "f32" else np.float64
+    lib = load(precision)
+    p = np.ascontiguousarray(p, dt).reshape(12)
+    prev = np.ascontiguousarray(prev, dt).reshape(12)
+    w = np.ascontiguousarray(w, dt)
+    out = np.empty(12, dt)
+    lib.o_project_kuhn_tet(p.ctypes.data, prev.ctypes.data, w.ctypes.data, t, h, alpha_dev, alpha_hyd, gamma, relax,
+                           int(literal), out.ctypes.data)
+    return out.reshape(4, 3)
+
+
 class OracleScene:
     """A Scene's state and tables in the oracle's layout, in f32 or f64."""
 
-    def __init__(self, scene, precision="f32", pair_cap=0):
+    def __init__(self, scene, precision="f32", pair_cap=0, literal=False):
         self.scene = scene
         self.precision = precision
         self.dtype = np.float32 if precision == "f32" else np.float64
@@ -158,6 +173,7 @@ class OracleScene:
         sc.grid_dims[:] = dims
         sc.bucket_slots = 32
         sc.key_out = self.keys.ctypes.data
+        sc.literal = int(literal)
         if pair_cap:
             sc.pair_count = self.pair_count.ctypes.data
             sc.pairs = self.pairs.ctypes.data

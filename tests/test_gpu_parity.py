@@ -146,3 +146,38 @@ def test_batched_equals_alone_and_graph_equals_eager():
                 alone.set_target(a, *sc.drive.frame_target(f, sc.substeps * sc.dt))
             alone.frame_scene(a, 10, True)
         assert np.array_equal(alone.read_nodes(a)[0], got)
+
+
+def serve(requests, max_nodes, realtime=False):
+    nodes = max_nodes
+    cells = max(s.n_cells for s, _ in requests) * 4
+    w = native.World(max_scenes=8, max_bodies=8, max_nodes=nodes, max_cells=cells, max_driven=4096,
+                     max_splats=1024, persistent=512)
+    srv = native.Server(w, substeps=10, graph=True, keep_results=True)
+    ids = [srv.submit(sc, 0.0, frames) for sc, frames in requests]
+    rep = srv.run(realtime=realtime)
+    return srv, w, ids, rep
+
+
+def test_server_results_are_the_same_alone_and_in_the_mix():
+    from oracle.oracle import OracleScene
+    reqs = []
+    for i, side in enumerate([6, 10, 8, 12, 7, 9]):
+        sc = scenes.jelly_block(side, seed=40 + i, name=f"r{i}")
+        if i % 2:
+            sc.drive = scenes.handle_drive(sc, seed=i)
+        reqs.append((sc, 3 + i % 3))
+    budget = 2 * max(sc.n_nodes for sc, _ in reqs) + 16
+    srv, w, ids, rep = serve(reqs, budget)
+    print(rep)
+    assert rep["completed"] == len(reqs) and rep["max_active"] >= 2
+    assert rep["iterations"] < sum(f for _, f in reqs)   # frames of different scenes shared launches
+    for (sc, frames), rid in zip(reqs, ids):
+        mixed = srv.result(rid)
+        alone_srv, _, alone_ids, _ = serve([(sc, frames)], budget)
+        assert np.array_equal(alone_srv.result(alone_ids[0]), mixed)
+        if sc.drive is None:
+            o = OracleScene(sc, "f32")
+            for _ in range(frames):
+                o.frame(10)
+            assert rel_err(mixed, o.p) <= RTOL

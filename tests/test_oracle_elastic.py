@@ -249,3 +249,89 @@ def test_lattice_block_conserves_momentum(oracle):
     after = (o.p * mass).sum(axis=1)
     assert np.all(np.abs(after - before) < 1e-12)
     assert np.max(np.abs(o.p - o.q)) > 1e-4
+
+
+# ---- the fused evaluation order the GPU uses, pinned against the literal
+# WGSL formulas (project.wgsl:75-153) above.
+
+def kuhn(o, s, p, t, prev=None, relax=0.0, w=None, literal=False, precision="f64"):
+    return o.project_kuhn_tet(p, p if prev is None else prev, s["w"] if w is None else w, t, H, *s["c"], relax,
+                              literal=literal, precision=precision)
+
+
+@pytest.mark.parametrize("literal", [True, False])
+def test_both_forms_hold_rest_for_every_tet(oracle, literal):
+    for t in range(6):
+        for nu in RATIOS:
+            s = rest_tet(nu, t)
+            assert moved(s["p"], kuhn(oracle, s, s["p"], t, literal=literal)) < 1e-9
+
+
+def test_fused_form_equals_the_literal_formulas_in_f64(oracle):
+    rng = np.random.default_rng(12)
+    worst = 0.0
+    for trial in range(600):
+        t = trial % 6
+        s = rest_tet(RATIOS[trial % len(RATIOS)], t)
+        p = s["p"] + rng.normal(scale=0.2 * H, size=(4, 3))
+        prev = p - rng.normal(scale=0.05 * H, size=(4, 3))
+        w = s["w"] * rng.uniform(0.2, 2.0, 4)
+        if trial % 7 == 0:
+            w[trial % 4] = 0.0
+        relax = [0.0, 0.012 / DT][trial % 2]
+        a = kuhn(oracle, s, p, t, prev, relax, w, literal=True)
+        b = kuhn(oracle, s, p, t, prev, relax, w, literal=False)
+        worst = max(worst, np.max(np.abs(a - b)) / np.max(np.abs(a - p)))
+    assert worst < 1e-9
+
+
+def test_fused_form_conserves_momentum_and_leaves_inert_corners(oracle):
+    s = rest_tet(0.3, 2)
+    sq = s["p"] * np.array([1, 0.7, 1])
+    out = kuhn(oracle, s, sq, 2)
+    assert np.all(np.abs((out - sq).sum(axis=0)) < 1e-12)
+    w = s["w"].copy()
+    w[1] = 0
+    assert moved(sq[1:2], kuhn(oracle, s, sq, 2, w=w)[1:2]) == 0
+
+
+def test_fused_substep_is_within_the_parity_tolerance_of_the_literal_one(oracle):
+    # One substep of the 16^3 block: the GPU's f32 order against the literal
+    # WGSL formulas in f32 (<= 1e-5 relative), and both against f64.
+    sc = scenes.c1_scene(splats=0)
+    fused = oracle.OracleScene(sc, "f32", literal=False)
+    lit = oracle.OracleScene(sc, "f32", literal=True)
+    ref = oracle.OracleScene(sc, "f64", literal=True)
+    for o in (fused, lit, ref):
+        o.frame(10)
+    scale = np.max(np.abs(ref.p))
+    assert np.max(np.abs(fused.p - lit.p)) / scale <= 1e-5
+    e_f = np.max(np.abs(fused.p - ref.p)) / scale
+    e_l = np.max(np.abs(lit.p - ref.p)) / scale
+    print(f"f32 fused vs f64: {e_f:.3e}; f32 literal vs f64: {e_l:.3e}")
+    assert e_f < 1e-5 and e_l < 1e-5
+
+
+def test_fused_form_finds_the_same_collapse_barrier(oracle):
+    # The barrier bisection of elastic.test.js:539-567, run on the fused form.
+    s = rest_tet(0.3, 0)
+    rest = volume(s["p"])
+
+    def run_fused(p, substeps):
+        pos = np.array(p, float)
+        vel = np.zeros_like(pos)
+        for _ in range(substeps):
+            prev = pos.copy()
+            pos = kuhn(oracle, s, pos + vel * DT, 0, prev)
+            vel = (pos - prev) / DT
+        return pos
+
+    ok, lost = 1.0, 0.05
+    for _ in range(14):
+        mid = (ok + lost) / 2
+        r = volume(run_fused(s["p"] * mid, 3000)) / rest
+        if 0.5 < r < 2:
+            ok = mid
+        else:
+            lost = mid
+    assert abs(ok - collapse_scale(s["c"][2])) < 0.5 * 10 ** -2

@@ -1,12 +1,13 @@
 // kernels.hip -- the per-substep hot path as hand-written gfx950 kernels.
 //
-// Arithmetic follows the evaluation order the oracle defines (left-to-right
-// sums, `dot(a,a) = (ax*ax + ay*ay) + az*az`, WGSL matrix products column by
-// column, the mat3.js determinant), with -ffp-contract=off and IEEE divide and
-// square root, so a substep is bit-for-bit the oracle's f32 substep. Where a
-// reference formula multiplies by a constant 0 or +-1 (the Kuhn rest-shape
-// inverses, the tank plane normals) the kernels drop the zero terms and fold
-// the signs; in IEEE arithmetic that changes no finite result.
+// Arithmetic follows the evaluation order the oracle defines, with
+// -ffp-contract=off, IEEE divide and square root, and fused multiply-adds
+// only where written (fmaf), so a substep is bit-for-bit the oracle's f32
+// substep. The elastic pass uses the oracle's fused order (the Kuhn-path form
+// of F = Ds Dm^-1, fmaf in every product sum, one divide per block); the
+// other passes follow the WGSL formulas term by term, dropping products with
+// a constant 0 and folding constant signs (the tank plane normals), which
+// changes no finite result.
 //
 // Layout: nodes as SoA planes, one wavefront lane per cell / node / record,
 // workgroups over tiles (kernels.h). No MFMA: the work is 3x3 algebra per
@@ -26,128 +27,129 @@ __device__ __forceinline__ V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, 
 __device__ __forceinline__ V3 mul(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 __device__ __forceinline__ V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
 __device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ V3 cross(V3 a, V3 b)
-{
-    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
 
-// Kuhn tetrahedra (kuhn.js:28-35) and the adjugates of their unit-cube edge
-// matrices (every det is 1), [tet][col][row], computed at compile time.
+// Kuhn tetrahedra (kuhn.js:28-35). Each is a path of three axis steps from
+// corner 0 to corner 7 (corner c = (c&1, (c>>1)&1, (c>>2)&1)), so
+// F = Ds * Dm^-1 has the path's edge differences times 1/h as its columns,
+// axis by axis, and (dC/dF) * B^T hands each path corner the difference of
+// the per-axis gradients of the steps into and out of it. `path` lists the
+// corners in path order, `axis[j]` the axis of step j, `pos[i]` the path
+// position of the tet's i-th vertex; all computed at compile time.
 struct Tet {
-    int i[4];
-    int inv[3][3];
+    int v[4];
+    int path[4];
+    int axis[3];
+    int pos[4];
 };
-
-constexpr int corner_axis(int c, int a) { return (c >> a) & 1; }
 
 constexpr Tet make_tet(int a, int b, int c, int d)
 {
-    Tet t{ { a, b, c, d }, {} };
-    int m[9] = {};  // column-major edge matrix, m[col*3+row]
-    const int k[3] = { b, c, d };
-    for (int col = 0; col < 3; col++)
-        for (int row = 0; row < 3; row++) m[col * 3 + row] = corner_axis(k[col], row) - corner_axis(a, row);
-    const int inv[9] = { m[4] * m[8] - m[7] * m[5], m[7] * m[2] - m[1] * m[8], m[1] * m[5] - m[4] * m[2],
-                         m[6] * m[5] - m[3] * m[8], m[0] * m[8] - m[6] * m[2], m[3] * m[2] - m[0] * m[5],
-                         m[3] * m[7] - m[6] * m[4], m[6] * m[1] - m[0] * m[7], m[0] * m[4] - m[3] * m[1] };
-    for (int col = 0; col < 3; col++)
-        for (int row = 0; row < 3; row++) t.inv[col][row] = inv[col * 3 + row];
+    Tet t{ { a, b, c, d }, { 0, 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0, 0 } };
+    int at = 0;
+    for (int j = 0; j < 3; j++) {
+        for (int ax = 0; ax < 3; ax++) {
+            const int next = at | (1 << ax);
+            if (next != at && (next == a || next == b || next == c || next == d)) {
+                t.axis[j] = ax;
+                at = next;
+                break;
+            }
+        }
+        t.path[j + 1] = at;
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            if (t.path[j] == t.v[i]) t.pos[i] = j;
     return t;
 }
 
 constexpr Tet TETS[6] = { make_tet(0, 1, 3, 7), make_tet(0, 5, 1, 7), make_tet(0, 3, 2, 7),
                           make_tet(0, 2, 6, 7), make_tet(0, 4, 5, 7), make_tet(0, 6, 4, 7) };
 
-// sum_k sign_k * (x_k * s), left to right, the first non-zero term opening
-// the sum: the WGSL product with the zero entries (exact zeros) dropped.
-__device__ __forceinline__ V3 combine(const V3 *x, int s0, int s1, int s2, float s)
+__device__ __forceinline__ float dotf(V3 a, V3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+__device__ __forceinline__ float dotacc(V3 a, V3 b, float acc) { return fmaf(a.z, b.z, fmaf(a.y, b.y, fmaf(a.x, b.x, acc))); }
+__device__ __forceinline__ V3 crossf(V3 a, V3 b)
 {
-    V3 acc = mk(0.f, 0.f, 0.f);
-    bool open = false;
-    const int sg[3] = { s0, s1, s2 };
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        if (sg[k] == 0) continue;
-        const V3 t = mul(x[k], s);
-        if (!open) {
-            acc = sg[k] > 0 ? t : neg(t);
-            open = true;
-        } else {
-            acc = sg[k] > 0 ? add(acc, t) : sub(acc, t);
-        }
-    }
-    return acc;
+    return mk(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x)));
 }
 
-// xpbdCouple, xpbdCouple.wgsl:58-71.
-__device__ __forceinline__ void couple(float cD, float cH, float aD, float aH, float relax, float rD, float rH,
-                                       float gx, float gy, float gz, float &lD, float &lH)
-{
-    const float stiffened = 1.0f + relax;
-    const float a00 = gx * stiffened + aD;
-    const float a01 = gy * stiffened;
-    const float a11 = gz * stiffened + aH;
-    const float det = a00 * a11 - a01 * a01;
-    if (det < 1e-20f) {
-        lD = 0.f;
-        lH = 0.f;
-        return;
-    }
-    const float bD = -(cD + rD * relax);
-    const float bH = -(cH + rH * relax);
-    lD = (a11 * bD - a01 * bH) / det;
-    lH = (a00 * bH - a01 * bD) / det;
-}
-
-// One tetrahedron of project.wgsl:75-153.
+// One tetrahedron of project.wgsl:75-153 (the stable Neo-Hookean pair solved
+// as the 2x2 block of xpbdCouple.wgsl:58-71), in the fused evaluation order
+// the oracle's project_tet_fused defines.
 template <int T>
-__device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const float (&w)[8], float rs,
+__device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const float (&w)[8], float s,
                                             float gamma, float aD, float aH, float relax)
 {
     constexpr Tet tet = TETS[T];
-    constexpr int i0 = tet.i[0], i1 = tet.i[1], i2 = tet.i[2], i3 = tet.i[3];
-    const float w0 = w[i0], w1 = w[i1], w2 = w[i2], w3 = w[i3];
-    if (((w0 + w1) + w2) + w3 == 0.0f) return;
+    const float wt[4] = { w[tet.v[0]], w[tet.v[1]], w[tet.v[2]], w[tet.v[3]] };
+    if (((wt[0] + wt[1]) + wt[2]) + wt[3] == 0.0f) return;
 
-    const V3 e[3] = { sub(p[i1], p[i0]), sub(p[i2], p[i0]), sub(p[i3], p[i0]) };
-    // F[:,j] = sum_k e_k * B[j][k], B = inv * (1/h)
     V3 F[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) F[j] = combine(e, tet.inv[j][0], tet.inv[j][1], tet.inv[j][2], rs);
-
-    const float cDev = sqrtf((dot(F[0], F[0]) + dot(F[1], F[1])) + dot(F[2], F[2]));
-    // mat3Determinant (mat3.js:30-36) on the columns of F
-    const float cHyd = (F[0].x * (F[1].y * F[2].z - F[2].y * F[1].z) - F[1].x * (F[0].y * F[2].z - F[2].y * F[0].z) +
-                        F[2].x * (F[0].y * F[1].z - F[1].y * F[0].z)) -
-                       gamma;
-
-    const float k = cDev > 1e-9f ? 1.0f / cDev : 0.0f;
-    const V3 X[3] = { mul(F[0], k), mul(F[1], k), mul(F[2], k) };
-    const V3 H[3] = { cross(F[1], F[2]), cross(F[2], F[0]), cross(F[0], F[1]) };
-    // (X * B^T)[:,j] = sum_i X_i * B[i][j]
-    V3 d[4], h[4];
+    for (int j = 0; j < 3; j++) F[tet.axis[j]] = mul(sub(p[tet.path[j + 1]], p[tet.path[j]]), s);
+    float n2 = F[0].x * F[0].x;
+    n2 = fmaf(F[0].y, F[0].y, n2);
+    n2 = fmaf(F[0].z, F[0].z, n2);
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        d[j + 1] = combine(X, tet.inv[0][j], tet.inv[1][j], tet.inv[2][j], rs);
-        h[j + 1] = combine(H, tet.inv[0][j], tet.inv[1][j], tet.inv[2][j], rs);
+    for (int c = 1; c < 3; c++) {
+        n2 = fmaf(F[c].x, F[c].x, n2);
+        n2 = fmaf(F[c].y, F[c].y, n2);
+        n2 = fmaf(F[c].z, F[c].z, n2);
     }
-    d[0] = neg(add(add(d[1], d[2]), d[3]));
-    h[0] = neg(add(add(h[1], h[2]), h[3]));
-
-    const float gx = ((w0 * dot(d[0], d[0]) + w1 * dot(d[1], d[1])) + w2 * dot(d[2], d[2])) + w3 * dot(d[3], d[3]);
-    const float gy = ((w0 * dot(d[0], h[0]) + w1 * dot(d[1], h[1])) + w2 * dot(d[2], h[2])) + w3 * dot(d[3], h[3]);
-    const float gz = ((w0 * dot(h[0], h[0]) + w1 * dot(h[1], h[1])) + w2 * dot(h[2], h[2])) + w3 * dot(h[3], h[3]);
-
-    const V3 x0 = sub(p[i0], q[i0]), x1 = sub(p[i1], q[i1]), x2 = sub(p[i2], q[i2]), x3 = sub(p[i3], q[i3]);
-    const float rD = ((dot(d[0], x0) + dot(d[1], x1)) + dot(d[2], x2)) + dot(d[3], x3);
-    const float rH = ((dot(h[0], x0) + dot(h[1], x1)) + dot(h[2], x2)) + dot(h[3], x3);
-
-    float lD, lH;
-    couple(cDev, cHyd, aD, aH, relax, rD, rH, gx, gy, gz, lD, lH);
-    p[i0] = add(p[i0], mul(add(mul(d[0], lD), mul(h[0], lH)), w0));
-    p[i1] = add(p[i1], mul(add(mul(d[1], lD), mul(h[1], lH)), w1));
-    p[i2] = add(p[i2], mul(add(mul(d[2], lD), mul(h[2], lH)), w2));
-    p[i3] = add(p[i3], mul(add(mul(d[3], lD), mul(h[3], lH)), w3));
+    const float cDev = sqrtf(n2);
+    const V3 H[3] = { crossf(F[1], F[2]), crossf(F[2], F[0]), crossf(F[0], F[1]) };
+    const float cHyd = dotf(F[0], H[0]) - gamma;
+    const float k = cDev > 1e-9f ? 1.0f / cDev : 0.0f;
+    const float ks = k * s;
+    V3 gD[3], gH[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        gD[a] = mul(F[a], ks);
+        gH[a] = mul(H[a], s);
+    }
+    V3 d[4], h[4], x[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int j = tet.pos[i];
+        if (j == 0) {
+            d[i] = neg(gD[tet.axis[0]]);
+            h[i] = neg(gH[tet.axis[0]]);
+        } else if (j == 3) {
+            d[i] = gD[tet.axis[2]];
+            h[i] = gH[tet.axis[2]];
+        } else {
+            d[i] = sub(gD[tet.axis[j - 1]], gD[tet.axis[j]]);
+            h[i] = sub(gH[tet.axis[j - 1]], gH[tet.axis[j]]);
+        }
+        x[i] = sub(p[tet.v[i]], q[tet.v[i]]);
+    }
+    float gx = 0.f, gy = 0.f, gz = 0.f, rD = 0.f, rH = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        gx = fmaf(wt[i], dotf(d[i], d[i]), gx);
+        gy = fmaf(wt[i], dotf(d[i], h[i]), gy);
+        gz = fmaf(wt[i], dotf(h[i], h[i]), gz);
+        rD = dotacc(d[i], x[i], rD);
+        rH = dotacc(h[i], x[i], rH);
+    }
+    const float st = 1.0f + relax;
+    const float a00 = fmaf(gx, st, aD);
+    const float a01 = gy * st;
+    const float a11 = fmaf(gz, st, aH);
+    const float det = fmaf(a00, a11, -(a01 * a01));
+    if (det < 1e-20f) return;
+    const float bD = -fmaf(rD, relax, cDev);
+    const float bH = -fmaf(rH, relax, cHyd);
+    const float inv = 1.0f / det;
+    const float lD = fmaf(a11, bD, -(a01 * bH)) * inv;
+    const float lH = fmaf(a00, bH, -(a01 * bD)) * inv;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        V3 u = mul(d[i], lD);
+        u = mk(fmaf(h[i].x, lH, u.x), fmaf(h[i].y, lH, u.y), fmaf(h[i].z, lH, u.z));
+        V3 &pi = p[tet.v[i]];
+        pi = mk(fmaf(u.x, wt[i], pi.x), fmaf(u.y, wt[i], pi.y), fmaf(u.z, wt[i], pi.z));
+    }
 }
 
 // Tile scheduling. Interleaved: workgroup b takes tiles b, b+G, ... .

@@ -1,0 +1,290 @@
+// server.cpp -- continuous batching of simulation scenes on one world.
+//
+// Requests (a scene, a frame count, optional per-frame grab targets and an
+// arrival time) wait in arrival order. Every iteration the server admits the
+// requests that have arrived while the world's pools -- the HBM budget -- can
+// hold them, enqueues ONE batched frame of every admitted scene, and, while
+// the device works, admits newcomers on the upload stream; then it waits,
+// books frame latencies and retires the scenes that are done, freeing their
+// ranges for the next admissions. A scene's frame latency runs from the
+// moment the frame became eligible (the scene admitted, or its previous frame
+// done) to the completion of the batch that ran it. The reference steps one
+// scene per browser tab and has no counterpart.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/vsim.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// Owning copy of a vs_scene_desc and everything it points at.
+struct SceneCopy {
+    std::vector<vs_body_desc> bodies;
+    std::vector<std::vector<uint32_t>> cells, surface;
+    std::vector<std::vector<float>> rest, inv, mean, quat, scale;
+    std::vector<uint32_t> driven;
+    std::vector<float> offsets;
+    vs_scene_desc desc{};
+    uint64_t nodes = 0;
+
+    template <typename T>
+    static std::vector<T> take(const T *p, size_t n)
+    {
+        return p && n ? std::vector<T>(p, p + n) : std::vector<T>();
+    }
+
+    explicit SceneCopy(const vs_scene_desc &d)
+    {
+        desc = d;
+        bodies.assign(d.bodies, d.bodies + d.n_bodies);
+        for (vs_body_desc &b : bodies) {
+            const size_t nn = (size_t)(b.cell_dims[0] + 1) * (b.cell_dims[1] + 1) * (b.cell_dims[2] + 1);
+            nodes += nn;
+            cells.push_back(take(b.cells, b.n_cells));
+            surface.push_back(take(b.surface, b.n_surface));
+            rest.push_back(take(b.rest, 3 * nn));
+            inv.push_back(take(b.inv_mass, nn));
+            mean.push_back(take(b.splat_mean, 3 * (size_t)b.n_splats));
+            quat.push_back(take(b.splat_quat, 4 * (size_t)b.n_splats));
+            scale.push_back(take(b.splat_scale, 3 * (size_t)b.n_splats));
+        }
+        for (size_t i = 0; i < bodies.size(); i++) {
+            bodies[i].cells = cells[i].data();
+            bodies[i].surface = surface[i].data();
+            bodies[i].rest = rest[i].data();
+            bodies[i].inv_mass = inv[i].data();
+            bodies[i].splat_mean = mean[i].empty() ? nullptr : mean[i].data();
+            bodies[i].splat_quat = quat[i].empty() ? nullptr : quat[i].data();
+            bodies[i].splat_scale = scale[i].empty() ? nullptr : scale[i].data();
+        }
+        driven = take(d.driven, d.n_driven);
+        offsets = take(d.driven_offset, 3 * (size_t)d.n_driven);
+        desc.bodies = bodies.data();
+        desc.driven = driven.empty() ? nullptr : driven.data();
+        desc.driven_offset = offsets.empty() ? nullptr : offsets.data();
+    }
+};
+
+struct Request {
+    std::unique_ptr<SceneCopy> scene;
+    double arrival = 0;
+    uint32_t frames = 0, done = 0;
+    std::vector<float> targets;  // (frames + 1) x 3, or empty
+    int64_t scene_id = -1;
+    double admitted = -1, finished = -1, eligible = 0;
+    std::vector<float> result;   // final positions, 3 planes
+};
+
+}  // namespace
+
+struct vs_server {
+    vs_world *world = nullptr;
+    vs_server_config cfg{};
+    std::vector<Request> requests;
+    std::vector<float> latencies_ms;
+    std::string error;
+};
+
+extern "C" {
+
+int vs_server_create(vs_world *world, const vs_server_config *cfg, vs_server **out)
+{
+    if (!world || !cfg || !out || cfg->substeps == 0) return VS_E_INVALID;
+    vs_server *s = new vs_server();
+    s->world = world;
+    s->cfg = *cfg;
+    *out = s;
+    return VS_OK;
+}
+
+void vs_server_destroy(vs_server *s) { delete s; }
+
+const char *vs_server_error(const vs_server *s) { return s ? s->error.c_str() : "no server"; }
+
+int vs_server_submit(vs_server *s, const vs_scene_desc *desc, double arrival_s, uint32_t frames, const float *targets,
+                     uint32_t *request_id)
+{
+    if (!s || !desc || !frames || !(arrival_s >= 0)) return VS_E_INVALID;
+    if (!s->requests.empty() && arrival_s < s->requests.back().arrival) {
+        s->error = "requests must be submitted in arrival order";
+        return VS_E_INVALID;
+    }
+    Request r;
+    r.scene.reset(new SceneCopy(*desc));
+    r.arrival = arrival_s;
+    r.frames = frames;
+    if (targets) r.targets.assign(targets, targets + 3 * ((size_t)frames + 1));
+    s->requests.push_back(std::move(r));
+    if (request_id) *request_id = (uint32_t)(s->requests.size() - 1);
+    return VS_OK;
+}
+
+int vs_server_run(vs_server *s, int32_t realtime, vs_server_report *report)
+{
+    if (!s || !report) return VS_E_INVALID;
+    vs_world *w = s->world;
+    const auto t0 = Clock::now();
+    auto now = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+    std::deque<size_t> waiting;
+    std::vector<size_t> active;
+    size_t next = 0;
+    uint64_t node_substeps = 0, frames = 0;
+    uint32_t iterations = 0, max_active = 0;
+    s->latencies_ms.clear();
+
+    auto admit = [&](double t) -> int {
+        while (next < s->requests.size() && (!realtime || s->requests[next].arrival <= t)) waiting.push_back(next++);
+        while (!waiting.empty()) {
+            Request &r = s->requests[waiting.front()];
+            uint32_t id = 0;
+            const int rc = vs_world_add_scene(w, &r.scene->desc, &id);
+            if (rc == VS_E_NOMEM) break;  // the budget is full: wait for retirements
+            if (rc) {
+                s->error = std::string("admission failed: ") + vs_world_error(w);
+                return rc;
+            }
+            r.scene_id = id;
+            r.admitted = now();
+            r.eligible = r.admitted;
+            active.push_back(waiting.front());
+            waiting.pop_front();
+        }
+        return VS_OK;
+    };
+
+    if (int rc = admit(now())) return rc;
+    while (next < s->requests.size() || !waiting.empty() || !active.empty()) {
+        if (active.empty()) {
+            if (waiting.empty() && realtime && next < s->requests.size()) {
+                const double wait = s->requests[next].arrival - now();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            }
+            if (int rc = admit(now())) return rc;
+            if (active.empty()) {
+                if (!waiting.empty()) {
+                    s->error = "a waiting scene does not fit the empty world";
+                    return VS_E_NOMEM;
+                }
+                continue;
+            }
+        }
+        // Grab targets of this frame.
+        for (size_t k : active) {
+            Request &r = s->requests[k];
+            if (r.targets.empty()) continue;
+            const float *cur = &r.targets[3 * (size_t)(r.done + 1)], *prev = &r.targets[3 * (size_t)r.done];
+            const double fdt = (double)s->cfg.substeps * (double)r.scene->desc.dt;
+            float vel[3];
+            for (int a = 0; a < 3; a++) vel[a] = (float)(((double)cur[a] - (double)prev[a]) / fdt);
+            if (int rc = vs_world_set_target(w, (uint32_t)r.scene_id, cur, vel)) return rc;
+        }
+        const std::vector<size_t> batch = active;
+        if (int rc = vs_world_frame(w, s->cfg.substeps, s->cfg.splats, s->cfg.graph)) {
+            s->error = vs_world_error(w);
+            return rc;
+        }
+        // Newcomers are admitted while the device runs the frame; their first
+        // frame is the next batch.
+        if (int rc = admit(now())) return rc;
+        if (int rc = vs_world_synchronize(w)) return rc;
+        const double done_t = now();
+        iterations++;
+        max_active = std::max<uint32_t>(max_active, (uint32_t)batch.size());
+        std::vector<size_t> still;
+        for (size_t k : batch) {
+            Request &r = s->requests[k];
+            s->latencies_ms.push_back((float)((done_t - r.eligible) * 1e3));
+            r.eligible = done_t;
+            r.done++;
+            frames++;
+            node_substeps += r.scene->nodes * s->cfg.substeps;
+            if (r.done < r.frames) {
+                still.push_back(k);
+                continue;
+            }
+            r.finished = done_t;
+            if (s->cfg.keep_results) {
+                r.result.resize(3 * r.scene->nodes);
+                if (int rc = vs_world_read_nodes(w, (uint32_t)r.scene_id, r.result.data(), nullptr)) return rc;
+            }
+            if (int rc = vs_world_remove_scene(w, (uint32_t)r.scene_id)) return rc;
+            r.scene_id = -1;
+        }
+        for (size_t k : active)
+            if (std::find(batch.begin(), batch.end(), k) == batch.end()) still.push_back(k);
+        active = still;
+        if (int rc = admit(now())) return rc;
+    }
+    const double wall = now();
+    std::vector<float> sorted = s->latencies_ms;
+    std::sort(sorted.begin(), sorted.end());
+    auto pct = [&](double q) -> double {
+        if (sorted.empty()) return 0;
+        const size_t i = std::min(sorted.size() - 1, (size_t)std::ceil(q * (double)sorted.size()) - (q > 0 ? 1 : 0));
+        return sorted[i];
+    };
+    double qsum = 0, qmax = 0, msum = 0;
+    for (const Request &r : s->requests) {
+        const double q = r.admitted - (realtime ? r.arrival : 0.0);
+        qsum += q;
+        qmax = std::max(qmax, q);
+    }
+    for (float v : sorted) msum += v;
+    *report = vs_server_report{};
+    report->wall_s = wall;
+    report->node_substeps = node_substeps;
+    report->frames = frames;
+    report->requests = (uint32_t)s->requests.size();
+    report->iterations = iterations;
+    report->max_active = max_active;
+    report->p50_ms = pct(0.50);
+    report->p99_ms = pct(0.99);
+    report->mean_ms = sorted.empty() ? 0 : msum / (double)sorted.size();
+    report->max_ms = sorted.empty() ? 0 : sorted.back();
+    report->mean_queue_s = s->requests.empty() ? 0 : qsum / (double)s->requests.size();
+    report->max_queue_s = qmax;
+    uint32_t completed = 0;
+    for (const Request &r : s->requests) completed += r.finished >= 0;
+    report->completed = completed;
+    return VS_OK;
+}
+
+int vs_server_latencies(const vs_server *s, float *ms, uint64_t cap, uint64_t *count)
+{
+    if (!s || !count) return VS_E_INVALID;
+    *count = s->latencies_ms.size();
+    if (ms) std::memcpy(ms, s->latencies_ms.data(), sizeof(float) * std::min<uint64_t>(cap, *count));
+    return VS_OK;
+}
+
+int vs_server_result(const vs_server *s, uint32_t request_id, float *pos)
+{
+    if (!s || request_id >= s->requests.size() || !pos) return VS_E_INVALID;
+    const Request &r = s->requests[request_id];
+    if (r.result.empty()) return VS_E_INVALID;
+    std::memcpy(pos, r.result.data(), sizeof(float) * r.result.size());
+    return VS_OK;
+}
+
+int vs_server_times(const vs_server *s, uint32_t request_id, double *arrival, double *admitted, double *finished)
+{
+    if (!s || request_id >= s->requests.size()) return VS_E_INVALID;
+    const Request &r = s->requests[request_id];
+    if (arrival) *arrival = r.arrival;
+    if (admitted) *admitted = r.admitted;
+    if (finished) *finished = r.finished;
+    return VS_OK;
+}
+
+}  // extern "C"

@@ -340,7 +340,10 @@ uint32_t grid_for(vs_world *w, int p, int scene)
 {
     const uint32_t n = scene < 0 ? w->tile_count[p] : w->scene_ntiles[p][scene];
     if (n == 0) return 0;
-    if (scene < 0 && w->cfg.persistent && p >= P_CELL && p < P_CELL + 8) return std::min(n, w->cfg.persistent);
+    // Persistent: a fixed grid walks the table, so a captured frame stays
+    // valid whatever the admitted scenes (only a pass appearing or vanishing
+    // changes the shape).
+    if (scene < 0 && w->cfg.persistent) return w->cfg.persistent;
     return n;
 }
 

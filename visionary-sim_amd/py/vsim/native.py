@@ -94,6 +94,19 @@ class VsWorldStats(C.Structure):
     ]
 
 
+class VsServerConfig(C.Structure):
+    _fields_ = [("substeps", C.c_uint32), ("splats", C.c_int32), ("graph", C.c_int32), ("keep_results", C.c_int32)]
+
+
+class VsServerReport(C.Structure):
+    _fields_ = [
+        ("wall_s", C.c_double), ("node_substeps", C.c_uint64), ("frames", C.c_uint64),
+        ("requests", C.c_uint32), ("completed", C.c_uint32), ("iterations", C.c_uint32), ("max_active", C.c_uint32),
+        ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+        ("mean_queue_s", C.c_double), ("max_queue_s", C.c_double),
+    ]
+
+
 EXPORTS = [
     "vs_backend_create", "vs_backend_destroy", "vs_last_error", "vs_buffer_alloc", "vs_buffer_free",
     "vs_buffer_write", "vs_buffer_read", "vs_kernel_id", "vs_dispatch", "vs_synchronize",
@@ -102,6 +115,8 @@ EXPORTS = [
     "vs_world_synchronize", "vs_world_record", "vs_world_elapsed", "vs_world_read_nodes",
     "vs_world_read_splats", "vs_world_log_pairs", "vs_world_read_contacts", "vs_world_stats_get",
     "vs_world_autotune", "vs_world_profile",
+    "vs_server_create", "vs_server_destroy", "vs_server_error", "vs_server_submit", "vs_server_run",
+    "vs_server_latencies", "vs_server_result", "vs_server_times",
 ]
 
 PASS_KINDS = ["predict", "drive", "elastic", "bin", "sort", "collide", "settle", "finalize", "splat"]
@@ -148,6 +163,14 @@ def load(path=LIB_PATH):
         "vs_world_stats_get": (C.c_int, [p, C.POINTER(VsWorldStats)]),
         "vs_world_autotune": (C.c_int, [p, u32, C.c_char_p, u32]),
         "vs_world_profile": (C.c_int, [p, u32, i32, p, p]),
+        "vs_server_create": (C.c_int, [p, C.POINTER(VsServerConfig), C.POINTER(p)]),
+        "vs_server_destroy": (None, [p]),
+        "vs_server_error": (C.c_char_p, [p]),
+        "vs_server_submit": (C.c_int, [p, C.POINTER(VsSceneDesc), C.c_double, u32, p, C.POINTER(u32)]),
+        "vs_server_run": (C.c_int, [p, i32, C.POINTER(VsServerReport)]),
+        "vs_server_latencies": (C.c_int, [p, p, u64, C.POINTER(u64)]),
+        "vs_server_result": (C.c_int, [p, u32, p]),
+        "vs_server_times": (C.c_int, [p, u32, p, p, p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -291,3 +314,72 @@ class World:
         buf = C.create_string_buffer(1 << 16)
         self._check(self.lib.vs_world_autotune(self.h, substeps, buf, len(buf)), "autotune")
         return buf.value.decode()
+
+
+class Server:
+    """A vs_server: continuous batching of scene requests on a World."""
+
+    def __init__(self, world, substeps=10, splats=False, graph=True, keep_results=False):
+        self.lib = world.lib
+        self.world = world
+        cfg = VsServerConfig(substeps=substeps, splats=int(splats), graph=int(graph), keep_results=int(keep_results))
+        h = C.c_void_p()
+        rc = self.lib.vs_server_create(world.h, C.byref(cfg), C.byref(h))
+        if rc != VS_OK:
+            raise VsimError(f"vs_server_create failed ({rc})")
+        self.h = h
+        self.requests = []
+
+    def close(self):
+        if self.h:
+            self.lib.vs_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, scene, arrival, frames):
+        desc, keep = scene.to_desc()
+        targets = None
+        if scene.drive is not None:
+            fdt = scene.substeps * scene.dt
+            targets = np.array([scene.drive.target(f * fdt) for f in range(frames + 1)], np.float32)
+        rid = C.c_uint32()
+        rc = self.lib.vs_server_submit(self.h, C.byref(desc), float(arrival), frames,
+                                       targets.ctypes.data if targets is not None else None, C.byref(rid))
+        del keep
+        if rc != VS_OK:
+            raise VsimError(f"submit: {self.lib.vs_server_error(self.h).decode()} ({rc})")
+        self.requests.append(scene)
+        return rid.value
+
+    def run(self, realtime=True):
+        rep = VsServerReport()
+        rc = self.lib.vs_server_run(self.h, int(realtime), C.byref(rep))
+        if rc != VS_OK:
+            raise VsimError(f"run: {self.lib.vs_server_error(self.h).decode()} / "
+                            f"{self.lib.vs_world_error(self.world.h).decode()} ({rc})")
+        return {k: getattr(rep, k) for k, _ in VsServerReport._fields_}
+
+    def latencies(self):
+        n = C.c_uint64()
+        self.lib.vs_server_latencies(self.h, None, 0, C.byref(n))
+        out = np.empty(n.value, np.float32)
+        self.lib.vs_server_latencies(self.h, out.ctypes.data, n.value, C.byref(n))
+        return out
+
+    def result(self, rid):
+        sc = self.requests[rid]
+        pos = np.empty((3, sc.n_nodes), np.float32)
+        rc = self.lib.vs_server_result(self.h, rid, pos.ctypes.data)
+        if rc != VS_OK:
+            raise VsimError("no result kept for request %d" % rid)
+        return pos
+
+    def times(self, rid):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self.lib.vs_server_times(self.h, rid, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
