@@ -34,6 +34,18 @@ FINALIZE_BYTES_PER_NODE = 52
 SPLAT_BYTES = 76
 
 
+def elastic_traffic_bytes():
+    """HBM bytes per batched colour-pass launch from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/batched256_hbm_traffic.md, scaled
+    by the copy calibration), or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "batched256_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    rows = [line.split("|") for line in open(path) if line.startswith("| k_elastic")]
+    totals = [float(r[6]) for r in rows if len(r) > 6]
+    return sum(totals) / len(totals) * 1e6 if totals else None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -178,6 +190,8 @@ def main():
     tune = None
     if args.autotune:
         tune = json.loads(world.autotune(substeps))
+        for c in tune["candidates"]:
+            log(f"[bench] autotune {c}")
         log(f"[bench] autotune best {tune['best']}")
 
     # ---- batched, graph-replayed frames
@@ -233,7 +247,7 @@ def main():
         log(f"[bench] cpu baseline: {cpu['value']:.4e} node-substeps/s ({cpu['sample']})")
 
     out = {
-        "metric": "node-substeps/s (batched; plain per-scene beside it), f32 at <=1e-5 rel vs the f32 oracle",
+        "metric": "node-substeps/s, batched vs plain, at \u22641e-5 rel error; p50/p99 frame latency",
         "value": value,
         "unit": "node-substeps/s",
         "n_gpus": 1,
@@ -252,7 +266,7 @@ def main():
         "nodes": nodes, "cells": cells, "splats": splats, "substeps_per_frame": substeps,
         "profile_ms": {k: {"total": v[0], "launches": v[1]} for k, v in prof.items() if v[1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": elastic_traffic_bytes(),
                      "kernel": "k_elastic (colour pass)", "bytes_per_launch": el_bytes,
                      "avg_launch_us": el_avg_s * 1e6},
         "serving": serving,

@@ -248,6 +248,7 @@ int vs_dispatch(vs_backend *b, const vs_launch *launches, uint32_t count)
             return fail(b, VS_E_INVALID, std::string(name) + ": invalid dispatch " + std::to_string(l.grid) + "," +
                                              std::to_string(block));
         }
+        if (l.plane >= (1u << 29)) return fail(b, VS_E_INVALID, std::string(name) + ": plane exceeds 2^29 floats");
         const uint32_t need = required_slots(l.kernel);
         for (int s = 0; s < VS_B_COUNT; s++) {
             const bool wanted = (need >> s) & 1u;

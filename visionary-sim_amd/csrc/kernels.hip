@@ -177,6 +177,27 @@ __device__ __forceinline__ float edge_len(int a, int b)
     return d == 1 ? 1.0f : d == 2 ? 1.41421353816986083984375f : 1.73205077648162841796875f;
 }
 
+// A cell's corners c and c|1 are x-neighbours, adjacent in every node plane,
+// so the eight corners are four 8-byte pairs per plane. Lanes of a wave hold
+// consecutive cells of one colour (every other cell along x), so their pairs
+// tile a node row without gaps: each pair access is one contiguous span per
+// row instead of two half-used strided ones. Pairs start on odd nodes for
+// odd x-parity colours; the vector type is declared 4-byte aligned, which
+// the hardware serves with dword alignment.
+typedef float F2 __attribute__((ext_vector_type(2), aligned(4)));
+
+__device__ __forceinline__ F2 ld2(const float *plane, uint32_t off)
+{
+    return *reinterpret_cast<const F2 *>(reinterpret_cast<const char *>(plane) + off);
+}
+__device__ __forceinline__ void st2(float *plane, uint32_t off, float a, float b)
+{
+    F2 v;
+    v.x = a;
+    v.y = b;
+    *reinterpret_cast<F2 *>(reinterpret_cast<char *>(plane) + off) = v;
+}
+
 __device__ __forceinline__ V3 ld3(const float *x, const float *y, const float *z, uint32_t n)
 {
     return mk(x[n], y[n], z[n]);
@@ -208,15 +229,23 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
             const uint32_t lin = cells[tile.begin + i];
             const uint32_t cx = lin % cdx, cy = (lin / cdx) % cdy, cz = lin / (cdx * cdy);
             const uint32_t n0 = bd.node_offset + cx + ndx * cy + ndxy * cz;
-            uint32_t ids[8];
+            // Byte offsets of the four x-pairs: pair k holds corners 2k, 2k+1.
+            uint32_t off[4];
             V3 p[8], q[8];
             float w[8];
 #pragma unroll
-            for (int c = 0; c < 8; c++) {
-                ids[c] = n0 + (uint32_t)(c & 1) + ((c >> 1) & 1 ? ndx : 0u) + ((c >> 2) & 1 ? ndxy : 0u);
-                p[c] = ld3(n.px, n.py, n.pz, ids[c]);
-                q[c] = ld3(n.qx, n.qy, n.qz, ids[c]);
-                w[c] = n.w[ids[c]];
+            for (int k = 0; k < 4; k++) off[k] = 4u * (n0 + (k & 1 ? ndx : 0u) + (k & 2 ? ndxy : 0u));
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const F2 x = ld2(n.px, off[k]), y = ld2(n.py, off[k]), z = ld2(n.pz, off[k]);
+                const F2 u = ld2(n.qx, off[k]), v = ld2(n.qy, off[k]), t = ld2(n.qz, off[k]);
+                const F2 m = ld2(n.w, off[k]);
+                p[2 * k] = mk(x.x, y.x, z.x);
+                p[2 * k + 1] = mk(x.y, y.y, z.y);
+                q[2 * k] = mk(u.x, v.x, t.x);
+                q[2 * k + 1] = mk(u.y, v.y, t.y);
+                w[2 * k] = m.x;
+                w[2 * k + 1] = m.y;
             }
             if (!limit_only) {
                 project_tet<0>(p, q, w, rs, bd.gamma, aD, aH, relax);
@@ -247,10 +276,10 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
                 }
             }
 #pragma unroll
-            for (int c = 0; c < 8; c++) {
-                n.px[ids[c]] = p[c].x;
-                n.py[ids[c]] = p[c].y;
-                n.pz[ids[c]] = p[c].z;
+            for (int k = 0; k < 4; k++) {
+                st2(n.px, off[k], p[2 * k].x, p[2 * k + 1].x);
+                st2(n.py, off[k], p[2 * k].y, p[2 * k + 1].y);
+                st2(n.pz, off[k], p[2 * k].z, p[2 * k + 1].z);
             }
         }
     }

@@ -482,6 +482,8 @@ static int world_init(vs_world *w)
 {
     vs_world_config &c = w->cfg;
     if (c.max_scenes == 0 || c.max_bodies == 0 || c.max_nodes == 0) return VS_E_INVALID;
+    // Kernels address node planes by 32-bit byte offsets.
+    if (c.max_nodes >= (1ull << 29)) return fail(w, VS_E_INVALID, "max_nodes must stay below 2^29");
     if (c.max_cells == 0) c.max_cells = 1;
     if (c.max_surface == 0) c.max_surface = 1;
     if (c.max_buckets == 0) c.max_buckets = 1;
