@@ -289,6 +289,12 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out);
  * events around every launch: ms[k] is the summed duration of the launches of
  * kind k, launches[k] their number. Synchronous. */
 int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, uint32_t *launches);
+/* Times `rounds` back-to-back batched launches of one pass kind (elastic:
+ * one round = the eight colour passes) with a HIP event before the first and
+ * after the last, so *ms / *launches is the kernel's average duration with no
+ * per-launch event in between. Node state is saved first and restored after,
+ * so the simulation is left where it was. Synchronous. */
+int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, uint32_t *launches);
 /* Re-times the colour-pass launch shapes on the admitted scenes and keeps the
  * fastest (block size x cells per tile x persistent grid x XCD mapping). */
 int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t report_bytes);

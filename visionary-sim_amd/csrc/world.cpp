@@ -367,65 +367,103 @@ struct Profile {
     }
 };
 
+// Enqueues one launch of a pass over the whole table (scene < 0) or one
+// scene's slice: `color` picks the elastic colour, `limit_only` the
+// stretch-only sweeps. A pass with no tiles enqueues nothing.
+int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_only)
+{
+    using namespace vsim;
+    auto shape = [&](int p) { return Shape{ grid_for(w, p, scene), pass_block(w, p), scene < 0 ? w->cfg.xcd_chunked : 0u }; };
+    const NodeView n = node_view(w);
+    hipStream_t st = w->stream;
+    Shape sh{};
+    switch (kind) {
+    case VS_P_PREDICT:
+        sh = shape(P_NODE);
+        if (sh.grid) HIPW(w, launch_predict(st, sh, n, w->d_scenes, tile_view(w, P_NODE, scene)));
+        return VS_OK;
+    case VS_P_DRIVE:
+        sh = shape(P_DRIVE);
+        if (sh.grid) HIPW(w, launch_drive(st, sh, n, w->d_scenes, drive_view(w), tile_view(w, P_DRIVE, scene), -1));
+        return VS_OK;
+    case VS_P_ELASTIC:
+        sh = shape(P_CELL + color);
+        if (sh.grid)
+            HIPW(w, launch_elastic(st, sh, n, w->d_bodies, w->d_scenes, w->d_cells, tile_view(w, P_CELL + color, scene),
+                                   limit_only));
+        return VS_OK;
+    case VS_P_FINALIZE:
+        sh = shape(P_NODE);
+        if (sh.grid) HIPW(w, launch_finalize(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_NODE, scene)));
+        return VS_OK;
+    case VS_P_SPLAT:
+        sh = shape(P_SPLAT);
+        if (sh.grid)
+            HIPW(w, launch_splats(st, sh, n.px, n.py, n.pz, w->d_bodies, splat_view(w), tile_view(w, P_SPLAT, scene)));
+        return VS_OK;
+    default:
+        break;
+    }
+    sh = shape(P_SURF);
+    if (!sh.grid) return VS_OK;
+    const SurfaceView sv = surf_view(w);
+    const GridView g = grid_view(w);
+    const TileView t = tile_view(w, P_SURF, scene);
+    switch (kind) {
+    case VS_P_BIN: HIPW(w, launch_contact_bin(st, sh, n, w->d_scenes, sv, g, t)); break;
+    case VS_P_SORT: HIPW(w, launch_contact_sort(st, sh, w->d_scenes, sv, g, t)); break;
+    case VS_P_COLLIDE: {
+        const PairLog log{ w->d_pairlog,
+                           w->d_pairlog ? reinterpret_cast<uint32_t *>(w->d_pairlog + w->cfg.max_surface) : nullptr,
+                           w->pair_cap };
+        HIPW(w, launch_contact_collide(st, sh, n, w->d_bodies, w->d_scenes, sv, g, t, log));
+        break;
+    }
+    case VS_P_SETTLE: HIPW(w, launch_contact_settle(st, sh, n, w->d_scenes, sv, g, t)); break;
+    default: return fail(w, VS_E_INVALID, "unknown pass kind");
+    }
+    return VS_OK;
+}
+
 // Enqueues one frame: `substeps` substeps in Stepper.sequence order
 // (stepper.js:101-108, solve order simulator.js:83), then the splats.
 int enqueue_frame(vs_world *w, uint32_t substeps, bool splats, int scene, Profile *prof = nullptr)
 {
-    using namespace vsim;
-    NodeView n = node_view(w);
-    SurfaceView sv = surf_view(w);
-    GridView g = grid_view(w);
-    PairLog log{ w->d_pairlog, w->d_pairlog ? reinterpret_cast<uint32_t *>(w->d_pairlog + w->cfg.max_surface) : nullptr,
-                 w->pair_cap };
     bool stretch = w->any_stretch, contact = w->any_contact;
     if (scene >= 0) {
         stretch = w->recs[scene].stretch;
         contact = w->recs[scene].contact;
     }
-    auto shape = [&](int p) { return Shape{ grid_for(w, p, scene), pass_block(w, p), scene < 0 ? w->cfg.xcd_chunked : 0u }; };
     hipStream_t st = w->stream;
-#define TIMED(KIND, CALL)                                                   \
-    do {                                                                    \
-        if (prof) {                                                         \
-            prof->marks.emplace_back((KIND), prof->used);                   \
-            HIPW(w, hipEventRecord(prof->next(), st));                      \
-        }                                                                   \
-        HIPW(w, CALL);                                                      \
-        if (prof) HIPW(w, hipEventRecord(prof->next(), st));                \
-    } while (0)
+    // With a profile, events bracket every launch of a non-empty pass.
+    auto pass = [&](int kind, int color, uint32_t limit_only) -> int {
+        const int p = kind == VS_P_ELASTIC ? P_CELL + color
+                      : kind == VS_P_DRIVE ? P_DRIVE
+                      : kind == VS_P_SPLAT ? P_SPLAT
+                      : (kind == VS_P_PREDICT || kind == VS_P_FINALIZE) ? P_NODE
+                                                                         : P_SURF;
+        const bool timed = prof && grid_for(w, p, scene) != 0;
+        if (timed) {
+            prof->marks.emplace_back(kind, prof->used);
+            HIPW(w, hipEventRecord(prof->next(), st));
+        }
+        if (int rc = enqueue_pass(w, kind, color, scene, limit_only)) return rc;
+        if (timed) HIPW(w, hipEventRecord(prof->next(), st));
+        return VS_OK;
+    };
     for (uint32_t s = 0; s < substeps; s++) {
-        Shape sh = shape(P_NODE);
-        if (sh.grid) TIMED(VS_P_PREDICT, launch_predict(st, sh, n, w->d_scenes, tile_view(w, P_NODE, scene)));
-        sh = shape(P_DRIVE);
-        if (sh.grid)
-            TIMED(VS_P_DRIVE, launch_drive(st, sh, n, w->d_scenes, drive_view(w), tile_view(w, P_DRIVE, scene), -1));
-        for (int sweep = 0; sweep < (stretch ? 4 : 1); sweep++) {
-            for (int c = 0; c < 8; c++) {
-                sh = shape(P_CELL + c);
-                if (sh.grid)
-                    TIMED(VS_P_ELASTIC, launch_elastic(st, sh, n, w->d_bodies, w->d_scenes, w->d_cells,
-                                                       tile_view(w, P_CELL + c, scene), sweep > 0 ? 1u : 0u));
-            }
-        }
-        sh = shape(P_SURF);
-        if (contact && sh.grid) {
-            TileView t = tile_view(w, P_SURF, scene);
-            TIMED(VS_P_BIN, launch_contact_bin(st, sh, n, w->d_scenes, sv, g, t));
-            TIMED(VS_P_SORT, launch_contact_sort(st, sh, w->d_scenes, sv, g, t));
-            TIMED(VS_P_COLLIDE, launch_contact_collide(st, sh, n, w->d_bodies, w->d_scenes, sv, g, t, log));
-            TIMED(VS_P_SETTLE, launch_contact_settle(st, sh, n, w->d_scenes, sv, g, t));
-        }
-        sh = shape(P_NODE);
-        if (sh.grid)
-            TIMED(VS_P_FINALIZE, launch_finalize(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_NODE, scene)));
+        if (int rc = pass(VS_P_PREDICT, 0, 0)) return rc;
+        if (int rc = pass(VS_P_DRIVE, 0, 0)) return rc;
+        for (int sweep = 0; sweep < (stretch ? 4 : 1); sweep++)
+            for (int c = 0; c < 8; c++)
+                if (int rc = pass(VS_P_ELASTIC, c, sweep > 0 ? 1u : 0u)) return rc;
+        if (contact)
+            for (int kind : { VS_P_BIN, VS_P_SORT, VS_P_COLLIDE, VS_P_SETTLE })
+                if (int rc = pass(kind, 0, 0)) return rc;
+        if (int rc = pass(VS_P_FINALIZE, 0, 0)) return rc;
     }
-    if (splats) {
-        Shape sh = shape(P_SPLAT);
-        if (sh.grid)
-            TIMED(VS_P_SPLAT, launch_splats(st, sh, n.px, n.py, n.pz, w->d_bodies, splat_view(w),
-                                            tile_view(w, P_SPLAT, scene)));
-    }
-#undef TIMED
+    if (splats)
+        if (int rc = pass(VS_P_SPLAT, 0, 0)) return rc;
     return VS_OK;
 }
 
@@ -1043,6 +1081,47 @@ int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, 
         ms[m.first] += t;
         launches[m.first]++;
     }
+    return VS_OK;
+}
+
+int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, uint32_t *launches)
+{
+    if (!w || !ms || !launches || rounds == 0 || kind < 0 || kind >= VS_P_COUNT) return VS_E_INVALID;
+    HIPW(w, hipSetDevice(w->cfg.device));
+    if (int rc = prepare(w)) return rc;
+    const int p = kind == VS_P_ELASTIC ? P_CELL
+                  : kind == VS_P_DRIVE ? P_DRIVE
+                  : kind == VS_P_SPLAT ? P_SPLAT
+                  : (kind == VS_P_PREDICT || kind == VS_P_FINALIZE) ? P_NODE
+                                                                     : P_SURF;
+    uint32_t per_round = 0;
+    for (int c = 0; c < (kind == VS_P_ELASTIC ? 8 : 1); c++) per_round += grid_for(w, p + c, -1) != 0;
+    *ms = 0.f;
+    *launches = 0;
+    if (per_round == 0) return VS_OK;
+    const size_t bytes = sizeof(float) * 10 * w->cfg.max_nodes;
+    void *snap = nullptr;
+    HIPW(w, hipMalloc(&snap, bytes));
+    int rc = VS_OK;
+    hipEvent_t a = w->tune[0], b = w->tune[1];
+    auto run = [&]() -> int {
+        HIPW(w, hipMemcpyAsync(snap, w->d_nodes, bytes, hipMemcpyDeviceToDevice, w->stream));
+        HIPW(w, hipEventRecord(a, w->stream));
+        int e = VS_OK;
+        for (uint32_t r = 0; r < rounds && !e; r++)
+            for (int c = 0; c < (kind == VS_P_ELASTIC ? 8 : 1) && !e; c++) e = enqueue_pass(w, kind, c, -1, 0);
+        HIPW(w, hipEventRecord(b, w->stream));
+        // Restored whether or not every launch went in.
+        HIPW(w, hipMemcpyAsync(w->d_nodes, snap, bytes, hipMemcpyDeviceToDevice, w->stream));
+        HIPW(w, hipStreamSynchronize(w->stream));
+        if (e) return e;
+        HIPW(w, hipEventElapsedTime(ms, a, b));
+        return VS_OK;
+    };
+    rc = run();
+    (void)hipFree(snap);
+    if (rc) return rc;
+    *launches = rounds * per_round;
     return VS_OK;
 }
 

@@ -114,7 +114,7 @@ EXPORTS = [
     "vs_world_reset_scene", "vs_world_set_target", "vs_world_frame", "vs_world_frame_scene",
     "vs_world_synchronize", "vs_world_record", "vs_world_elapsed", "vs_world_read_nodes",
     "vs_world_read_splats", "vs_world_log_pairs", "vs_world_read_contacts", "vs_world_stats_get",
-    "vs_world_autotune", "vs_world_profile",
+    "vs_world_autotune", "vs_world_profile", "vs_world_time_pass",
     "vs_server_create", "vs_server_destroy", "vs_server_error", "vs_server_submit", "vs_server_run",
     "vs_server_latencies", "vs_server_result", "vs_server_times",
 ]
@@ -163,6 +163,7 @@ def load(path=LIB_PATH):
         "vs_world_stats_get": (C.c_int, [p, C.POINTER(VsWorldStats)]),
         "vs_world_autotune": (C.c_int, [p, u32, C.c_char_p, u32]),
         "vs_world_profile": (C.c_int, [p, u32, i32, p, p]),
+        "vs_world_time_pass": (C.c_int, [p, i32, u32, C.POINTER(C.c_float), C.POINTER(u32)]),
         "vs_server_create": (C.c_int, [p, C.POINTER(VsServerConfig), C.POINTER(p)]),
         "vs_server_destroy": (None, [p]),
         "vs_server_error": (C.c_char_p, [p]),
@@ -309,6 +310,14 @@ class World:
         n = np.zeros(len(PASS_KINDS), np.uint32)
         self._check(self.lib.vs_world_profile(self.h, substeps, int(splats), ms.ctypes.data, n.ctypes.data), "profile")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(PASS_KINDS)}
+
+    def time_pass(self, kind, rounds):
+        """(total ms, launches) of `rounds` back-to-back batched launches of one
+        pass kind (elastic: eight colours per round); state is restored."""
+        ms, n = C.c_float(), C.c_uint32()
+        self._check(self.lib.vs_world_time_pass(self.h, PASS_KINDS.index(kind), rounds, C.byref(ms), C.byref(n)),
+                    "time_pass")
+        return float(ms.value), int(n.value)
 
     def autotune(self, substeps):
         buf = C.create_string_buffer(1 << 16)

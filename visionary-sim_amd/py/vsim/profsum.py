@@ -19,7 +19,7 @@ def short(name):
     return name.split("(")[0].replace("vsim::", "")
 
 
-def trace(path, top=20):
+def trace(path, top=20, batched_min=1 << 20):
     d = collections.defaultdict(list)
     for r in csv.DictReader(open(path)):
         grid = int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0)
@@ -28,6 +28,16 @@ def trace(path, top=20):
     out = ["| kernel | grid (threads) | calls | avg us | total ms |", "|---|---:|---:|---:|---:|"]
     for (k, g), v in rows:
         out.append(f"| {k} | {g} | {len(v)} | {sum(v) / len(v) / 1e3:.2f} | {sum(v) / 1e6:.2f} |")
+    # The batched launches, one per pass across every scene, have grids far
+    # above any single scene's.
+    agg = collections.defaultdict(list)
+    for (k, g), v in d.items():
+        if g >= batched_min:
+            agg[k].extend(v)
+    out += ["", f"Batched launches (grid >= {batched_min} threads), all grids pooled:", "",
+            "| kernel | calls | avg us | min us | max us |", "|---|---:|---:|---:|---:|"]
+    for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        out.append(f"| {k} | {len(v)} | {sum(v) / len(v) / 1e3:.2f} | {min(v) / 1e3:.2f} | {max(v) / 1e3:.2f} |")
     return "\n".join(out)
 
 
