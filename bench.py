@@ -221,6 +221,15 @@ def main():
     el_bytes = cells * ELASTIC_BYTES_PER_CELL / 8
     achieved = el_bytes / el_avg_s / 1e9
     log(f"[bench] colour pass: {el_n} launches, {el_avg_s * 1e6:.2f} us avg, {achieved:.0f} GB/s algorithmic")
+    # ---- the splat pass alone, covariance output and with the engine format
+    # (rotation + axis lengths by four Jacobi sweeps) as well
+    from vsim import native
+    splat_us = {}
+    for label, mode in (("cov", native.SPLAT_COV), ("cov+axes", native.SPLAT_COV | native.SPLAT_AXES)):
+        world.frame(0, mode, False)
+        ms, n = world.time_pass("splat", 3)
+        splat_us[label] = ms / n * 1e3 if n else None
+    log(f"[bench] splat pass: " + ", ".join(f"{k} {v:.1f} us" for k, v in splat_us.items() if v))
 
     # ---- plain per-scene launches (the reference's dispatch shape)
     plain = None
@@ -273,6 +282,7 @@ def main():
                      "kernel": "k_elastic (colour pass)", "bytes_per_launch": el_bytes,
                      "avg_launch_us": el_avg_s * 1e6,
                      "avg_launch_us_with_events": prof["elastic"][0] / prof["elastic"][1] * 1e3},
+        "splat_pass_us": splat_us,
         "serving": serving,
         "cpu_baseline": cpu,
     }

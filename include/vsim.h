@@ -109,6 +109,13 @@ typedef struct vs_backend vs_backend;
 #define VS_K_SPLAT 9            /* splat deformation    sample.wgsl:40-80 + covariance.wgsl:30-42 */
 #define VS_K_COUNT 10
 
+/* Outputs of the splat pass (a bit mask). VS_SPLAT_AXES is the engine's splat
+ * format: the covariance taken apart into a unit rotation (xyzw) and three
+ * axis lengths by four sweeps of cyclic Jacobi, the step that follows the
+ * deformation in the reference (apps/simulator/src/render/ellipsoid.wgsl:32-91). */
+#define VS_SPLAT_COV 1
+#define VS_SPLAT_AXES 2
+
 /* Buffer slots of a launch. Node buffers hold 3 planes of `plane` floats. */
 #define VS_B_POS 0        /* nodePos  (3 planes)                              */
 #define VS_B_PREV 1       /* nodePrev (3 planes)                              */
@@ -124,7 +131,9 @@ typedef struct vs_backend vs_backend;
                              splats: float4 quaternion xyzw per splat, then 6 planes
                                      (mean x, y, z, scale x, y, z)                */
 #define VS_B_GRID 8       /* contact: counters then slots                     */
-#define VS_B_OUT 9        /* splats: 9 planes (mean x, y, z, cov xx xy xz yy yz zz);
+#define VS_B_OUT 9        /* splats: 9 planes (mean x, y, z, cov xx xy xz yy yz zz),
+                                     16 with VS_SPLAT_AXES (then rotation x, y, z, w
+                                     and axis lengths x, y, z);
                              contact collide: optional pair log, i32 count per
                              record then pair_cap u32 mate ids per record     */
 #define VS_B_COUNT 10
@@ -135,7 +144,7 @@ typedef struct vs_launch {
     uint32_t block;     /* threads per workgroup; 0 = 64                      */
     uint32_t plane;     /* floats per plane of the node buffers               */
     uint32_t items;     /* floats per plane of the item buffer (contact, drive, splats) */
-    uint32_t flags;     /* elastic: 1 = limit-only sweep                      */
+    uint32_t flags;     /* elastic: 1 = limit-only sweep; splats: VS_SPLAT_* outputs (0 = cov) */
     uint32_t buckets;   /* contact: buckets in the grid buffer (counters, then 32 slots each) */
     uint32_t pair_cap;  /* contact collide: pair log capacity per record (0 = off) */
     uint64_t buffers[VS_B_COUNT];
@@ -248,8 +257,9 @@ int vs_world_reset_scene(vs_world *w, uint32_t scene_id);
 /* Per-frame grab target of a scene's drive (simulator.js:309-315). */
 int vs_world_set_target(vs_world *w, uint32_t scene_id, const float target[3], const float velocity[3]);
 /* One frame of every admitted scene: `substeps` substeps as one batch per
- * pass, then splat deformation when `splats` is set. `graph` replays a
- * captured hipGraph of the frame. Asynchronous. */
+ * pass, then splat deformation when `splats` is non-zero: VS_SPLAT_COV,
+ * VS_SPLAT_AXES or both (any other non-zero value: the covariances). `graph`
+ * replays a captured hipGraph of the frame. Asynchronous. */
 int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph);
 /* The plain path: one scene's frame with per-scene launches, as the
  * reference dispatches a scene (stepper.js:101-108). Asynchronous. */
@@ -263,6 +273,9 @@ int vs_world_elapsed(vs_world *w, uint32_t from, uint32_t to, float *ms);
 int vs_world_read_nodes(vs_world *w, uint32_t scene_id, float *pos, float *vel);
 /* Copies a scene's deformed splats: means (3 planes) and covariances (6 planes). */
 int vs_world_read_splats(vs_world *w, uint32_t scene_id, float *mean, float *cov);
+/* The engine format of the last VS_SPLAT_AXES frame: means (3 planes),
+ * rotations (4 planes, xyzw) and axis lengths (3 planes). */
+int vs_world_read_splat_axes(vs_world *w, uint32_t scene_id, float *mean, float *rot, float *axis);
 /* Contact results of the last substep: bucket of each record and, when the
  * world was asked to log pairs (vs_world_log_pairs), up to `cap` mates each. */
 int vs_world_log_pairs(vs_world *w, uint32_t cap);
@@ -290,7 +303,7 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out);
  * kind k, launches[k] their number. Synchronous. */
 int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, uint32_t *launches);
 /* Times `rounds` back-to-back batched launches of one pass kind (elastic:
- * one round = the eight colour passes) with a HIP event before the first and
+ * one round = the eight colour passes; splats: the outputs of the last frame) with a HIP event before the first and
  * after the last, so *ms / *launches is the kernel's average duration with no
  * per-launch event in between. Node state is saved first and restored after,
  * so the simulation is left where it was. Synchronous. */

@@ -23,6 +23,7 @@ typedef REAL real;
 /* Correctly rounded square root and exact floor in the working precision. */
 #define SQRT(x) ((real)sqrt((double)(x)))
 #define FLOOR(x) ((real)floor((double)(x)))
+#define FABS(x) ((real)fabs((double)(x)))
 
 typedef struct { real x, y, z; } v3;
 typedef struct { v3 c[3]; } m3; /* column-major, like WGSL mat3x3f */
@@ -716,6 +717,8 @@ void o_substeps(o_scene *sc, int count)
 
 /* deformationAt (sample.wgsl:40-80) followed by deformedCovariance
  * (covariance.wgsl:30-42) for every splat. */
+void o_ellipsoid_axes(const REAL cov6[6], REAL rot[4], REAL scale3[3]);
+
 void o_deform_splats(const o_scene *sc, const o_splats *sp)
 {
     for (int i = 0; i < sp->n; i++) {
@@ -772,7 +775,68 @@ void o_deform_splats(const o_scene *sc, const o_splats *sp)
         sp->cov[3 * n + i] = (a0.y * a0.y + a1.y * a1.y) + a2.y * a2.y;
         sp->cov[4 * n + i] = (a0.y * a0.z + a1.y * a1.z) + a2.y * a2.z;
         sp->cov[5 * n + i] = (a0.z * a0.z + a1.z * a1.z) + a2.z * a2.z;
+        if (sp->rot) {
+            REAL c6[6], q[4], s3[3];
+            for (int k = 0; k < 6; k++) c6[k] = sp->cov[(size_t)k * n + i];
+            o_ellipsoid_axes(c6, q, s3);
+            for (int k = 0; k < 4; k++) sp->rot[(size_t)k * n + i] = q[k];
+            for (int k = 0; k < 3; k++) sp->axis[(size_t)k * n + i] = s3[k];
+        }
     }
+}
+
+/* A covariance taken apart into the engine's splat format: a unit rotation
+ * (xyzw) and three axis lengths. Cyclic Jacobi, four sweeps of the three
+ * cyclic planes (p, p+1, other = p+2) in order, skipping an exactly-zero
+ * off-diagonal; the smaller angle, tangent t = sign(theta)/(|theta| +
+ * sqrt(theta^2+1)); the plane rotation joined onto the accumulated
+ * quaternion through the half-angle tangent -sin/(1+cos) about +other, and
+ * renormalised; axis lengths sqrt(max(eigenvalue, 0))
+ * (apps/simulator/src/render/ellipsoid.wgsl:32-91, ellipsoid.js:40-149).
+ *
+ * The symmetric matrix is held as diagonal d[k] and off-diagonal o[k] of the
+ * pair (k, k+1 mod 3), so plane p reads o[p] and writes o[other] and o[r].
+ * The quaternion product with (0.., half, 1) keeps only its non-zero terms
+ * (zero products change no non-zero sum), and normalisation multiplies by the
+ * reciprocal length. Arithmetic otherwise as written, in REAL. */
+void o_ellipsoid_axes(const REAL cov6[6], REAL rot[4], REAL scale3[3])
+{
+    real d[3] = { cov6[0], cov6[3], cov6[5] };
+    real o[3] = { cov6[1], cov6[4], cov6[2] };  /* (0,1), (1,2), (2,0) */
+    real q[4] = { 0, 0, 0, 1 };
+    for (int sweep = 0; sweep < 4; sweep++) {
+        for (int p = 0; p < 3; p++) {
+            const int r = (p + 1) % 3, other = (p + 2) % 3;
+            const real apr = o[p];
+            if (apr == (real)0) continue;
+            const real app = d[p], arr = d[r];
+            const real theta = (arr - app) / ((real)2 * apr);
+            const real sign = theta >= (real)0 ? (real)1 : (real)-1;
+            const real tangent = sign / (FABS(theta) + SQRT(theta * theta + (real)1));
+            const real cosine = (real)1 / SQRT(tangent * tangent + (real)1);
+            const real sine = tangent * cosine;
+            const real apo = o[other], aro = o[r];
+            o[other] = cosine * apo - sine * aro;
+            o[r] = sine * apo + cosine * aro;
+            d[p] = app - tangent * apr;
+            d[r] = arr + tangent * apr;
+            o[p] = 0;
+            const real half = -(sine / ((real)1 + cosine));
+            real x = q[0], y = q[1], z = q[2], w = q[3];
+            real nx, ny, nz, nw;
+            if (other == 0) {        /* q * (half, 0, 0, 1) */
+                nx = w * half + x; ny = y + z * half; nz = z - y * half; nw = w - x * half;
+            } else if (other == 1) { /* q * (0, half, 0, 1) */
+                nx = x - z * half; ny = w * half + y; nz = x * half + z; nw = w - y * half;
+            } else {                 /* q * (0, 0, half, 1) */
+                nx = x + y * half; ny = y - x * half; nz = w * half + z; nw = w - z * half;
+            }
+            const real inv = (real)1 / SQRT(((nx * nx + ny * ny) + nz * nz) + nw * nw);
+            q[0] = nx * inv; q[1] = ny * inv; q[2] = nz * inv; q[3] = nw * inv;
+        }
+    }
+    for (int k = 0; k < 4; k++) rot[k] = q[k];
+    for (int k = 0; k < 3; k++) scale3[k] = SQRT(d[k] > (real)0 ? d[k] : (real)0);
 }
 
 /* Residuals of one cell's six tetrahedra at the current positions, for the

@@ -78,6 +78,8 @@ typedef struct {
     const REAL *sx, *sy, *sz;    /* axis standard deviations */
     REAL *ox, *oy, *oz;          /* deformed means */
     REAL *cov;                   /* 6 * n */
+    REAL *rot;                   /* engine axes (NULL to skip): 4 planes xyzw */
+    REAL *axis;                  /* and 3 planes of axis lengths */
 } o_splats;
 
 #endif

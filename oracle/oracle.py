@@ -55,6 +55,7 @@ class OSplats(C.Structure):
         ("mx", C.c_void_p), ("my", C.c_void_p), ("mz", C.c_void_p), ("quat", C.c_void_p),
         ("sx", C.c_void_p), ("sy", C.c_void_p), ("sz", C.c_void_p),
         ("ox", C.c_void_p), ("oy", C.c_void_p), ("oz", C.c_void_p), ("cov", C.c_void_p),
+        ("rot", C.c_void_p), ("axis", C.c_void_p),
     ]
 
 
@@ -72,6 +73,7 @@ def load(precision="f32"):
     lib.o_substeps.argtypes = [sp, C.c_int]
     lib.o_elastic_color.argtypes = [sp, C.c_int, C.c_int]
     lib.o_deform_splats.argtypes = [sp, C.POINTER(OSplats)]
+    lib.o_ellipsoid_axes.argtypes = [C.c_void_p] * 3
     lib.o_project_tet.argtypes = [C.c_void_p] * 4 + [C.c_double] * 4 + [C.c_void_p]
     lib.o_project_kuhn_tet.argtypes = [C.c_void_p] * 3 + [C.c_int] + [C.c_double] * 5 + [C.c_int, C.c_void_p]
     lib.o_cell_residuals.argtypes = [sp, C.c_uint32, C.c_uint32, C.c_void_p]
@@ -79,6 +81,19 @@ def load(precision="f32"):
     assert lib.o_real_size() == (4 if precision == "f32" else 8)
     _libs[precision] = lib
     return lib
+
+
+def ellipsoid_axes(cov6, precision="f64"):
+    """Rotation (xyzw) and axis lengths of a covariance given as (xx, xy, xz,
+    yy, yz, zz): the oracle's four-sweep cyclic Jacobi
+    (apps/simulator/src/render/ellipsoid.js:40-149)."""
+    lib = load(precision)
+    dt = np.float32 if precision == "f32" else np.float64
+    c = np.ascontiguousarray(cov6, dt)
+    rot = np.zeros(4, dt)
+    scale = np.zeros(3, dt)
+    lib.o_ellipsoid_axes(c.ctypes.data, rot.ctypes.data, scale.ctypes.data)
+    return rot, scale
 
 
 def project_tet(p, prev, w, B, alpha_dev, alpha_hyd, gamma, relax=0.0, precision="f64"):
@@ -215,6 +230,8 @@ class OracleScene:
             sp.sx, sp.sy, sp.sz = (self.ss[k].ctypes.data for k in range(3))
             sp.ox, sp.oy, sp.oz = (self.out_mean[k].ctypes.data for k in range(3))
             sp.cov = self.out_cov.ctypes.data
+            self.out_rot = np.zeros((4, G), dt)
+            self.out_axis = np.zeros((3, G), dt)
             self.sp = sp
 
     # passes
@@ -255,8 +272,13 @@ class OracleScene:
         if splats:
             self.deform_splats()
 
-    def deform_splats(self):
+    def deform_splats(self, axes=False):
+        """Deformed means and covariances (6 planes); with `axes`, also the
+        engine format, returned as self.out_rot (xyzw planes) and
+        self.out_axis (axis lengths)."""
         if self.G:
+            self.sp.rot = self.out_rot.ctypes.data if axes else None
+            self.sp.axis = self.out_axis.ctypes.data if axes else None
             self.lib.o_deform_splats(C.byref(self.sc), C.byref(self.sp))
         return getattr(self, "out_mean", None), getattr(self, "out_cov", None)
 

@@ -35,7 +35,7 @@ async function main() {
             means: f32(path.join(spec.splats, 'means.bin')),
             quats: f32(path.join(spec.splats, 'quats.bin')),
             scales: f32(path.join(spec.splats, 'scales.bin'))
-        });
+        }, { axes: true });
     }
     if (spec.drive) {
         sim.setDriveLaw(spec.drive.law === 0 ? 'hard' : 'compliant');
@@ -60,9 +60,11 @@ async function main() {
     write(out, 'cells.bin', lat.cells);
     write(out, 'surface.bin', surfaceNodes(lat));
     if (splats) {
-        const { means, cov } = await splats.read();
+        const { means, cov, rotation, scale } = await splats.read();
         write(out, 'splat_means.bin', means);
         write(out, 'splat_cov.bin', cov);
+        write(out, 'splat_rot.bin', rotation);
+        write(out, 'splat_axis.bin', scale);
     }
     fs.writeFileSync(path.join(out, 'meta.json'), JSON.stringify({ nodes: N, ms, stats: sim.stats() }));
     if (splats) splats.destroy();

@@ -127,6 +127,52 @@ def test_splats_after_frames():
     assert rel_err(cov, o.out_cov) <= RTOL
 
 
+def rebuilt_cov(rot, axis):
+    # R diag(axis)^2 R^T per splat, in f64, as 6 planes (xx xy xz yy yz zz).
+    x, y, z, w = (r.astype(np.float64) for r in rot)
+    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)]),
+                  np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)]),
+                  np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)])])
+    s2 = axis.astype(np.float64) ** 2
+    c = np.einsum("ikn,kn,jkn->ijn", R, s2, R)
+    return np.stack([c[0, 0], c[0, 1], c[0, 2], c[1, 1], c[1, 2], c[2, 2]])
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_splat_engine_axes(graph):
+    # Engine format after three frames: rotations and axis lengths against the
+    # f32 oracle's Jacobi, and the covariance they describe against the one
+    # the same launch wrote.
+    sc = scenes.c1_scene(splats=50_000)
+    w, sid, o, pos, vel = step_both(sc, 3, splats=False)
+    w.frame(0, native.SPLAT_COV | native.SPLAT_AXES, graph)
+    o.deform_splats(axes=True)
+    mean, cov = w.read_splats(sid)
+    m2, rot, axis = w.read_splat_axes(sid)
+    print(f"axes: rotation exact {np.mean(rot == o.out_rot):.6f}, lengths exact {np.mean(axis == o.out_axis):.6f}")
+    assert np.array_equal(m2, mean)
+    assert rel_err(cov, o.out_cov) <= RTOL
+    assert np.max(np.abs(rot.astype(np.float64) - o.out_rot)) <= RTOL
+    assert rel_err(axis, o.out_axis) <= RTOL
+    assert np.max(np.abs(np.sum(rot.astype(np.float64) ** 2, axis=0) - 1)) < 1e-6
+    back = rebuilt_cov(rot, axis)
+    big = np.max(np.abs(cov), axis=0)
+    assert np.max(np.max(np.abs(back - cov), axis=0) / big) < 1e-5
+
+
+def test_splat_axes_only_leaves_covariance_untouched():
+    sc = scenes.jelly_block(6, splats=500, seed=3, name="ax")
+    w = world_for(sc)
+    sid = w.add(sc)
+    w.frame(1, native.SPLAT_COV, False)
+    _, cov1 = w.read_splats(sid)
+    w.frame(0, native.SPLAT_AXES, False)
+    _, cov2 = w.read_splats(sid)
+    _, rot, axis = w.read_splat_axes(sid)
+    assert np.array_equal(cov1, cov2)
+    assert np.all(np.isfinite(rot)) and np.all(axis >= 0)
+
+
 def test_batched_equals_alone_and_graph_equals_eager():
     mix = [scenes.jelly_block(s, splats=300, seed=s, name=f"m{s}") for s in (5, 9, 13)]
     mix[1].drive = scenes.handle_drive(mix[1], seed=2)

@@ -85,7 +85,8 @@ def test_js_simulator_matches_oracle(tmp_path, case):
     assert np.array_equal(load(out, "w.bin"), arrays["w"])
     o = OracleScene(sc, "f32")
     for _ in range(frames):
-        o.frame(sc.substeps, splats=True)
+        o.frame(sc.substeps)
+        o.deform_splats(axes=True)
     pos = load(out, "pos.bin").reshape(3, -1)
     vel = load(out, "vel.bin").reshape(3, -1)
     scale = np.max(np.abs(o.p))
@@ -98,3 +99,9 @@ def test_js_simulator_matches_oracle(tmp_path, case):
         cov = load(out, "splat_cov.bin").reshape(6, -1)
         assert np.max(np.abs(mean - o.out_mean)) / np.max(np.abs(o.out_mean)) <= 1e-5
         assert np.max(np.abs(cov - o.out_cov)) / np.max(np.abs(o.out_cov)) <= 1e-5
+        rot = load(out, "splat_rot.bin").reshape(4, -1)
+        axis = load(out, "splat_axis.bin").reshape(3, -1)
+        print(f"engine axes: rotation exact {np.mean(rot == o.out_rot):.6f}, lengths exact "
+              f"{np.mean(axis == o.out_axis):.6f}")
+        assert np.max(np.abs(rot - o.out_rot)) <= 1e-5
+        assert np.max(np.abs(axis - o.out_axis)) / np.max(np.abs(o.out_axis)) <= 1e-5

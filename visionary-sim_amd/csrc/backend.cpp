@@ -85,7 +85,7 @@ uint64_t slot_bytes(const vs_launch &l, int slot)
         return 36 * I;  // nine surface planes
     case VS_B_GRID: return 4ull * l.buckets * (1 + 32);
     case VS_B_OUT:
-        if (l.kernel == VS_K_SPLAT) return 36 * I;
+        if (l.kernel == VS_K_SPLAT) return (l.flags & VS_SPLAT_AXES ? 64 : 36) * I;
         return l.pair_cap ? 4ull * I * (1 + l.pair_cap) : 0;
     default: return 0;
     }
@@ -308,9 +308,12 @@ int vs_dispatch(vs_backend *b, const vs_launch *launches, uint32_t count)
         }
         case VS_K_SPLAT: {
             float *in = F(VS_B_ITEMS), *out = F(VS_B_OUT);
-            SplatView sp{ in + 4 * I, in + 5 * I, in + 6 * I, reinterpret_cast<const float4 *>(in),
-                          in + 7 * I, in + 8 * I, in + 9 * I, out, out + I, out + 2 * I, out + 3 * I,
-                          (uint32_t)I };
+            const uint32_t mode = l.flags & (VS_SPLAT_COV | VS_SPLAT_AXES) ? l.flags & (VS_SPLAT_COV | VS_SPLAT_AXES)
+                                                                            : VS_SPLAT_COV;
+            SplatView sp{ in + 4 * I, in + 5 * I,  in + 6 * I,  reinterpret_cast<const float4 *>(in),
+                          in + 7 * I, in + 8 * I,  in + 9 * I,  out,
+                          out + I,    out + 2 * I, out + 3 * I, out + 9 * I,
+                          out + 13 * I, (uint32_t)I, mode };
             e = launch_splats(b->stream, sh, n.px, n.py, n.pz, bodies, sp, t);
             break;
         }

@@ -59,7 +59,10 @@ struct SplatView {
     const float *sx, *sy, *sz;
     float *ox, *oy, *oz;
     float *cov;        // 6 planes of `plane` floats: xx xy xz yy yz zz
+    float *rot;        // engine format: 4 planes (xyzw), then
+    float *axis;       // 3 planes of axis lengths
     uint32_t plane;
+    uint32_t mode;     // VS_SPLAT_COV | VS_SPLAT_AXES
 };
 
 struct Shape {

@@ -604,6 +604,69 @@ __global__ __launch_bounds__(BLOCK) void k_drive(NodeView n, const vs_scene_para
 }
 
 // --------------------------------------------------------------- splats
+// A covariance taken apart into the engine's splat format, a unit rotation
+// (xyzw) and three axis lengths, by four sweeps of cyclic Jacobi
+// (ellipsoid.wgsl:32-91) in the oracle's o_ellipsoid_axes order: planes
+// (p, p+1, other = p+2), an exactly-zero off-diagonal skipped, the smaller
+// angle, the half-angle tangent -sin/(1+cos) about +other joined onto the
+// quaternion (only its non-zero terms), renormalised by the reciprocal
+// length; lengths sqrt(max(eigenvalue, 0)). d = diagonal, o[k] = entry
+// (k, k+1 mod 3).
+__device__ __forceinline__ void ellipsoid_axes(float d[3], float o[3], float q[4])
+{
+    q[0] = 0.f;
+    q[1] = 0.f;
+    q[2] = 0.f;
+    q[3] = 1.f;
+#pragma unroll
+    for (int sweep = 0; sweep < 4; sweep++) {
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            const int r = (p + 1) % 3, other = (p + 2) % 3;
+            const float apr = o[p];
+            if (apr == 0.0f) continue;
+            const float app = d[p], arr = d[r];
+            const float theta = (arr - app) / (2.0f * apr);
+            const float sign = theta >= 0.0f ? 1.0f : -1.0f;
+            const float tangent = sign / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
+            const float cosine = 1.0f / sqrtf(tangent * tangent + 1.0f);
+            const float sine = tangent * cosine;
+            const float apo = o[other], aro = o[r];
+            o[other] = cosine * apo - sine * aro;
+            o[r] = sine * apo + cosine * aro;
+            d[p] = app - tangent * apr;
+            d[r] = arr + tangent * apr;
+            o[p] = 0.0f;
+            const float half = -(sine / (1.0f + cosine));
+            const float x = q[0], y = q[1], z = q[2], w = q[3];
+            float nx, ny, nz, nw;
+            if (other == 0) {
+                nx = w * half + x;
+                ny = y + z * half;
+                nz = z - y * half;
+                nw = w - x * half;
+            } else if (other == 1) {
+                nx = x - z * half;
+                ny = w * half + y;
+                nz = x * half + z;
+                nw = w - y * half;
+            } else {
+                nx = x + y * half;
+                ny = y - x * half;
+                nz = w * half + z;
+                nw = w - z * half;
+            }
+            const float inv = 1.0f / sqrtf(((nx * nx + ny * ny) + nz * nz) + nw * nw);
+            q[0] = nx * inv;
+            q[1] = ny * inv;
+            q[2] = nz * inv;
+            q[3] = nw * inv;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) d[k] = sqrtf(d[k] > 0.0f ? d[k] : 0.0f);
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_splats(const float *__restrict__ px, const float *__restrict__ py,
                                                   const float *__restrict__ pz, const vs_body *__restrict__ bodies,
@@ -662,12 +725,26 @@ __global__ __launch_bounds__(BLOCK) void k_splats(const float *__restrict__ px, 
             const V3 a1 = mul(add(add(mul(c0, r1.x), mul(c1, r1.y)), mul(c2, r1.z)), sp.sy[k]);
             const V3 a2 = mul(add(add(mul(c0, r2.x), mul(c1, r2.y)), mul(c2, r2.z)), sp.sz[k]);
             const size_t P = sp.plane;
-            sp.cov[0 * P + k] = (a0.x * a0.x + a1.x * a1.x) + a2.x * a2.x;
-            sp.cov[1 * P + k] = (a0.x * a0.y + a1.x * a1.y) + a2.x * a2.y;
-            sp.cov[2 * P + k] = (a0.x * a0.z + a1.x * a1.z) + a2.x * a2.z;
-            sp.cov[3 * P + k] = (a0.y * a0.y + a1.y * a1.y) + a2.y * a2.y;
-            sp.cov[4 * P + k] = (a0.y * a0.z + a1.y * a1.z) + a2.y * a2.z;
-            sp.cov[5 * P + k] = (a0.z * a0.z + a1.z * a1.z) + a2.z * a2.z;
+            float d[3] = { (a0.x * a0.x + a1.x * a1.x) + a2.x * a2.x, (a0.y * a0.y + a1.y * a1.y) + a2.y * a2.y,
+                           (a0.z * a0.z + a1.z * a1.z) + a2.z * a2.z };
+            float o[3] = { (a0.x * a0.y + a1.x * a1.y) + a2.x * a2.y, (a0.y * a0.z + a1.y * a1.z) + a2.y * a2.z,
+                           (a0.x * a0.z + a1.x * a1.z) + a2.x * a2.z };
+            if (sp.mode & VS_SPLAT_COV) {
+                sp.cov[0 * P + k] = d[0];
+                sp.cov[1 * P + k] = o[0];
+                sp.cov[2 * P + k] = o[2];
+                sp.cov[3 * P + k] = d[1];
+                sp.cov[4 * P + k] = o[1];
+                sp.cov[5 * P + k] = d[2];
+            }
+            if (sp.mode & VS_SPLAT_AXES) {
+                float q[4];
+                ellipsoid_axes(d, o, q);
+#pragma unroll
+                for (int a = 0; a < 4; a++) sp.rot[a * P + k] = q[a];
+#pragma unroll
+                for (int a = 0; a < 3; a++) sp.axis[a * P + k] = d[a];
+            }
         }
     }
 }

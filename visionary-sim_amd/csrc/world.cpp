@@ -37,6 +37,7 @@ constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, NP
 constexpr int NODE_PLANES = 13;        // p xyz, prev xyz, vel xyz, w, rest xyz
 constexpr int SURF_PLANES = 9;
 constexpr int DRIVE_PLANES = 4;
+constexpr int SPLAT_OUT_PLANES = 16;    // mean xyz, cov 6, rotation xyzw, axis lengths xyz
 constexpr uint32_t EVENT_POOL = 8192;
 
 // First-fit range allocator with coalescing.
@@ -164,6 +165,7 @@ struct vs_world {
     std::vector<uint32_t> graph_key;
     uint32_t captures = 0;
 
+    uint32_t splat_mode = VS_SPLAT_COV;  // outputs of the splat pass being enqueued
     uint32_t block_node = 256, block_cells = 256, block_surf = 256, block_drive = 64, block_splat = 256;
     uint32_t tile_nodes = 256, tile_cells = 256, tile_surf = 256, tile_drive = 64, tile_splat = 256;
 };
@@ -183,6 +185,15 @@ int fail(vs_world *w, int code, const std::string &m)
     } while (0)
 
 float *plane(vs_world *w, int k) { return w->d_nodes + (size_t)k * w->cfg.max_nodes; }
+
+// Splat outputs of a frame: 0 none, else VS_SPLAT_COV and/or VS_SPLAT_AXES
+// (any other non-zero value means the covariances).
+uint32_t splat_mode(int32_t splats)
+{
+    if (!splats) return 0;
+    const uint32_t m = (uint32_t)splats & (VS_SPLAT_COV | VS_SPLAT_AXES);
+    return m ? m : VS_SPLAT_COV;
+}
 
 NodeView node_view(vs_world *w)
 {
@@ -219,9 +230,10 @@ SplatView splat_view(vs_world *w)
 {
     const size_t G = w->cfg.max_splats;
     float *in = w->d_splat_in, *out = w->d_splat_out;
-    return SplatView{ in + 4 * G,  in + 5 * G,  in + 6 * G,  reinterpret_cast<const float4 *>(in),
-                      in + 7 * G,  in + 8 * G,  in + 9 * G,  out, out + G, out + 2 * G, out + 3 * G,
-                      (uint32_t)G };
+    return SplatView{ in + 4 * G,  in + 5 * G,     in + 6 * G,     reinterpret_cast<const float4 *>(in),
+                      in + 7 * G,  in + 8 * G,     in + 9 * G,     out,
+                      out + G,     out + 2 * G,    out + 3 * G,    out + 9 * G,
+                      out + 13 * G, (uint32_t)G,   w->splat_mode };
 }
 
 uint32_t pass_block(const vs_world *w, int p)
@@ -427,8 +439,9 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
 
 // Enqueues one frame: `substeps` substeps in Stepper.sequence order
 // (stepper.js:101-108, solve order simulator.js:83), then the splats.
-int enqueue_frame(vs_world *w, uint32_t substeps, bool splats, int scene, Profile *prof = nullptr)
+int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Profile *prof = nullptr)
 {
+    if (splats) w->splat_mode = splats;
     bool stretch = w->any_stretch, contact = w->any_contact;
     if (scene >= 0) {
         stretch = w->recs[scene].stretch;
@@ -543,7 +556,7 @@ static int world_init(vs_world *w)
     ALLOC(W->d_grid, sizeof(uint32_t) * (1 + BUCKET_SLOTS) * c.max_buckets);
     ALLOC(W->d_drive, sizeof(uint32_t) * DRIVE_PLANES * c.max_driven);
     ALLOC(W->d_splat_in, sizeof(float) * 10 * c.max_splats);
-    ALLOC(W->d_splat_out, sizeof(float) * 9 * c.max_splats);
+    ALLOC(W->d_splat_out, sizeof(float) * SPLAT_OUT_PLANES * c.max_splats);
     ALLOC(W->d_scene_counts, sizeof(uint32_t) * NPASS * c.max_scenes);
     HIPW(W, hipMemset(W->d_nodes, 0, sizeof(float) * NODE_PLANES * c.max_nodes));
     HIPW(W, hipMemset(W->d_grid, 0, sizeof(uint32_t) * c.max_buckets));
@@ -917,8 +930,9 @@ int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph
     if (!w) return VS_E_INVALID;
     HIPW(w, hipSetDevice(w->cfg.device));
     if (int rc = prepare(w)) return rc;
-    if (!graph) return enqueue_frame(w, substeps, splats != 0, -1);
-    std::vector<uint32_t> key = { substeps, (uint32_t)(splats != 0), (uint32_t)w->any_stretch, (uint32_t)w->any_contact,
+    const uint32_t mode = splat_mode(splats);
+    if (!graph) return enqueue_frame(w, substeps, mode, -1);
+    std::vector<uint32_t> key = { substeps, mode, (uint32_t)w->any_stretch, (uint32_t)w->any_contact,
                                   w->pair_cap, w->block_cells, w->cfg.xcd_chunked };
     for (int p = 0; p < NPASS; p++) key.push_back(grid_for(w, p, -1));
     if (!w->exec || key != w->graph_key) {
@@ -927,7 +941,7 @@ int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph
         w->exec = nullptr;
         w->graph = nullptr;
         HIPW(w, hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_frame(w, substeps, splats != 0, -1);
+        const int rc = enqueue_frame(w, substeps, mode, -1);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(w->stream, &g);
         if (rc) {
@@ -949,7 +963,7 @@ int vs_world_frame_scene(vs_world *w, uint32_t id, uint32_t substeps, int32_t sp
     if (!w || id >= w->cfg.max_scenes || !w->recs[id].live) return VS_E_INVALID;
     HIPW(w, hipSetDevice(w->cfg.device));
     if (int rc = prepare(w)) return rc;
-    return enqueue_frame(w, substeps, splats != 0, (int)id);
+    return enqueue_frame(w, substeps, splat_mode(splats), (int)id);
 }
 
 int vs_world_synchronize(vs_world *w)
@@ -1007,6 +1021,25 @@ int vs_world_read_splats(vs_world *w, uint32_t id, float *mean, float *cov)
     return VS_OK;
 }
 
+int vs_world_read_splat_axes(vs_world *w, uint32_t id, float *mean, float *rot, float *axis)
+{
+    if (!w || id >= w->cfg.max_scenes || !w->recs[id].live) return VS_E_INVALID;
+    if (int rc = vs_world_synchronize(w)) return rc;
+    const SceneRec &r = w->recs[id];
+    const size_t G = w->cfg.max_splats, n = r.splat_count;
+    std::vector<float> tmp(n);
+    const int planes[10] = { 0, 1, 2, 9, 10, 11, 12, 13, 14, 15 };
+    for (int k = 0; k < 10; k++) {
+        float *dst = k < 3 ? (mean ? mean + k * n : nullptr)
+                     : k < 7 ? (rot ? rot + (k - 3) * n : nullptr)
+                             : (axis ? axis + (k - 7) * n : nullptr);
+        if (!dst || !n) continue;
+        HIPW(w, hipMemcpy(tmp.data(), w->d_splat_out + planes[k] * G + r.splat_begin, n * 4, hipMemcpyDeviceToHost));
+        for (size_t j = 0; j < n; j++) dst[r.splat_perm[j]] = tmp[j];
+    }
+    return VS_OK;
+}
+
 int vs_world_log_pairs(vs_world *w, uint32_t cap)
 {
     if (!w) return VS_E_INVALID;
@@ -1058,7 +1091,7 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out)
     out->bytes_reserved = 4ull * NODE_PLANES * c.max_nodes + 4ull * c.max_cells + sizeof(vs_body) * c.max_bodies +
                           sizeof(vs_scene_params) * c.max_scenes + 4ull * SURF_PLANES * c.max_surface +
                           4ull * (1 + BUCKET_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
-                          76ull * c.max_splats;
+                          4ull * (10 + SPLAT_OUT_PLANES) * c.max_splats;
     out->graph_captures = w->captures;
     return VS_OK;
 }
@@ -1069,7 +1102,7 @@ int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, 
     HIPW(w, hipSetDevice(w->cfg.device));
     if (int rc = prepare(w)) return rc;
     Profile prof;
-    if (int rc = enqueue_frame(w, substeps, splats != 0, -1, &prof)) return rc;
+    if (int rc = enqueue_frame(w, substeps, splat_mode(splats), -1, &prof)) return rc;
     HIPW(w, hipStreamSynchronize(w->stream));
     for (int k = 0; k < VS_P_COUNT; k++) {
         ms[k] = 0.f;

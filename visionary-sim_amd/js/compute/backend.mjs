@@ -21,9 +21,9 @@ const KERNEL_PARAMS = {
     'sim-contact-settle': [...NODE, 'surface', 'contactGrid', 'items', 'buckets'],
     'sim-drive-hard': [...NODE, 'nodeVel', 'driven', 'items'],
     'sim-drive-compliant': [...NODE, 'nodeVel', 'driven', 'items'],
-    'sim-splat-deform': ['nodePos', 'bodies', 'tiles', 'splatIn', 'splatOut', 'items', 'plane', 'block']
+    'sim-splat-deform': ['nodePos', 'bodies', 'tiles', 'splatIn', 'splatOut', 'items', 'plane', 'block', 'outputs']
 };
-const OPTIONAL = new Set(['block', 'pairLog', 'pairCap', 'limitOnly', 'nodeInvMass', 'nodePrev', 'params']);
+const OPTIONAL = new Set(['block', 'pairLog', 'pairCap', 'limitOnly', 'nodeInvMass', 'nodePrev', 'params', 'outputs']);
 const BUFFER_SLOT = {
     nodePos: SLOT.nodePos, nodePrev: SLOT.nodePrev, nodeVel: SLOT.nodeVel, nodeInvMass: SLOT.nodeInvMass,
     bodies: SLOT.bodies, params: SLOT.params, tiles: SLOT.tiles, cells: SLOT.items, surface: SLOT.items,
@@ -131,7 +131,8 @@ class Pass {
         view.setUint32(at + 8, v('block') || 64, true);
         view.setUint32(at + 12, v('plane') || 0, true);
         view.setUint32(at + 16, v('items') || 0, true);
-        view.setUint32(at + 20, v('limitOnly') ? 1 : 0, true);
+        // flags: the elastic limit-only sweep, or the splat outputs (VS_SPLAT_*).
+        view.setUint32(at + 20, v('limitOnly') ? 1 : (v('outputs') || 0), true);
         view.setUint32(at + 24, v('buckets') || 0, true);
         view.setUint32(at + 28, v('pairCap') || 0, true);
         for (const name of this.kernel.params) {

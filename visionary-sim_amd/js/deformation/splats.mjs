@@ -5,6 +5,12 @@
 import { tileWords } from '../state/particles.mjs';
 
 const SPLAT_TILE = 256;
+// Outputs of the pass (VS_SPLAT_* of include/vsim.h): the covariance, and the
+// engine's splat format -- a unit rotation and three axis lengths, the
+// covariance taken apart by four sweeps of cyclic Jacobi
+// (apps/simulator/src/render/ellipsoid.wgsl:32-91).
+export const SPLAT_COV = 1;
+export const SPLAT_AXES = 2;
 
 export class SplatDeformer {
     /**
@@ -12,8 +18,9 @@ export class SplatDeformer {
      * @param {number} bodyIndex - Which body the splats ride.
      * @param {{ means: Float32Array, quats: Float32Array, scales: Float32Array }} splats -
      *   xyz, xyzw and xyz per splat.
+     * @param {{ axes?: boolean }} [options] - `axes`: also write the engine format.
      */
-    constructor(sim, bodyIndex, { means, quats, scales }) {
+    constructor(sim, bodyIndex, { means, quats, scales }, { axes = false } = {}) {
         const backend = sim.backend;
         const G = means.length / 3;
         this.count = G;
@@ -28,7 +35,8 @@ export class SplatDeformer {
         }
         this.input = backend.buffer(input.byteLength);
         this.input.write(0, input);
-        this.output = backend.buffer(9 * 4 * Math.max(1, G));
+        this.axes = axes;
+        this.output = backend.buffer((axes ? 16 : 9) * 4 * Math.max(1, G));
         const { words, count } = tileWords([{ begin: 0, count: G, body: bodyIndex }], SPLAT_TILE);
         this.tiles = backend.buffer(words.byteLength);
         this.tiles.write(0, words);
@@ -41,6 +49,7 @@ export class SplatDeformer {
         pass.set('items', G);
         pass.set('plane', sim.particles.nodeCount);
         pass.set('block', SPLAT_TILE);
+        pass.set('outputs', axes ? SPLAT_COV | SPLAT_AXES : SPLAT_COV);
         pass.setupDispatch(count);
         this.pass = pass;
     }
@@ -48,11 +57,18 @@ export class SplatDeformer {
     /** Enqueues the deformation of every splat at the current node positions. */
     deform() { this.backend.dispatch([this.pass], 'splat-deform'); }
 
-    /** @returns {Promise<{ means: Float32Array, cov: Float32Array }>} Planes: 3 of means, 6 of covariance (xx xy xz yy yz zz). */
+    /** @returns {Promise<{ means: Float32Array, cov: Float32Array, rotation?: Float32Array, scale?: Float32Array }>}
+     *  Planes: 3 of means, 6 of covariance (xx xy xz yy yz zz); with `axes`, 4 of rotation (xyzw) and 3 of axis lengths. */
     async read() {
-        const out = new Float32Array(9 * this.count);
+        const G = this.count;
+        const out = new Float32Array((this.axes ? 16 : 9) * G);
         await this.output.read(0, out.byteLength, out);
-        return { means: out.subarray(0, 3 * this.count), cov: out.subarray(3 * this.count) };
+        const result = { means: out.subarray(0, 3 * G), cov: out.subarray(3 * G, 9 * G) };
+        if (this.axes) {
+            result.rotation = out.subarray(9 * G, 13 * G);  // planes x, y, z, w
+            result.scale = out.subarray(13 * G, 16 * G);
+        }
+        return result;
     }
 
     destroy() {

@@ -113,11 +113,13 @@ EXPORTS = [
     "vs_world_create", "vs_world_destroy", "vs_world_error", "vs_world_add_scene", "vs_world_remove_scene",
     "vs_world_reset_scene", "vs_world_set_target", "vs_world_frame", "vs_world_frame_scene",
     "vs_world_synchronize", "vs_world_record", "vs_world_elapsed", "vs_world_read_nodes",
-    "vs_world_read_splats", "vs_world_log_pairs", "vs_world_read_contacts", "vs_world_stats_get",
+    "vs_world_read_splats", "vs_world_read_splat_axes", "vs_world_log_pairs", "vs_world_read_contacts", "vs_world_stats_get",
     "vs_world_autotune", "vs_world_profile", "vs_world_time_pass",
     "vs_server_create", "vs_server_destroy", "vs_server_error", "vs_server_submit", "vs_server_run",
     "vs_server_latencies", "vs_server_result", "vs_server_times",
 ]
+
+SPLAT_COV, SPLAT_AXES = 1, 2  # VS_SPLAT_* output bits of a frame's `splats`
 
 PASS_KINDS = ["predict", "drive", "elastic", "bin", "sort", "collide", "settle", "finalize", "splat"]
 
@@ -158,6 +160,7 @@ def load(path=LIB_PATH):
         "vs_world_elapsed": (C.c_int, [p, u32, u32, C.POINTER(f)]),
         "vs_world_read_nodes": (C.c_int, [p, u32, p, p]),
         "vs_world_read_splats": (C.c_int, [p, u32, p, p]),
+        "vs_world_read_splat_axes": (C.c_int, [p, u32, p, p, p]),
         "vs_world_log_pairs": (C.c_int, [p, u32]),
         "vs_world_read_contacts": (C.c_int, [p, u32, p, p, p]),
         "vs_world_stats_get": (C.c_int, [p, C.POINTER(VsWorldStats)]),
@@ -284,6 +287,17 @@ class World:
         cov = np.empty((6, g), np.float32)
         self._check(self.lib.vs_world_read_splats(self.h, sid, mean.ctypes.data, cov.ctypes.data), "read_splats")
         return mean, cov
+
+    def read_splat_axes(self, sid):
+        """Means (3, G), rotations (4, G, xyzw) and axis lengths (3, G) of the
+        last frame run with splats=SPLAT_AXES."""
+        g = self.scenes[sid].n_splats
+        mean = np.empty((3, g), np.float32)
+        rot = np.empty((4, g), np.float32)
+        axis = np.empty((3, g), np.float32)
+        self._check(self.lib.vs_world_read_splat_axes(self.h, sid, mean.ctypes.data, rot.ctypes.data,
+                                                      axis.ctypes.data), "read_splat_axes")
+        return mean, rot, axis
 
     def log_pairs(self, cap):
         self._check(self.lib.vs_world_log_pairs(self.h, cap), "log_pairs")
