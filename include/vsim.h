@@ -223,6 +223,16 @@ typedef struct vs_body_desc {
     const float *splat_mean;     /* 3 * n_splats                            */
     const float *splat_quat;     /* 4 * n_splats, xyzw                      */
     const float *splat_scale;    /* 3 * n_splats                            */
+    /* Lattice built on the device (buildLattice and surfaceNodes,
+     * lattice.js:128-239): when `occupancy` is set, n_cells, cells,
+     * color_count, origin, h, rest, inv_mass, n_surface and surface are
+     * ignored and computed from the fields below. */
+    const uint8_t *occupancy;    /* cell_dims product bytes, x fastest, non-zero = occupied */
+    double origin64[3];          /* min corner of cell (0,0,0); rest = f32(origin + i*h) in f64 */
+    double h64;                  /* cell edge                               */
+    double density;              /* kg/m^3: corner mass density*h^3/8 per incident cell */
+    uint32_t n_static;           /* body-local nodes prescribed static (w = 0, */
+    const uint32_t *static_nodes;/*   staticPrescription.js:9-20)             */
 } vs_body_desc;
 
 typedef struct vs_scene_desc {
@@ -273,6 +283,16 @@ int vs_world_elapsed(vs_world *w, uint32_t from, uint32_t to, float *ms);
 int vs_world_read_nodes(vs_world *w, uint32_t scene_id, float *pos, float *vel);
 /* Copies a scene's deformed splats: means (3 planes) and covariances (6 planes). */
 int vs_world_read_splats(vs_world *w, uint32_t scene_id, float *mean, float *cov);
+/* Sizes of one body as admitted (after a device lattice build, the counts it
+ * found), and its lattice arrays: rest (3 planes), GPU inverse mass, cells
+ * grouped by colour, surface nodes (body-local, ascending). */
+typedef struct vs_body_info {
+    uint32_t node_count, n_cells, n_surface, pad0;
+    uint32_t color_count[8];
+} vs_body_info;
+int vs_world_body_info(vs_world *w, uint32_t scene_id, uint32_t body, vs_body_info *out);
+int vs_world_read_lattice(vs_world *w, uint32_t scene_id, uint32_t body, float *rest, float *inv_mass, uint32_t *cells,
+                          uint32_t *surface);
 /* The engine format of the last VS_SPLAT_AXES frame: means (3 planes),
  * rotations (4 planes, xyzw) and axis lengths (3 planes). */
 int vs_world_read_splat_axes(vs_world *w, uint32_t scene_id, float *mean, float *rot, float *axis);

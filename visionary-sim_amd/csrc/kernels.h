@@ -65,6 +65,24 @@ struct SplatView {
     uint32_t mode;     // VS_SPLAT_COV | VS_SPLAT_AXES
 };
 
+// One body's lattice build on the device (lattice.hip).
+struct LatticeView {
+    const uint8_t *occ;          // cell_dims product bytes, x fastest
+    uint32_t cell_dims[3];
+    uint32_t node_offset;        // world id of the body's node 0
+    uint32_t body;               // world body id of the surface records
+    float radius;                // contact radius of the surface records
+    double origin[3], h;         // rest = f32(origin + i * h), in f64
+    float velocity[3];
+    float wtab[9];               // inverse mass by occupied incident cells
+    uint32_t tile_base[9];       // first tile of each colour; [8] = first node tile
+    uint32_t *tile_count;        // count pass: kept elements per tile
+    const uint32_t *tile_offset; // write pass: exclusive offset per tile
+    uint32_t *cells;             // the body's range of the cell pool
+    uint32_t *s_node, *s_body;   // the body's range of the surface planes
+    float *s_radius;
+};
+
 struct Shape {
     uint32_t grid;     // workgroups
     uint32_t block;    // 64, 128, 256 or 512
@@ -89,6 +107,12 @@ hipError_t launch_contact_settle(hipStream_t s, Shape sh, NodeView n, const vs_s
                                  SurfaceView sv, GridView g, TileView t);
 hipError_t launch_drive(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, DriveView d,
                         TileView t, int compliant);
+// Lattice build: tile layout of one body (returns all tiles, colour tiles
+// first), the count / write passes, the node planes, and mass zeroing.
+uint32_t lattice_tiles(const uint32_t cell_dims[3], uint32_t tile_base[9], uint32_t *node_tiles);
+hipError_t launch_lattice_lists(hipStream_t s, const LatticeView &L, uint32_t tiles, uint32_t write);
+hipError_t launch_lattice_nodes(hipStream_t s, const LatticeView &L, NodeView n, float *rx, float *ry, float *rz);
+hipError_t launch_zero_mass(hipStream_t s, float *w, const uint32_t *nodes, uint32_t count);
 hipError_t launch_splats(hipStream_t s, Shape sh, const float *px, const float *py, const float *pz,
                          const vs_body *bodies, SplatView sp, TileView t);
 

@@ -31,7 +31,8 @@ using Clock = std::chrono::steady_clock;
 // Owning copy of a vs_scene_desc and everything it points at.
 struct SceneCopy {
     std::vector<vs_body_desc> bodies;
-    std::vector<std::vector<uint32_t>> cells, surface;
+    std::vector<std::vector<uint32_t>> cells, surface, statics;
+    std::vector<std::vector<uint8_t>> occupancy;
     std::vector<std::vector<float>> rest, inv, mean, quat, scale;
     std::vector<uint32_t> driven;
     std::vector<float> offsets;
@@ -58,6 +59,8 @@ struct SceneCopy {
             mean.push_back(take(b.splat_mean, 3 * (size_t)b.n_splats));
             quat.push_back(take(b.splat_quat, 4 * (size_t)b.n_splats));
             scale.push_back(take(b.splat_scale, 3 * (size_t)b.n_splats));
+            occupancy.push_back(take(b.occupancy, b.occupancy ? (size_t)b.cell_dims[0] * b.cell_dims[1] * b.cell_dims[2] : 0));
+            statics.push_back(take(b.static_nodes, b.n_static));
         }
         for (size_t i = 0; i < bodies.size(); i++) {
             bodies[i].cells = cells[i].data();
@@ -67,6 +70,8 @@ struct SceneCopy {
             bodies[i].splat_mean = mean[i].empty() ? nullptr : mean[i].data();
             bodies[i].splat_quat = quat[i].empty() ? nullptr : quat[i].data();
             bodies[i].splat_scale = scale[i].empty() ? nullptr : scale[i].data();
+            bodies[i].occupancy = occupancy[i].empty() ? nullptr : occupancy[i].data();
+            bodies[i].static_nodes = statics[i].empty() ? nullptr : statics[i].data();
         }
         driven = take(d.driven, d.n_driven);
         offsets = take(d.driven_offset, 3 * (size_t)d.n_driven);

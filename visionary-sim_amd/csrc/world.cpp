@@ -89,6 +89,7 @@ struct Pool {
 
 struct BodyRec {
     uint64_t node_begin = 0, node_count = 0;
+    uint64_t surf_begin = 0, surf_count = 0;  // its contact records (0 without contact)
     uint64_t color_begin[8] = {}, color_count[8] = {};
     uint64_t splat_begin = 0, splat_count = 0;
     float damping = 0;
@@ -500,6 +501,52 @@ const T *keep(vs_world *w, std::vector<T> &&v)
     return reinterpret_cast<const T *>(w->pending.back().data());
 }
 
+// Per-body sizes of an admission, and the device temporaries of a lattice
+// built from occupancy.
+struct BodyPlan {
+    uint32_t n_cells = 0, n_surface = 0, color_count[8] = {};
+    float origin[3] = {}, h = 0.f;
+    bool device = false;
+    uint8_t *d_occ = nullptr;
+    uint32_t *d_counts = nullptr, *d_offsets = nullptr;
+    uint32_t tiles = 0, node_tiles = 0, tile_base[9] = {};
+    std::vector<uint32_t> counts, offsets;
+};
+
+// buildLattice's inverse mass of a node with k occupied incident cells: the
+// f64 corner mass density*h*h*h/8 added k times into an f32 slot, then
+// 1/mass in f64, stored f32 (lattice.js:190-233).
+void mass_table(double density, double h, float out[9])
+{
+    const double corner = density * h * h * h / 8;
+    float mass = 0.f;
+    out[0] = 0.f;
+    for (int k = 1; k <= 8; k++) {
+        mass = (float)((double)mass + corner);
+        out[k] = mass > 0.f ? (float)(1.0 / (double)mass) : 0.f;
+    }
+}
+
+vsim::LatticeView lattice_view(const vs_body_desc &bd, const BodyPlan &pl, uint64_t node_offset, uint64_t body)
+{
+    vsim::LatticeView L{};
+    L.occ = pl.d_occ;
+    for (int a = 0; a < 3; a++) {
+        L.cell_dims[a] = bd.cell_dims[a];
+        L.origin[a] = bd.origin64[a];
+        L.velocity[a] = bd.velocity[a];
+    }
+    L.node_offset = (uint32_t)node_offset;
+    L.body = (uint32_t)body;
+    L.radius = bd.contact_radius;
+    L.h = bd.h64;
+    mass_table(bd.density, bd.h64, L.wtab);
+    for (int c = 0; c < 9; c++) L.tile_base[c] = pl.tile_base[c];
+    L.tile_count = pl.d_counts;
+    L.tile_offset = pl.d_offsets;
+    return L;
+}
+
 float damp_factor(float damping, float dt)
 {
     // exp(-damping * dt) of finalize.wgsl:72: the f32 product of the two
@@ -640,29 +687,108 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
     if (!w || !d || !scene_id || (d->n_bodies && !d->bodies)) return VS_E_INVALID;
     if (d->n_bodies == 0) return fail(w, VS_E_INVALID, "a scene needs at least one body");
     if (!(d->dt > 0.0f)) return fail(w, VS_E_INVALID, "stepDuration must be a finite positive number");
-    // Sizes.
+    HIPW(w, hipSetDevice(w->cfg.device));
+    hipStream_t up = w->upload;
+    // Retire host copies whose uploads have landed.
+    if (hipEventQuery(w->upload_done) == hipSuccess) w->pending.clear();
+    // Device temporaries of lattice builds, released in stream order on the
+    // upload stream whichever way this returns.
+    struct Temps {
+        hipStream_t s;
+        std::vector<void *> p;
+        ~Temps()
+        {
+            for (void *q : p) (void)hipFreeAsync(q, s);
+        }
+    } temps{ up, {} };
+    auto temp = [&](void **out, size_t bytes) -> hipError_t {
+        const hipError_t e = hipMallocAsync(out, std::max<size_t>(bytes, 4), up);
+        if (e == hipSuccess) temps.p.push_back(*out);
+        return e;
+    };
+    // Sizes. A body either brings its lattice (buildLattice on the host) or
+    // an occupancy grid the device turns into one; for those, a count pass
+    // runs first so that the pools can be sized.
+    std::vector<BodyPlan> plan(d->n_bodies);
     SceneRec r;
     uint64_t nodes = 0, ncells = 0, nsurf = 0, nsplats = 0;
+    bool counting = false;
     for (uint32_t b = 0; b < d->n_bodies; b++) {
         const vs_body_desc &bd = d->bodies[b];
-        if (!bd.rest || !bd.inv_mass || (bd.n_cells && !bd.cells)) return fail(w, VS_E_INVALID, "body arrays missing");
-        uint64_t colored = 0;
-        for (int c = 0; c < 8; c++) colored += bd.color_count[c];
-        if (colored != bd.n_cells) return fail(w, VS_E_INVALID, "colour counts do not sum to the cell count");
-        if (!(bd.h > 0.0f)) return fail(w, VS_E_INVALID, "cell size must be positive");
+        BodyPlan &pl = plan[b];
         const uint64_t cells_total = (uint64_t)bd.cell_dims[0] * bd.cell_dims[1] * bd.cell_dims[2];
-        for (uint32_t i = 0; i < bd.n_cells; i++) {
-            if (bd.cells[i] >= cells_total) return fail(w, VS_E_INVALID, "cell index outside the lattice");
-        }
         const uint64_t nn = (uint64_t)(bd.cell_dims[0] + 1) * (bd.cell_dims[1] + 1) * (bd.cell_dims[2] + 1);
-        for (uint32_t i = 0; i < bd.n_surface; i++) {
-            if (bd.surface[i] >= nn) return fail(w, VS_E_INVALID, "surface node outside the lattice");
+        if (bd.occupancy) {
+            if (!(bd.h64 > 0.0) || !(bd.density > 0.0)) return fail(w, VS_E_INVALID, "cell size and density must be positive");
+            if (cells_total == 0 || nn >= (1ull << 31)) return fail(w, VS_E_INVALID, "lattice dimensions out of range");
+            for (uint32_t i = 0; i < bd.n_static; i++) {
+                if (!bd.static_nodes || bd.static_nodes[i] >= nn) return fail(w, VS_E_INVALID, "static node outside the lattice");
+            }
+            pl.device = true;
+            for (int a = 0; a < 3; a++) pl.origin[a] = (float)bd.origin64[a];
+            pl.h = (float)bd.h64;
+            pl.tiles = vsim::lattice_tiles(bd.cell_dims, pl.tile_base, &pl.node_tiles);
+            HIPW(w, temp(reinterpret_cast<void **>(&pl.d_occ), cells_total));
+            HIPW(w, temp(reinterpret_cast<void **>(&pl.d_counts), 4ull * pl.tiles));
+            HIPW(w, temp(reinterpret_cast<void **>(&pl.d_offsets), 4ull * pl.tiles));
+            const uint8_t *ho = keep(w, std::vector<uint8_t>(bd.occupancy, bd.occupancy + cells_total));
+            HIPW(w, hipMemcpyAsync(pl.d_occ, ho, cells_total, hipMemcpyHostToDevice, up));
+            vsim::LatticeView L = lattice_view(bd, pl, 0, 0);
+            HIPW(w, vsim::launch_lattice_lists(up, L, pl.tiles, 0));
+            pl.counts.resize(pl.tiles);
+            HIPW(w, hipMemcpyAsync(pl.counts.data(), pl.d_counts, 4ull * pl.tiles, hipMemcpyDeviceToHost, up));
+            counting = true;
+        } else {
+            if (!bd.rest || !bd.inv_mass || (bd.n_cells && !bd.cells)) return fail(w, VS_E_INVALID, "body arrays missing");
+            uint64_t colored = 0;
+            for (int c = 0; c < 8; c++) colored += bd.color_count[c];
+            if (colored != bd.n_cells) return fail(w, VS_E_INVALID, "colour counts do not sum to the cell count");
+            if (!(bd.h > 0.0f)) return fail(w, VS_E_INVALID, "cell size must be positive");
+            for (uint32_t i = 0; i < bd.n_cells; i++) {
+                if (bd.cells[i] >= cells_total) return fail(w, VS_E_INVALID, "cell index outside the lattice");
+            }
+            for (uint32_t i = 0; i < bd.n_surface; i++) {
+                if (bd.surface[i] >= nn) return fail(w, VS_E_INVALID, "surface node outside the lattice");
+            }
+            pl.n_cells = bd.n_cells;
+            pl.n_surface = bd.n_surface;
+            for (int c = 0; c < 8; c++) pl.color_count[c] = bd.color_count[c];
+            for (int a = 0; a < 3; a++) pl.origin[a] = bd.origin[a];
+            pl.h = bd.h;
         }
         nodes += nn;
-        ncells += bd.n_cells;
-        if (d->contact) nsurf += bd.n_surface;
         nsplats += bd.n_splats;
         r.stretch |= bd.max_stretch > 1.0f;
+    }
+    if (counting) {
+        HIPW(w, hipStreamSynchronize(up));
+        for (BodyPlan &pl : plan) {
+            if (!pl.device) continue;
+            // Exclusive offsets: one running sum over the colour tiles (so it
+            // carries each colour's base), another over the node tiles.
+            pl.offsets.resize(pl.tiles);
+            uint32_t at = 0;
+            for (uint32_t t = 0; t < pl.tile_base[8]; t++) {
+                pl.offsets[t] = at;
+                at += pl.counts[t];
+            }
+            pl.n_cells = at;
+            for (int c = 0; c < 8; c++) {
+                uint32_t n = 0;
+                for (uint32_t t = pl.tile_base[c]; t < pl.tile_base[c + 1]; t++) n += pl.counts[t];
+                pl.color_count[c] = n;
+            }
+            at = 0;
+            for (uint32_t t = pl.tile_base[8]; t < pl.tiles; t++) {
+                pl.offsets[t] = at;
+                at += pl.counts[t];
+            }
+            pl.n_surface = at;
+        }
+    }
+    for (const BodyPlan &pl : plan) {
+        ncells += pl.n_cells;
+        if (d->contact) nsurf += pl.n_surface;
     }
     for (uint32_t i = 0; i < d->n_driven; i++) {
         if (d->driven[i] >= nodes) return fail(w, VS_E_INVALID, "driven node outside the scene");
@@ -708,8 +834,6 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
         release_ranges(w, r);
         return fail(w, VS_E_NOMEM, "scene does not fit the HBM budget");
     }
-    // Retire host copies whose uploads have landed.
-    if (hipEventQuery(w->upload_done) == hipSuccess) w->pending.clear();
 
     vs_scene_params &P = r.params;
     P = vs_scene_params{};
@@ -731,14 +855,17 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
     P.surf_count = (uint32_t)nsurf;
     P.bucket_slots = BUCKET_SLOTS;
 
-    hipStream_t up = w->upload;
     const size_t N = w->cfg.max_nodes;
-    uint64_t node_at = r.node_begin, cell_at = r.cell_begin, splat_at = r.splat_begin;
-    std::vector<uint32_t> s_node, s_body;
-    std::vector<float> s_rad;
+    uint64_t node_at = r.node_begin, cell_at = r.cell_begin, splat_at = r.splat_begin, surf_at = r.surf_begin;
+    const size_t S = w->cfg.max_surface;
+    // Hard-held nodes carry no mass (kinematicDrive.js:358-364): a host-built
+    // body's inv_mass arrives with them zeroed, a device-built one gets them
+    // zeroed here, with its static nodes.
+    const bool hard = d->n_driven && d->drive_law == 0;
     r.bodies.resize(d->n_bodies);
     for (uint32_t b = 0; b < d->n_bodies; b++) {
         const vs_body_desc &bd = d->bodies[b];
+        const BodyPlan &pl = plan[b];
         BodyRec &br = r.bodies[b];
         const uint32_t nd[3] = { bd.cell_dims[0] + 1, bd.cell_dims[1] + 1, bd.cell_dims[2] + 1 };
         const uint64_t nn = (uint64_t)nd[0] * nd[1] * nd[2];
@@ -750,12 +877,12 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
         for (int a = 0; a < 3; a++) {
             vb.cell_dims[a] = bd.cell_dims[a];
             vb.node_dims[a] = nd[a];
-            vb.origin[a] = bd.origin[a];
+            vb.origin[a] = pl.origin[a];
         }
         vb.node_offset = (uint32_t)node_at;
         vb.node_count = (uint32_t)nn;
         vb.scene = id;
-        vb.h = bd.h;
+        vb.h = pl.h;
         vb.alpha_dev = bd.alpha_dev;
         vb.alpha_hyd = bd.alpha_hyd;
         vb.gamma = bd.gamma;
@@ -763,49 +890,84 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
         vb.max_stretch = bd.max_stretch;
         vb.friction = bd.friction;
         vb.damp = damp_factor(bd.damping, d->dt);
-        // Node planes: p = prev = rest, vel = 0, w, rest.
-        std::vector<float> xyz(3 * nn);
-        for (uint64_t i = 0; i < nn; i++) {
-            xyz[i] = bd.rest[3 * i];
-            xyz[nn + i] = bd.rest[3 * i + 1];
-            xyz[2 * nn + i] = bd.rest[3 * i + 2];
-        }
-        const float *hx = keep(w, std::move(xyz));
-        for (int k : { 0, 3, 10 }) {
-            for (int a = 0; a < 3; a++)
-                HIPW(w, hipMemcpyAsync(w->d_nodes + (k + a) * N + node_at, hx + a * nn, nn * 4, hipMemcpyHostToDevice, up));
-        }
-        for (int a = 0; a < 3; a++) {
-            if (bd.velocity[a] == 0.0f) {
-                HIPW(w, hipMemsetAsync(w->d_nodes + (6 + a) * N + node_at, 0, nn * 4, up));
-            } else {
-                const float *hv = keep(w, std::vector<float>(nn, bd.velocity[a]));
-                HIPW(w, hipMemcpyAsync(w->d_nodes + (6 + a) * N + node_at, hv, nn * 4, hipMemcpyHostToDevice, up));
+        const bool surface = d->contact && pl.n_surface;
+        if (pl.device) {
+            vsim::LatticeView L = lattice_view(bd, pl, node_at, r.body_begin + b);
+            L.cells = w->d_cells + cell_at;
+            if (surface) {
+                L.s_node = w->d_surf + surf_at;
+                L.s_body = w->d_surf + S + surf_at;
+                L.s_radius = reinterpret_cast<float *>(w->d_surf + 2 * S + surf_at);
+            }
+            const uint32_t *hoff = keep(w, std::vector<uint32_t>(pl.offsets));
+            HIPW(w, hipMemcpyAsync(pl.d_offsets, hoff, 4ull * pl.tiles, hipMemcpyHostToDevice, up));
+            HIPW(w, vsim::launch_lattice_lists(up, L, surface ? pl.tiles : pl.tile_base[8], 1));
+            HIPW(w, vsim::launch_lattice_nodes(up, L, node_view(w), plane(w, 10), plane(w, 11), plane(w, 12)));
+            std::vector<uint32_t> zero;
+            for (uint32_t i = 0; i < bd.n_static; i++) zero.push_back((uint32_t)(node_at + bd.static_nodes[i]));
+            const uint64_t first = node_at - r.node_begin;
+            for (uint32_t i = 0; hard && i < d->n_driven; i++) {
+                if (d->driven[i] >= first && d->driven[i] < first + nn) zero.push_back((uint32_t)(r.node_begin + d->driven[i]));
+            }
+            if (!zero.empty()) {
+                uint32_t *dz = nullptr;
+                HIPW(w, temp(reinterpret_cast<void **>(&dz), 4 * zero.size()));
+                const size_t nz = zero.size();
+                const uint32_t *hz = keep(w, std::move(zero));
+                HIPW(w, hipMemcpyAsync(dz, hz, 4 * nz, hipMemcpyHostToDevice, up));
+                HIPW(w, vsim::launch_zero_mass(up, plane(w, 9), dz, (uint32_t)nz));
+            }
+        } else {
+            // Node planes: p = prev = rest, vel, w, rest.
+            std::vector<float> xyz(3 * nn);
+            for (uint64_t i = 0; i < nn; i++) {
+                xyz[i] = bd.rest[3 * i];
+                xyz[nn + i] = bd.rest[3 * i + 1];
+                xyz[2 * nn + i] = bd.rest[3 * i + 2];
+            }
+            const float *hx = keep(w, std::move(xyz));
+            for (int k : { 0, 3, 10 }) {
+                for (int a = 0; a < 3; a++)
+                    HIPW(w, hipMemcpyAsync(w->d_nodes + (k + a) * N + node_at, hx + a * nn, nn * 4, hipMemcpyHostToDevice, up));
+            }
+            for (int a = 0; a < 3; a++) {
+                if (bd.velocity[a] == 0.0f) {
+                    HIPW(w, hipMemsetAsync(w->d_nodes + (6 + a) * N + node_at, 0, nn * 4, up));
+                } else {
+                    const float *hv = keep(w, std::vector<float>(nn, bd.velocity[a]));
+                    HIPW(w, hipMemcpyAsync(w->d_nodes + (6 + a) * N + node_at, hv, nn * 4, hipMemcpyHostToDevice, up));
+                }
+            }
+            const float *hw = keep(w, std::vector<float>(bd.inv_mass, bd.inv_mass + nn));
+            HIPW(w, hipMemcpyAsync(w->d_nodes + 9 * N + node_at, hw, nn * 4, hipMemcpyHostToDevice, up));
+            if (bd.n_cells) {
+                const uint32_t *hc = keep(w, std::vector<uint32_t>(bd.cells, bd.cells + bd.n_cells));
+                HIPW(w, hipMemcpyAsync(w->d_cells + cell_at, hc, (size_t)bd.n_cells * 4, hipMemcpyHostToDevice, up));
+            }
+            // Surface records, body by body (bodyContact.js:72-85).
+            if (surface) {
+                std::vector<uint32_t> sn(pl.n_surface), sb(pl.n_surface, (uint32_t)(r.body_begin + b));
+                std::vector<float> sr(pl.n_surface, bd.contact_radius);
+                for (uint32_t i = 0; i < pl.n_surface; i++) sn[i] = (uint32_t)(node_at + bd.surface[i]);
+                const uint32_t *hn = keep(w, std::move(sn));
+                const uint32_t *hb = keep(w, std::move(sb));
+                const float *hr = keep(w, std::move(sr));
+                HIPW(w, hipMemcpyAsync(w->d_surf + surf_at, hn, pl.n_surface * 4ull, hipMemcpyHostToDevice, up));
+                HIPW(w, hipMemcpyAsync(w->d_surf + S + surf_at, hb, pl.n_surface * 4ull, hipMemcpyHostToDevice, up));
+                HIPW(w, hipMemcpyAsync(w->d_surf + 2 * S + surf_at, hr, pl.n_surface * 4ull, hipMemcpyHostToDevice, up));
             }
         }
-        const float *hw = keep(w, std::vector<float>(bd.inv_mass, bd.inv_mass + nn));
-        HIPW(w, hipMemcpyAsync(w->d_nodes + 9 * N + node_at, hw, nn * 4, hipMemcpyHostToDevice, up));
         // Cells, colour by colour.
         uint64_t src = 0;
         for (int c = 0; c < 8; c++) {
             br.color_begin[c] = cell_at + src;
-            br.color_count[c] = bd.color_count[c];
-            src += bd.color_count[c];
+            br.color_count[c] = pl.color_count[c];
+            src += pl.color_count[c];
         }
-        if (bd.n_cells) {
-            const uint32_t *hc = keep(w, std::vector<uint32_t>(bd.cells, bd.cells + bd.n_cells));
-            HIPW(w, hipMemcpyAsync(w->d_cells + cell_at, hc, (size_t)bd.n_cells * 4, hipMemcpyHostToDevice, up));
-        }
-        cell_at += bd.n_cells;
-        // Surface records, body by body (bodyContact.js:72-85).
-        if (d->contact) {
-            for (uint32_t i = 0; i < bd.n_surface; i++) {
-                s_node.push_back((uint32_t)(node_at + bd.surface[i]));
-                s_body.push_back((uint32_t)(r.body_begin + b));
-                s_rad.push_back(bd.contact_radius);
-            }
-
-        }
+        cell_at += pl.n_cells;
+        br.surf_begin = surf_at;
+        br.surf_count = d->contact ? pl.n_surface : 0;
+        if (d->contact) surf_at += pl.n_surface;
         // Splats.
         br.splat_begin = splat_at;
         br.splat_count = bd.n_splats;
@@ -816,11 +978,11 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
             const size_t G = w->cfg.max_splats;
             const uint32_t ns = bd.n_splats;
             std::vector<uint64_t> key(ns);
-            const double inv = 1.0 / (double)bd.h;
+            const double inv = 1.0 / (double)pl.h;
             for (uint32_t i = 0; i < ns; i++) {
                 uint64_t c[3];
                 for (int a = 0; a < 3; a++) {
-                    double g = std::floor(((double)bd.splat_mean[3 * i + a] - (double)bd.origin[a]) * inv);
+                    double g = std::floor(((double)bd.splat_mean[3 * i + a] - (double)pl.origin[a]) * inv);
                     g = std::min(std::max(g, 0.0), (double)bd.cell_dims[a] - 1.0);
                     c[a] = (uint64_t)g;
                 }
@@ -852,15 +1014,6 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
     const vs_body *hb_rec = keep(w, std::vector<vs_body>(w->h_bodies.begin() + r.body_begin,
                                                          w->h_bodies.begin() + r.body_begin + d->n_bodies));
     HIPW(w, hipMemcpyAsync(w->d_bodies + r.body_begin, hb_rec, sizeof(vs_body) * d->n_bodies, hipMemcpyHostToDevice, up));
-    if (nsurf) {
-        const size_t S = w->cfg.max_surface;
-        const uint32_t *hn = keep(w, std::move(s_node));
-        const uint32_t *hb = keep(w, std::move(s_body));
-        const float *hr = keep(w, std::move(s_rad));
-        HIPW(w, hipMemcpyAsync(w->d_surf + r.surf_begin, hn, nsurf * 4, hipMemcpyHostToDevice, up));
-        HIPW(w, hipMemcpyAsync(w->d_surf + S + r.surf_begin, hb, nsurf * 4, hipMemcpyHostToDevice, up));
-        HIPW(w, hipMemcpyAsync(w->d_surf + 2 * S + r.surf_begin, hr, nsurf * 4, hipMemcpyHostToDevice, up));
-    }
     if (nbuckets) HIPW(w, hipMemsetAsync(w->d_grid + r.bucket_begin, 0, nbuckets * 4, up));
     if (d->n_driven) {
         const size_t D = w->cfg.max_driven;
@@ -1017,6 +1170,41 @@ int vs_world_read_splats(vs_world *w, uint32_t id, float *mean, float *cov)
         if (!dst || !n) continue;
         HIPW(w, hipMemcpy(tmp.data(), w->d_splat_out + k * G + r.splat_begin, n * 4, hipMemcpyDeviceToHost));
         for (size_t j = 0; j < n; j++) dst[r.splat_perm[j]] = tmp[j];
+    }
+    return VS_OK;
+}
+
+int vs_world_body_info(vs_world *w, uint32_t id, uint32_t body, vs_body_info *out)
+{
+    if (!w || !out || id >= w->cfg.max_scenes || !w->recs[id].live || body >= w->recs[id].bodies.size())
+        return VS_E_INVALID;
+    const BodyRec &br = w->recs[id].bodies[body];
+    *out = vs_body_info{};
+    out->node_count = (uint32_t)br.node_count;
+    out->n_surface = (uint32_t)br.surf_count;
+    for (int c = 0; c < 8; c++) {
+        out->color_count[c] = (uint32_t)br.color_count[c];
+        out->n_cells += (uint32_t)br.color_count[c];
+    }
+    return VS_OK;
+}
+
+int vs_world_read_lattice(vs_world *w, uint32_t id, uint32_t body, float *rest, float *inv_mass, uint32_t *cells,
+                          uint32_t *surface)
+{
+    vs_body_info info;
+    if (int rc = vs_world_body_info(w, id, body, &info)) return rc;
+    if (int rc = vs_world_synchronize(w)) return rc;
+    const BodyRec &br = w->recs[id].bodies[body];
+    const size_t n = br.node_count;
+    for (int a = 0; a < 3 && rest; a++)
+        HIPW(w, hipMemcpy(rest + a * n, plane(w, 10 + a) + br.node_begin, n * 4, hipMemcpyDeviceToHost));
+    if (inv_mass) HIPW(w, hipMemcpy(inv_mass, plane(w, 9) + br.node_begin, n * 4, hipMemcpyDeviceToHost));
+    if (cells && info.n_cells)
+        HIPW(w, hipMemcpy(cells, w->d_cells + br.color_begin[0], info.n_cells * 4ull, hipMemcpyDeviceToHost));
+    if (surface && info.n_surface) {
+        HIPW(w, hipMemcpy(surface, w->d_surf + br.surf_begin, info.n_surface * 4ull, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < info.n_surface; i++) surface[i] -= (uint32_t)br.node_begin;
     }
     return VS_OK;
 }

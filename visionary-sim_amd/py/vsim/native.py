@@ -72,7 +72,14 @@ class VsBodyDesc(C.Structure):
         ("n_surface", C.c_uint32), ("surface", C.POINTER(C.c_uint32)), ("contact_radius", C.c_float),
         ("n_splats", C.c_uint32), ("splat_mean", C.POINTER(C.c_float)),
         ("splat_quat", C.POINTER(C.c_float)), ("splat_scale", C.POINTER(C.c_float)),
+        ("occupancy", C.POINTER(C.c_uint8)), ("origin64", C.c_double * 3), ("h64", C.c_double),
+        ("density", C.c_double), ("n_static", C.c_uint32), ("static_nodes", C.POINTER(C.c_uint32)),
     ]
+
+
+class VsBodyInfo(C.Structure):
+    _fields_ = [("node_count", C.c_uint32), ("n_cells", C.c_uint32), ("n_surface", C.c_uint32),
+                ("pad0", C.c_uint32), ("color_count", C.c_uint32 * 8)]
 
 
 class VsSceneDesc(C.Structure):
@@ -113,7 +120,7 @@ EXPORTS = [
     "vs_world_create", "vs_world_destroy", "vs_world_error", "vs_world_add_scene", "vs_world_remove_scene",
     "vs_world_reset_scene", "vs_world_set_target", "vs_world_frame", "vs_world_frame_scene",
     "vs_world_synchronize", "vs_world_record", "vs_world_elapsed", "vs_world_read_nodes",
-    "vs_world_read_splats", "vs_world_read_splat_axes", "vs_world_log_pairs", "vs_world_read_contacts", "vs_world_stats_get",
+    "vs_world_read_splats", "vs_world_read_splat_axes", "vs_world_body_info", "vs_world_read_lattice", "vs_world_log_pairs", "vs_world_read_contacts", "vs_world_stats_get",
     "vs_world_autotune", "vs_world_profile", "vs_world_time_pass",
     "vs_server_create", "vs_server_destroy", "vs_server_error", "vs_server_submit", "vs_server_run",
     "vs_server_latencies", "vs_server_result", "vs_server_times",
@@ -161,6 +168,8 @@ def load(path=LIB_PATH):
         "vs_world_read_nodes": (C.c_int, [p, u32, p, p]),
         "vs_world_read_splats": (C.c_int, [p, u32, p, p]),
         "vs_world_read_splat_axes": (C.c_int, [p, u32, p, p, p]),
+        "vs_world_body_info": (C.c_int, [p, u32, u32, C.POINTER(VsBodyInfo)]),
+        "vs_world_read_lattice": (C.c_int, [p, u32, u32, p, p, p, p]),
         "vs_world_log_pairs": (C.c_int, [p, u32]),
         "vs_world_read_contacts": (C.c_int, [p, u32, p, p, p]),
         "vs_world_stats_get": (C.c_int, [p, C.POINTER(VsWorldStats)]),
@@ -225,9 +234,10 @@ class World:
         except Exception:
             pass
 
-    def add(self, scene):
-        """Admits a scenes.Scene; returns its id (raises VsimError on VS_E_NOMEM)."""
-        desc, keep = scene.to_desc()
+    def add(self, scene, device_lattice=False):
+        """Admits a scenes.Scene; returns its id (raises MemoryError on VS_E_NOMEM).
+        `device_lattice`: send occupancy grids and build the lattices on the device."""
+        desc, keep = scene.to_desc(device_lattice)
         sid = C.c_uint32()
         rc = self.lib.vs_world_add_scene(self.h, C.byref(desc), C.byref(sid))
         del keep
@@ -237,9 +247,9 @@ class World:
         self.scenes[sid.value] = scene
         return sid.value
 
-    def try_add(self, scene):
+    def try_add(self, scene, device_lattice=False):
         try:
-            return self.add(scene)
+            return self.add(scene, device_lattice)
         except MemoryError:
             return None
 
@@ -287,6 +297,25 @@ class World:
         cov = np.empty((6, g), np.float32)
         self._check(self.lib.vs_world_read_splats(self.h, sid, mean.ctypes.data, cov.ctypes.data), "read_splats")
         return mean, cov
+
+    def body_info(self, sid, body):
+        info = VsBodyInfo()
+        self._check(self.lib.vs_world_body_info(self.h, sid, body, C.byref(info)), "body_info")
+        return {"node_count": info.node_count, "n_cells": info.n_cells, "n_surface": info.n_surface,
+                "color_count": list(info.color_count)}
+
+    def read_lattice(self, sid, body):
+        """The admitted lattice of one body: rest (3, N), GPU inverse mass (N,),
+        cells by colour, surface nodes (body-local, empty without contact)."""
+        info = self.body_info(sid, body)
+        n = info["node_count"]
+        rest = np.empty((3, n), np.float32)
+        w = np.empty(n, np.float32)
+        cells = np.empty(info["n_cells"], np.uint32)
+        surf = np.empty(info["n_surface"], np.uint32)
+        self._check(self.lib.vs_world_read_lattice(self.h, sid, body, rest.ctypes.data, w.ctypes.data,
+                                                   cells.ctypes.data, surf.ctypes.data), "read_lattice")
+        return rest, w, cells, surf, info
 
     def read_splat_axes(self, sid):
         """Means (3, G), rotations (4, G, xyzw) and axis lengths (3, G) of the
@@ -364,8 +393,8 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, scene, arrival, frames):
-        desc, keep = scene.to_desc()
+    def submit(self, scene, arrival, frames, device_lattice=False):
+        desc, keep = scene.to_desc(device_lattice)
         targets = None
         if scene.drive is not None:
             fdt = scene.substeps * scene.dt

@@ -94,6 +94,8 @@ class Lattice:
     color_counts: np.ndarray  # (8,)
     mass: float
     volume: float
+    occupancy: np.ndarray = None  # (cz, cy, cx) bool, the grid it was built from
+    density: float = None
 
     @property
     def node_count(self):
@@ -136,7 +138,7 @@ def build_lattice(occupancy, origin, h, density):
     counts = np.array([(col == c).sum() for c in range(COLOR_COUNT)], np.uint32)
     occupied = int(occ.sum())
     return Lattice(tuple(float(v) for v in origin), float(h), (cx, cy, cz), (nx, ny, nz), rest, inv, cells, counts,
-                   occupied * density * h ** 3, occupied * h ** 3)
+                   occupied * density * h ** 3, occupied * h ** 3, occ, float(density))
 
 
 def surface_nodes(lat):
@@ -320,7 +322,9 @@ class Scene:
         return origin, float(np.float32(edge)), dims
 
     # -------- C ABI descriptor
-    def to_desc(self):
+    def to_desc(self, device_lattice=False):
+        """The scene as a vs_scene_desc. With `device_lattice`, every body goes
+        as its occupancy grid and the device builds the lattice."""
         keep = []
         offs = self.node_offsets()
         pres = self.prescribed()
@@ -352,6 +356,16 @@ class Scene:
             d.n_surface = len(surf)
             d.surface = native.ptr(surf, C.c_uint32)
             d.contact_radius = lat.h * CONTACT_RADIUS_CELLS
+            if device_lattice:
+                occ = np.ascontiguousarray(lat.occupancy, np.uint8).reshape(-1)
+                st = np.ascontiguousarray(b.static_nodes if b.static_nodes is not None else [], np.uint32)
+                keep += [occ, st]
+                d.occupancy = occ.ctypes.data_as(C.POINTER(C.c_uint8))
+                d.origin64[:] = lat.origin
+                d.h64 = lat.h
+                d.density = lat.density
+                d.n_static = len(st)
+                d.static_nodes = native.ptr(st, C.c_uint32)
             if b.n_splats:
                 mean = np.ascontiguousarray(b.splat_mean, np.float32)
                 quat = np.ascontiguousarray(b.splat_quat, np.float32)
