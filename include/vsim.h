@@ -107,7 +107,20 @@ typedef struct vs_backend vs_backend;
 #define VS_K_DRIVE_HARD 7       /* sim-drive-hard       driveHard.wgsl:15-35      */
 #define VS_K_DRIVE_COMPLIANT 8  /* sim-drive-compliant  driveCompliant.wgsl:24-41 */
 #define VS_K_SPLAT 9            /* splat deformation    sample.wgsl:40-80 + covariance.wgsl:30-42 */
-#define VS_K_COUNT 10
+#define VS_K_PICK 10            /* sim-pick-ray         nearestNodeToRay, pick.js:41-68           */
+#define VS_K_GRIP 11            /* sim-grip-radius      withinRadius, grip.js:99-138              */
+#define VS_K_COUNT 12
+
+/* The question a grab asks (simulator.js:383-425), the `params` buffer of
+ * VS_K_PICK and VS_K_GRIP. f64, as the JS computes it. */
+typedef struct vs_grab_params {
+    double origin[3];      /* pick: ray origin                                  */
+    double direction[3];   /* pick: unit ray direction                          */
+    double radius;         /* pick: aim tolerance; grip: hand radius, metres    */
+    uint32_t anchor;       /* grip: the node the ray picked                     */
+    uint32_t begin, end;   /* grip: the anchor's body's node range (bodyOfNode) */
+    uint32_t pad0;
+} vs_grab_params;
 
 /* Outputs of the splat pass (a bit mask). VS_SPLAT_AXES is the engine's splat
  * format: the covariance taken apart into a unit rotation (xyzw) and three
@@ -120,7 +133,8 @@ typedef struct vs_backend vs_backend;
 #define VS_B_POS 0        /* nodePos  (3 planes)                              */
 #define VS_B_PREV 1       /* nodePrev (3 planes)                              */
 #define VS_B_VEL 2        /* nodeVel  (3 planes)                              */
-#define VS_B_INVMASS 3    /* nodeInvMass                                      */
+#define VS_B_INVMASS 3    /* nodeInvMass (pick, grip: material inverse mass with
+                             static nodes zeroed, the grabbable test)         */
 #define VS_B_BODIES 4     /* vs_body[]                                        */
 #define VS_B_SCENES 5     /* vs_scene_params[] (the pass's uniforms)          */
 #define VS_B_TILES 6      /* tile buffer                                      */
@@ -131,7 +145,14 @@ typedef struct vs_backend vs_backend;
                              splats: float4 quaternion xyzw per splat, then 6 planes
                                      (mean x, y, z, scale x, y, z)                */
 #define VS_B_GRID 8       /* contact: counters then slots                     */
-#define VS_B_OUT 9        /* splats: 9 planes (mean x, y, z, cov xx xy xz yy yz zz),
+#define VS_B_OUT 9        /* pick: {u32, i32 node (-1 = miss), f64 depth} then 16 B per
+                                   workgroup;
+                             grip: {u32 count, u32 done, u32 found, u32 kept, u32 status
+                                   (0 ok, 1 anchor not grabbable, 2 over capacity,
+                                   3 bad range), 3 pad}, then `pair_cap` u32 held nodes
+                                   (anchor first, nearest first; padded to 8 B), then
+                                   `pair_cap` f64 xyz offsets from the anchor;
+                             splats: 9 planes (mean x, y, z, cov xx xy xz yy yz zz),
                                      16 with VS_SPLAT_AXES (then rotation x, y, z, w
                                      and axis lengths x, y, z);
                              contact collide: optional pair log, i32 count per
@@ -146,7 +167,8 @@ typedef struct vs_launch {
     uint32_t items;     /* floats per plane of the item buffer (contact, drive, splats) */
     uint32_t flags;     /* elastic: 1 = limit-only sweep; splats: VS_SPLAT_* outputs (0 = cov) */
     uint32_t buckets;   /* contact: buckets in the grid buffer (counters, then 32 slots each) */
-    uint32_t pair_cap;  /* contact collide: pair log capacity per record (0 = off) */
+    uint32_t pair_cap;  /* contact collide: pair log capacity per record (0 = off);
+                           grip: most nodes held, anchor included (drive capacity) */
     uint64_t buffers[VS_B_COUNT];
 } vs_launch;
 

@@ -839,6 +839,80 @@ void o_ellipsoid_axes(const REAL cov6[6], REAL rot[4], REAL scale3[3])
     for (int k = 0; k < 3; k++) scale3[k] = SQRT(d[k] > (real)0 ? d[k] : (real)0);
 }
 
+/* nearestNodeToRay (packages/visionary-sim/src/sim/pick.js:41-68): the
+ * grabbable node (mass > 0; `mass` is the material inverse mass with static
+ * nodes zeroed) with the smallest depth along the ray among those in front of
+ * it and within `radius` of the line, the lower index on equal depth; -1 for
+ * a miss. f64 arithmetic over the f32 positions, as the JS does it. */
+int64_t o_pick_ray(const float *px, const float *py, const float *pz, const float *mass, uint32_t n,
+                   const double origin[3], const double dir[3], double radius, double *depth_out)
+{
+    const double r2 = radius * radius;
+    int64_t best = -1;
+    double best_depth = INFINITY;
+    for (uint32_t i = 0; i < n; i++) {
+        if (!(mass[i] > 0.0f)) continue;
+        const double dx = (double)px[i] - origin[0], dy = (double)py[i] - origin[1], dz = (double)pz[i] - origin[2];
+        const double depth = dx * dir[0] + dy * dir[1] + dz * dir[2];
+        if (depth <= 0 || depth >= best_depth) continue;
+        const double off = dx * dx + dy * dy + dz * dz - depth * depth;
+        if (off > r2) continue;
+        best = i;
+        best_depth = depth;
+    }
+    if (depth_out) *depth_out = best < 0 ? 0.0 : best_depth;
+    return best;
+}
+
+typedef struct { double d2; uint32_t node; } grip_cand;
+
+static int grip_cmp(const void *a, const void *b)
+{
+    const grip_cand *x = (const grip_cand *)a, *y = (const grip_cand *)b;
+    if (x->d2 != y->d2) return x->d2 < y->d2 ? -1 : 1;
+    return x->node < y->node ? -1 : (x->node > y->node);
+}
+
+/* withinRadius (grip.js:99-138): every grabbable node of [begin, end) within
+ * `radius` of the anchor, nearest first with ties by index, truncated to
+ * `limit` entries (anchor included, first, zero offset). Returns the entries
+ * written; *found counts all of them (0 when the anchor is not grabbable). */
+uint32_t o_grip_radius(const float *px, const float *py, const float *pz, const float *mass, uint32_t begin,
+                       uint32_t end, uint32_t anchor, double radius, uint32_t limit, uint32_t *nodes,
+                       double *offsets, uint32_t *found)
+{
+    *found = 0;
+    if (!(mass[anchor] > 0.0f)) return 0;
+    const double ax = px[anchor], ay = py[anchor], az = pz[anchor];
+    const double r2 = radius * radius;
+    grip_cand *c = (grip_cand *)malloc(sizeof(grip_cand) * (end - begin + 1));
+    uint32_t m = 0;
+    for (uint32_t i = begin; i < end; i++) {
+        if (i == anchor || !(mass[i] > 0.0f)) continue;
+        const double dx = (double)px[i] - ax, dy = (double)py[i] - ay, dz = (double)pz[i] - az;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 <= r2) {
+            c[m].d2 = d2;
+            c[m].node = i;
+            m++;
+        }
+    }
+    qsort(c, m, sizeof(grip_cand), grip_cmp);
+    *found = m + 1;
+    const uint32_t keep = m < (limit > 0 ? limit - 1 : 0) ? m : (limit > 0 ? limit - 1 : 0);
+    nodes[0] = anchor;
+    offsets[0] = offsets[1] = offsets[2] = 0.0;
+    for (uint32_t k = 0; k < keep; k++) {
+        const uint32_t i = c[k].node;
+        nodes[1 + k] = i;
+        offsets[3 * (1 + k)] = (double)px[i] - ax;
+        offsets[3 * (1 + k) + 1] = (double)py[i] - ay;
+        offsets[3 * (1 + k) + 2] = (double)pz[i] - az;
+    }
+    free(c);
+    return keep + 1;
+}
+
 /* Residuals of one cell's six tetrahedra at the current positions, for the
  * error report: out[12] = (C_D, C_H) per tet. */
 void o_cell_residuals(const o_scene *sc, uint32_t body, uint32_t lin, REAL *out12)

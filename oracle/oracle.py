@@ -74,6 +74,10 @@ def load(precision="f32"):
     lib.o_elastic_color.argtypes = [sp, C.c_int, C.c_int]
     lib.o_deform_splats.argtypes = [sp, C.POINTER(OSplats)]
     lib.o_ellipsoid_axes.argtypes = [C.c_void_p] * 3
+    lib.o_pick_ray.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+    lib.o_pick_ray.restype = C.c_int64
+    lib.o_grip_radius.argtypes = [C.c_void_p] * 4 + [C.c_uint32] * 3 + [C.c_double, C.c_uint32] + [C.c_void_p] * 3
+    lib.o_grip_radius.restype = C.c_uint32
     lib.o_project_tet.argtypes = [C.c_void_p] * 4 + [C.c_double] * 4 + [C.c_void_p]
     lib.o_project_kuhn_tet.argtypes = [C.c_void_p] * 3 + [C.c_int] + [C.c_double] * 5 + [C.c_int, C.c_void_p]
     lib.o_cell_residuals.argtypes = [sp, C.c_uint32, C.c_uint32, C.c_void_p]
@@ -81,6 +85,36 @@ def load(precision="f32"):
     assert lib.o_real_size() == (4 if precision == "f32" else 8)
     _libs[precision] = lib
     return lib
+
+
+def _planes(pos):
+    p = np.ascontiguousarray(np.asarray(pos, np.float32).reshape(3, -1))
+    return p, [p[k].ctypes.data for k in range(3)]
+
+
+def pick_ray(pos, mass, origin, direction, radius):
+    """nearestNodeToRay (pick.js:41-68) over positions (3, N) and the grab
+    mass (material inverse mass, static nodes zeroed): (node or -1, depth)."""
+    p, ptrs = _planes(pos)
+    m = np.ascontiguousarray(mass, np.float32)
+    o = np.ascontiguousarray(origin, np.float64)
+    d = np.ascontiguousarray(direction, np.float64)
+    depth = C.c_double()
+    node = load("f64").o_pick_ray(*ptrs, m.ctypes.data, len(m), o.ctypes.data, d.ctypes.data, float(radius),
+                                  C.byref(depth))
+    return int(node), depth.value
+
+
+def grip_radius(pos, mass, anchor, begin, end, radius, limit):
+    """withinRadius (grip.js:99-138): (nodes, offsets (k, 3) f64, found)."""
+    p, ptrs = _planes(pos)
+    m = np.ascontiguousarray(mass, np.float32)
+    nodes = np.zeros(max(limit, 1), np.uint32)
+    offs = np.zeros((max(limit, 1), 3), np.float64)
+    found = C.c_uint32()
+    k = load("f64").o_grip_radius(*ptrs, m.ctypes.data, begin, end, anchor, float(radius), limit,
+                                  nodes.ctypes.data, offs.ctypes.data, C.byref(found))
+    return nodes[:k].copy(), offs[:k].copy(), found.value
 
 
 def ellipsoid_axes(cov6, precision="f64"):

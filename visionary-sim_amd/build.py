@@ -33,7 +33,7 @@ FP_FLAGS = [
 ]
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
-SOURCES = ["kernels.hip", "lattice.hip", "backend.cpp", "world.cpp", "server.cpp"]
+SOURCES = ["kernels.hip", "lattice.hip", "grab.hip", "backend.cpp", "world.cpp", "server.cpp"]
 
 
 def _run(cmds):

@@ -40,6 +40,7 @@ int hip_fail(vs_backend *b, hipError_t e, const char *what)
 const char *const KERNEL_NAMES[VS_K_COUNT] = {
     "sim-predict",         "sim-finalize",        "sim-elastic-cells", "sim-contact-bin", "sim-contact-sort",
     "sim-contact-collide", "sim-contact-settle", "sim-drive-hard",    "sim-drive-compliant", "sim-splat-deform",
+    "sim-pick-ray",        "sim-grip-radius",
 };
 
 const char *const SLOT_NAMES[VS_B_COUNT] = { "nodePos", "nodePrev", "nodeVel", "nodeInvMass", "bodies",
@@ -62,6 +63,9 @@ uint32_t required_slots(int kernel)
     case VS_K_DRIVE_COMPLIANT: return node | (1u << VS_B_VEL) | common | (1u << VS_B_ITEMS);
     case VS_K_SPLAT: return (1u << VS_B_POS) | (1u << VS_B_BODIES) | (1u << VS_B_TILES) | (1u << VS_B_ITEMS) |
                             (1u << VS_B_OUT);
+    case VS_K_PICK: return (1u << VS_B_POS) | (1u << VS_B_INVMASS) | (1u << VS_B_SCENES) | (1u << VS_B_OUT);
+    case VS_K_GRIP:
+        return (1u << VS_B_POS) | (1u << VS_B_INVMASS) | (1u << VS_B_SCENES) | (1u << VS_B_ITEMS) | (1u << VS_B_OUT);
     default: return 0;
     }
 }
@@ -76,16 +80,19 @@ uint64_t slot_bytes(const vs_launch &l, int slot)
     case VS_B_VEL: return 12 * P;
     case VS_B_INVMASS: return 4 * P;
     case VS_B_BODIES: return sizeof(vs_body);
-    case VS_B_SCENES: return sizeof(vs_scene_params);
+    case VS_B_SCENES: return l.kernel == VS_K_PICK || l.kernel == VS_K_GRIP ? sizeof(vs_grab_params) : sizeof(vs_scene_params);
     case VS_B_TILES: return 16;
     case VS_B_ITEMS:
         if (l.kernel == VS_K_ELASTIC) return 4;
         if (l.kernel == VS_K_DRIVE_HARD || l.kernel == VS_K_DRIVE_COMPLIANT) return 16 * I;
         if (l.kernel == VS_K_SPLAT) return 40 * I;
+        if (l.kernel == VS_K_GRIP) return 16 * I;  // candidates: f64 distance^2, u32 node, pad
         return 36 * I;  // nine surface planes
     case VS_B_GRID: return 4ull * l.buckets * (1 + 32);
     case VS_B_OUT:
         if (l.kernel == VS_K_SPLAT) return (l.flags & VS_SPLAT_AXES ? 64 : 36) * I;
+        if (l.kernel == VS_K_PICK) return 16ull + 16ull * l.grid;
+        if (l.kernel == VS_K_GRIP) return 32ull + 4ull * (l.pair_cap + (l.pair_cap & 1u)) + 24ull * l.pair_cap;
         return l.pair_cap ? 4ull * I * (1 + l.pair_cap) : 0;
     default: return 0;
     }
@@ -306,6 +313,17 @@ int vs_dispatch(vs_backend *b, const vs_launch *launches, uint32_t count)
             e = launch_drive(b->stream, sh, n, scenes, d, t, l.kernel == VS_K_DRIVE_COMPLIANT ? 1 : 0);
             break;
         }
+        case VS_K_PICK:
+            e = launch_pick(b->stream, l.grid, n.px, n.py, n.pz, n.w, l.plane,
+                            reinterpret_cast<const vs_grab_params *>(l.buffers[VS_B_SCENES]),
+                            reinterpret_cast<void *>(l.buffers[VS_B_OUT]));
+            break;
+        case VS_K_GRIP:
+            e = launch_grip(b->stream, l.grid, n.px, n.py, n.pz, n.w, l.plane,
+                            reinterpret_cast<const vs_grab_params *>(l.buffers[VS_B_SCENES]),
+                            reinterpret_cast<void *>(l.buffers[VS_B_ITEMS]), l.items, l.pair_cap,
+                            reinterpret_cast<void *>(l.buffers[VS_B_OUT]));
+            break;
         case VS_K_SPLAT: {
             float *in = F(VS_B_ITEMS), *out = F(VS_B_OUT);
             const uint32_t mode = l.flags & (VS_SPLAT_COV | VS_SPLAT_AXES) ? l.flags & (VS_SPLAT_COV | VS_SPLAT_AXES)

@@ -113,6 +113,12 @@ uint32_t lattice_tiles(const uint32_t cell_dims[3], uint32_t tile_base[9], uint3
 hipError_t launch_lattice_lists(hipStream_t s, const LatticeView &L, uint32_t tiles, uint32_t write);
 hipError_t launch_lattice_nodes(hipStream_t s, const LatticeView &L, NodeView n, float *rx, float *ry, float *rz);
 hipError_t launch_zero_mass(hipStream_t s, float *w, const uint32_t *nodes, uint32_t count);
+// Grab queries (grab.hip): one launch each; `out` as VS_B_OUT documents.
+hipError_t launch_pick(hipStream_t s, uint32_t grid, const float *px, const float *py, const float *pz,
+                       const float *mass, uint32_t count, const vs_grab_params *params, void *out);
+hipError_t launch_grip(hipStream_t s, uint32_t grid, const float *px, const float *py, const float *pz,
+                       const float *mass, uint32_t count, const vs_grab_params *params, void *candidates,
+                       uint32_t cap, uint32_t limit, void *out);
 hipError_t launch_splats(hipStream_t s, Shape sh, const float *px, const float *py, const float *pz,
                          const vs_body *bodies, SplatView sp, TileView t);
 

@@ -201,6 +201,76 @@ class VsimError(RuntimeError):
     pass
 
 
+SLOTS = ["nodePos", "nodePrev", "nodeVel", "nodeInvMass", "bodies", "params", "tiles", "items", "contactGrid", "out"]
+
+
+class VsGrabParams(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("direction", C.c_double * 3), ("radius", C.c_double),
+                ("anchor", C.c_uint32), ("begin", C.c_uint32), ("end", C.c_uint32), ("pad0", C.c_uint32)]
+
+
+class Backend:
+    """Layer 1 (ComputeBackend's seam): device buffers and named launches."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.vs_backend_create(device, C.byref(h))
+        if rc != VS_OK:
+            raise VsimError(f"vs_backend_create: {self.lib.vs_last_error(None).decode()} ({rc})")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != VS_OK:
+            raise VsimError(f"{what}: {self.lib.vs_last_error(self.h).decode()} ({rc})")
+
+    def close(self):
+        if self.h:
+            self.lib.vs_backend_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def alloc(self, nbytes):
+        out = C.c_uint64()
+        self._check(self.lib.vs_buffer_alloc(self.h, int(nbytes), C.byref(out)), "alloc")
+        return out.value
+
+    def free(self, handle):
+        self._check(self.lib.vs_buffer_free(self.h, handle), "free")
+
+    def write(self, handle, data, offset=0):
+        buf = bytes(data) if isinstance(data, C.Structure) else np.ascontiguousarray(data).tobytes()
+        self._check(self.lib.vs_buffer_write(self.h, handle, offset, buf, len(buf)), "write")
+
+    def read(self, handle, nbytes, offset=0):
+        out = (C.c_char * nbytes)()
+        self._check(self.lib.vs_buffer_read(self.h, handle, offset, nbytes, out), "read")
+        return bytes(out)
+
+    def dispatch(self, *launches):
+        arr = (VsLaunch * len(launches))(*launches)
+        self._check(self.lib.vs_dispatch(self.h, arr, len(launches)), "dispatch")
+
+    def sync(self):
+        self._check(self.lib.vs_synchronize(self.h), "synchronize")
+
+    def launch(self, kernel, grid, block=64, plane=0, items=0, flags=0, buckets=0, pair_cap=0, **buffers):
+        """A vs_launch record; `buffers` by slot name (nodePos=..., out=...)."""
+        kid = self.lib.vs_kernel_id(kernel.encode())
+        if kid < 0:
+            raise VsimError(f"unknown kernel {kernel}")
+        l = VsLaunch(kernel=kid, grid=grid, block=block, plane=plane, items=items, flags=flags, buckets=buckets,
+                     pair_cap=pair_cap)
+        for name, handle in buffers.items():
+            l.buffers[SLOTS.index(name)] = handle
+        return l
+
+
 class World:
     """A vs_world: scenes resident on one GPU, stepped batched or plain."""
 
