@@ -2,7 +2,8 @@
 // test/helpers/fakeCompute.js): allocations are host byte arrays, launches
 // are recorded, nothing executes.
 const IDS = ['sim-predict', 'sim-finalize', 'sim-elastic-cells', 'sim-contact-bin', 'sim-contact-sort',
-    'sim-contact-collide', 'sim-contact-settle', 'sim-drive-hard', 'sim-drive-compliant', 'sim-splat-deform'];
+    'sim-contact-collide', 'sim-contact-settle', 'sim-drive-hard', 'sim-drive-compliant', 'sim-splat-deform',
+    'sim-pick-ray', 'sim-grip-radius'];
 
 export function fakeAddon() {
     let next = 256n;

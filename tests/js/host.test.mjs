@@ -115,6 +115,57 @@ export const tests = {
         sim.destroy();
         backend.destroy();
     },
+    async 'a grab is one pick and one grip launch and reads back only the answer'() {
+        const native = fakeAddon();
+        const backend = new ComputeBackend(0, native);
+        const sim = new Simulator(backend, [jelly(4)], { min: [-1, -1, -1], max: [1, 1, 1] });
+        const N = sim.particles.nodeCount;
+        // Grab mass: the material inverse mass with the static nodes zeroed.
+        const grab = new Float32Array(native.memory.get(sim.particles.nodeGrabMass.native).buffer, 0, N);
+        for (let n = 0; n < N; n++) {
+            const want = sim.particles.staticPrescribed.has(n) ? 0 : sim.particles.invMass[n];
+            assert.strictEqual(grab[n], want);
+        }
+        // The fake runs nothing: plant the device's answers (node 70 picked;
+        // a held set of 70 and 71) and check what the host does with them.
+        const q = sim.grabQueries;
+        new Int32Array(native.memory.get(q.pickOut.native).buffer, 4, 1)[0] = 70;
+        const head = new Uint32Array(native.memory.get(q.gripOut.native).buffer, 0, 8);
+        head[2] = 2;
+        head[3] = 2;
+        const cap = sim.drive.capacity;
+        const nodes = new Uint32Array(native.memory.get(q.gripOut.native).buffer, 32, 2);
+        nodes[0] = 70;
+        nodes[1] = 71;
+        const offs = new Float64Array(native.memory.get(q.gripOut.native).buffer, 32 + 4 * (cap + (cap & 1)), 6);
+        offs.set([0, 0, 0, 0.016, 0, 0]);
+        native.launches.length = 0;
+        const hit = await sim.grab([0.032, 0.032, -1], [0, 0, 1]);
+        assert.deepStrictEqual(native.launches.map(l => l.kernel), ['sim-pick-ray', 'sim-grip-radius']);
+        const [pick, grip] = native.launches;
+        assert.strictEqual(pick.buffers[3], sim.particles.nodeGrabMass.native);
+        assert.strictEqual(pick.buffers[5], q.query.native);
+        assert.strictEqual(pick.buffers[9], q.pickOut.native);
+        assert.strictEqual(grip.buffers[7], q.candidates.native);
+        assert.strictEqual(pick.plane, N);
+        // The last query is the grip's: hand radius, anchor and its body's range.
+        const query = new Float64Array(native.memory.get(q.query.native).buffer, 0, 7);
+        const range = new Uint32Array(native.memory.get(q.query.native).buffer, 56, 3);
+        assert.strictEqual(query[6], 0.03);
+        assert.deepStrictEqual(Array.from(range), [70, 0, N]);
+        assert.deepStrictEqual(sim.drive.heldNodes, [70, 71]);
+        assert.strictEqual(hit.held, 2);
+        assert.strictEqual(hit.bodyIndex, 0);
+        assert.throws(() => sim.setGrip('claw'), /unknown strategy/);
+        sim.setGrip('singleNode');
+        native.launches.length = 0;
+        await sim.grab([0.032, 0.032, -1], [0, 0, 1]);
+        assert.deepStrictEqual(native.launches.map(l => l.kernel), ['sim-pick-ray']);
+        // The pick's query: the ray and the aim tolerance, 1.2 x the widest spacing.
+        assert.deepStrictEqual(Array.from(query), [0.032, 0.032, -1, 0, 0, 1, 0.016 * 1.2]);
+        assert.deepStrictEqual(sim.drive.heldNodes, [70]);
+        sim.destroy();
+    },
     'construction failure frees what it allocated'() {
         const native = fakeAddon();
         const backend = new ComputeBackend(0, native);

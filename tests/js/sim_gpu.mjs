@@ -66,7 +66,15 @@ async function main() {
         write(out, 'splat_rot.bin', rotation);
         write(out, 'splat_axis.bin', scale);
     }
-    fs.writeFileSync(path.join(out, 'meta.json'), JSON.stringify({ nodes: N, ms, stats: sim.stats() }));
+    let grab = null;
+    if (spec.grab) {
+        // Pick and grip on the device against the positions written above.
+        const hit = await sim.grab(spec.grab.origin, spec.grab.direction);
+        const f32 = new Float32Array(sim.drive.words);
+        const offsets = sim.drive.heldNodes.map((n, i) => [1, 2, 3].map(a => f32[a * sim.drive.capacity + i]));
+        grab = { hit, held: sim.drive.heldNodes, offsets };
+    }
+    fs.writeFileSync(path.join(out, 'meta.json'), JSON.stringify({ nodes: N, ms, stats: sim.stats(), grab }));
     if (splats) splats.destroy();
     sim.destroy();
     console.log(`js host: ${N} nodes, ${spec.frames} frames in ${ms.toFixed(1)} ms`);

@@ -29,7 +29,7 @@ def test_js_host_cpu_suite():
     assert r.returncode == 0
 
 
-def spec_for(sc, frames, tmp, splats=True):
+def spec_for(sc, frames, tmp, splats=True, grab=None):
     b = sc.bodies[0]
     lat = b.lattice
     spec = dict(cells=list(lat.cell_dims), origin=list(lat.origin), h=lat.h,
@@ -37,7 +37,7 @@ def spec_for(sc, frames, tmp, splats=True):
                               density=b.material.density, friction=b.material.friction, damping=b.material.damping,
                               relaxation=b.material.relaxation, maxStretch=b.material.max_stretch),
                 pinned=b.static_nodes is not None, tank=dict(min=list(sc.tank_min), max=list(sc.tank_max)),
-                dt=sc.dt, substeps=sc.substeps, frames=frames, splats=None, drive=None)
+                dt=sc.dt, substeps=sc.substeps, frames=frames, splats=None, drive=None, grab=grab)
     if splats and b.n_splats:
         d = os.path.join(tmp, "splats")
         os.makedirs(d, exist_ok=True)
@@ -71,7 +71,10 @@ def test_js_simulator_matches_oracle(tmp_path, case):
         sc.tank_min = (sc.tank_min[0], 0.0, sc.tank_min[2])
         sc.drive = scenes.handle_drive(sc, seed=4)
     frames = 5
-    spec = spec_for(sc, frames, str(tmp_path))
+    lat0 = sc.bodies[0].lattice
+    centre = [lat0.origin[a] + lat0.cell_dims[a] * lat0.h / 2 + (0.003 if a != 1 else 0) for a in range(3)]
+    grab = dict(origin=[centre[0], centre[1] + 1.0, centre[2]], direction=[0.0, -1.0, 0.0]) if case == "block" else None
+    spec = spec_for(sc, frames, str(tmp_path), grab=grab)
     out = str(tmp_path / "out")
     r = run_node("tests/js/sim_gpu.mjs", spec, out)
     print(r.stdout, r.stderr[-3000:])
@@ -104,4 +107,20 @@ def test_js_simulator_matches_oracle(tmp_path, case):
         print(f"engine axes: rotation exact {np.mean(rot == o.out_rot):.6f}, lengths exact "
               f"{np.mean(axis == o.out_axis):.6f}")
         assert np.max(np.abs(rot - o.out_rot)) <= 1e-5
+    if grab is not None:
+        # The device pick and grip against the oracle's restatement on the
+        # same final positions (grab mass: material inverse mass, statics 0).
+        from oracle import oracle as O
+        got = json.load(open(os.path.join(out, "meta.json")))["grab"]
+        body = sc.bodies[0]
+        gm = body.lattice.inv_mass.copy()
+        gm[body.static_nodes] = 0
+        anchor, _ = O.pick_ray(pos, gm, grab["origin"], grab["direction"], body.lattice.h * 1.2)
+        assert anchor >= 0
+        nodes, offs, found = O.grip_radius(pos, gm, anchor, 0, pos.shape[1], 0.03, 4096)
+        print(f"grab: anchor {anchor}, held {len(nodes)} of {found}")
+        assert got["held"] == [int(n) for n in nodes]
+        assert np.array_equal(np.asarray(got["offsets"], np.float32), offs.astype(np.float32))
+        assert got["hit"]["position"] == [float(v) for v in pos[:, anchor]]
+        assert got["hit"]["held"] == len(nodes)
         assert np.max(np.abs(axis - o.out_axis)) / np.max(np.abs(o.out_axis)) <= 1e-5

@@ -21,13 +21,16 @@ const KERNEL_PARAMS = {
     'sim-contact-settle': [...NODE, 'surface', 'contactGrid', 'items', 'buckets'],
     'sim-drive-hard': [...NODE, 'nodeVel', 'driven', 'items'],
     'sim-drive-compliant': [...NODE, 'nodeVel', 'driven', 'items'],
-    'sim-splat-deform': ['nodePos', 'bodies', 'tiles', 'splatIn', 'splatOut', 'items', 'plane', 'block', 'outputs']
+    'sim-splat-deform': ['nodePos', 'bodies', 'tiles', 'splatIn', 'splatOut', 'items', 'plane', 'block', 'outputs'],
+    'sim-pick-ray': ['nodePos', 'grabMass', 'query', 'pickOut', 'plane', 'block'],
+    'sim-grip-radius': ['nodePos', 'grabMass', 'query', 'candidates', 'gripOut', 'plane', 'items', 'pairCap', 'block']
 };
 const OPTIONAL = new Set(['block', 'pairLog', 'pairCap', 'limitOnly', 'nodeInvMass', 'nodePrev', 'params', 'outputs']);
 const BUFFER_SLOT = {
     nodePos: SLOT.nodePos, nodePrev: SLOT.nodePrev, nodeVel: SLOT.nodeVel, nodeInvMass: SLOT.nodeInvMass,
     bodies: SLOT.bodies, params: SLOT.params, tiles: SLOT.tiles, cells: SLOT.items, surface: SLOT.items,
-    driven: SLOT.items, splatIn: SLOT.items, contactGrid: SLOT.contactGrid, splatOut: SLOT.out, pairLog: SLOT.out
+    driven: SLOT.items, splatIn: SLOT.items, contactGrid: SLOT.contactGrid, splatOut: SLOT.out, pairLog: SLOT.out,
+    grabMass: SLOT.nodeInvMass, query: SLOT.params, candidates: SLOT.items, pickOut: SLOT.out, gripOut: SLOT.out
 };
 
 class Allocation {

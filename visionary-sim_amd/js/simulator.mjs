@@ -2,8 +2,8 @@
 // (packages/visionary-sim/src/sim/simulator.js:35-518): assembly in
 // dependency order, step(dt) through the fixed clock, dispatch(steps) as one
 // submission per frame, stats, material changes, reset, readback. The
-// compute backend underneath is the HIP one. Interactive picking and grip
-// strategies stay out of scope; a held node set is given directly (hold).
+// compute backend underneath is the HIP one. grab(origin, direction) picks
+// and grips on the device (grab.mjs); hold() takes a held set directly.
 import { ComputeBackend } from './compute/backend.mjs';
 import { proxyFor } from './discretize/lattice.mjs';
 import { Particles } from './state/particles.mjs';
@@ -14,6 +14,7 @@ import { Stepper } from './solver/stepper.mjs';
 import { FixedTimestep, stableStepDuration } from './solver/timestep.mjs';
 import { collapseScale, gammaOf, waveRatio } from './constraints/compliance.mjs';
 import { DEFAULT_MATERIAL, validateMaxStretch } from './body/material.mjs';
+import { DEFAULT_GRIPPED_PART_RADIUS, GRIP_STRATEGIES, GrabQueries, bodyOfNode } from './grab.mjs';
 
 const DEFAULT_PARAMS = { gravity: [0, -9.81, 0], paused: false };
 
@@ -72,6 +73,9 @@ export class Simulator {
                 this.particles.setParams(b, { damping: body.material.damping, friction: body.material.friction });
             });
             this.grabTarget = { position: [0, 0, 0], previousPosition: [0, 0, 0] };
+            this.grabQueries = new GrabQueries(backend, this.particles, this.drive.capacity);
+            this.grip = { strategy: 'withinRadius', radius: DEFAULT_GRIPPED_PART_RADIUS };
+            this.grabGeneration = 0;
             this.pendingStep = false;
         } catch (error) {
             if (owned) backend.destroy();
@@ -154,6 +158,63 @@ export class Simulator {
         this.backend.dispatch(this.stepper.sequence(steps), 'softbody-step');
     }
 
+    /**
+     * What a line through the scene hits, holding it if it hit something
+     * (simulator.js:366-418). The pick and the grip run on the device; only
+     * the anchor, its position and the held set come back.
+     *
+     * @param {number[]} origin - Ray origin in world space.
+     * @param {number[]} direction - Unit ray direction.
+     * @returns {Promise<{ bodyIndex: number, position: number[], held: number }|null>}
+     */
+    async grab(origin, direction) {
+        if (this.destroyed) return null;
+        const generation = ++this.grabGeneration;
+        // Aim tolerance: a little over the widest node spacing.
+        const aimRadius = this.coarsestSpacing() * 1.2;
+        const { node: anchor } = await this.grabQueries.pick(origin, direction, aimRadius);
+        if (generation !== this.grabGeneration || this.destroyed) return null;
+        if (anchor < 0) return null;
+        let grip;
+        if (this.grip.strategy === 'singleNode') {
+            // The pick only returns grabbable nodes, so the anchor is held.
+            grip = { nodes: [anchor], offsets: [[0, 0, 0]], found: 1 };
+        } else {
+            grip = await this.grabQueries.grip(anchor, {
+                radius: this.grip.radius, nodeOffsets: this.particles.offsets, limit: this.drive.capacity
+            });
+        }
+        if (generation !== this.grabGeneration || this.destroyed) return null;
+        if (grip.found > grip.nodes.length) {
+            console.warn(`[grip] ${grip.found} nodes within ${this.grip.radius} m, only the nearest ` +
+                `${grip.nodes.length} held -- this grab's extent is resolution-dependent again.`);
+        }
+        const position = await this.grabQueries.position(anchor);
+        this.drive.hold(grip.nodes, grip.offsets);
+        this.grabTarget.position = position;
+        this.grabTarget.previousPosition = position;
+        return { bodyIndex: bodyOfNode(this.particles.offsets, anchor), position, held: grip.nodes.length };
+    }
+
+    /** Widest node spacing of any body, the aim tolerance's scale (simulator.js:420-423). */
+    coarsestSpacing() {
+        return this.simBodies.reduce((coarsest, { proxy }) => Math.max(coarsest, proxy.spacing.max), 0);
+    }
+
+    /** Closest node spacing of any body (simulator.js:425-435). */
+    finestSpacing() {
+        return this.simBodies.reduce((finest, { proxy }) => Math.min(finest, proxy.spacing.min), Infinity);
+    }
+
+    /** `'singleNode'` or `'withinRadius'` (simulator.js:437-447). */
+    setGrip(strategy) {
+        if (!GRIP_STRATEGIES.includes(strategy)) throw new Error(`[grip] unknown strategy "${strategy}"`);
+        this.grip.strategy = strategy;
+    }
+
+    /** World-space hand radius for `withinRadius`, metres. */
+    setGripRadius(radius) { this.grip.radius = radius; }
+
     /** Holds `nodes` (global ids) at `offsets` from an anchor at `position`. */
     hold(nodes, offsets, position) {
         this.drive.hold(nodes, offsets);
@@ -214,6 +275,8 @@ export class Simulator {
     destroy() {
         if (this.destroyed) return;
         this.destroyed = true;
+        this.grabGeneration++;
+        this.grabQueries.destroy();
         this.stepper.destroy();
         this.contact.destroy();
         for (const elastic of this.elasticSets) elastic.destroy();

@@ -60,6 +60,9 @@ export class Particles {
         this.nodePrev = backend.buffer(12 * N);
         this.nodeVel = backend.buffer(12 * N);
         this.nodeInvMass = backend.buffer(4 * N);
+        // What a grab may take (isGrabbableNode, pick.js:14-24): the material
+        // inverse mass with static nodes zeroed. Held nodes stay grabbable.
+        this.nodeGrabMass = backend.buffer(4 * N);
         this.bodyWords = new ArrayBuffer(BODY_STRIDE * bodies.length);
         this.bodyF32 = new Float32Array(this.bodyWords);
         this.bodyU32 = new Uint32Array(this.bodyWords);
@@ -79,6 +82,7 @@ export class Particles {
         this.tileCount = count;
         this.reset();
         this.uploadGpuMasses();
+        this.uploadGrabMass();
         this.uploadParams();
     }
 
@@ -120,6 +124,13 @@ export class Particles {
     uploadMasses() {
         for (let b = 0; b < this.bodies.length; b++) this.invMass.set(this.bodies[b].invMass, this.offsets[b]);
         this.uploadGpuMasses();
+        this.uploadGrabMass();
+    }
+
+    uploadGrabMass() {
+        const grab = new Float32Array(this.invMass);
+        for (const n of this.staticPrescribed) grab[n] = 0;
+        this.nodeGrabMass.write(0, grab);
     }
 
     uploadGpuMasses() {
@@ -169,6 +180,7 @@ export class Particles {
     }
 
     destroy() {
-        for (const b of [this.nodePos, this.nodePrev, this.nodeVel, this.nodeInvMass, this.bodyTable, this.tiles]) b.destroy();
+        for (const b of [this.nodePos, this.nodePrev, this.nodeVel, this.nodeInvMass, this.nodeGrabMass, this.bodyTable,
+            this.tiles]) b.destroy();
     }
 }
