@@ -222,7 +222,9 @@ typedef struct vs_world_config {
     uint32_t block_cells;   /* threads per workgroup of the colour passes      */
     uint32_t persistent;    /* workgroups of a persistent launch (0 = 1 per tile) */
     uint32_t xcd_chunked;   /* 1: give each XCD a contiguous run of tiles      */
-    uint32_t pad0;
+    uint32_t fuse;          /* 1: fold predict into the first colour pass and
+                               finalize into the last where nothing runs between
+                               (same results; node passes cover the rest)     */
 } vs_world_config;
 
 /* One body of a scene, in the reference's host layout: the dense node grid
@@ -351,7 +353,8 @@ int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, 
  * so the simulation is left where it was. Synchronous. */
 int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, uint32_t *launches);
 /* Re-times the colour-pass launch shapes on the admitted scenes and keeps the
- * fastest (block size x cells per tile x persistent grid x XCD mapping). */
+ * fastest (block size x cells per tile x persistent grid x XCD mapping), then
+ * the fastest shape with the fused schedule (vs_world_config.fuse). */
 int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t report_bytes);
 
 /* ------------------------------------------------------ the server */

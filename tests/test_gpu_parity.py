@@ -227,3 +227,43 @@ def test_server_results_are_the_same_alone_and_in_the_mix():
             for _ in range(frames):
                 o.frame(10)
             assert rel_err(mixed, o.p) <= RTOL
+
+
+def _sparse_scene(seed, shape=(9, 12, 10), fill=0.65, pinned=False):
+    from tests.test_gpu_lattice import occupied_scene
+    return occupied_scene(np.random.default_rng(seed).random(shape) < fill, pinned=pinned, name=f"sp{seed}")
+
+
+@pytest.mark.parametrize("case", ["c1", "odd_block", "sparse", "sparse_pinned", "hard_drive"])
+def test_fused_frames_equal_unfused_and_oracle(case):
+    # Predict folded into colour 0 and finalize into colour 7, the node passes
+    # covering what those colours do not: the same frames, bit for bit.
+    from oracle.oracle import OracleScene
+    if case == "c1":
+        sc = scenes.c1_scene(splats=0)
+    elif case == "odd_block":
+        sc = scenes.jelly_block((5, 6, 7), seed=3, name="odd")
+    elif case == "sparse":
+        sc = _sparse_scene(11)
+    elif case == "sparse_pinned":
+        sc = _sparse_scene(12, shape=(8, 9, 14), pinned=True)
+    else:
+        sc = scenes.jelly_block(10, seed=5, pinned=False, tank_margin=0.3, name="hd")
+        sc.drive = scenes.handle_drive(sc, seed=6, law=0)
+    out = []
+    for fuse in (1, 0):
+        w = world_for(sc, fuse=fuse)
+        sid = w.add(sc)
+        for f in range(3):
+            if sc.drive is not None:
+                w.set_target(sid, *sc.drive.frame_target(f, sc.substeps * sc.dt))
+            w.frame(sc.substeps, False, f == 1)
+        out.append(w.read_nodes(sid))
+        w.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    o = OracleScene(sc, "f32")
+    for f in range(3):
+        o.frame(sc.substeps)
+    print(f"{case}: exact vs oracle {np.mean(out[0][0] == o.p):.6f}")
+    assert rel_err(out[0][0], o.p) <= RTOL
+    assert rel_err(out[0][1], o.v) <= RTOL

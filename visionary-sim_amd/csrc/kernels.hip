@@ -203,8 +203,127 @@ __device__ __forceinline__ V3 ld3(const float *x, const float *y, const float *z
     return mk(x[n], y[n], z[n]);
 }
 
-// ------------------------------------------------------------- elastic
+// ------------------------------------------------------ predict/finalize
+// predict.wgsl:19-36 for one node with w != 0: symplectic Euler.
+__device__ __forceinline__ void predict_node(V3 &p, V3 &q, V3 &v, float gx, float gy, float gz, float dt)
+{
+    v = mk(v.x + gx * dt, v.y + gy * dt, v.z + gz * dt);
+    q = p;
+    p = mk(p.x + v.x * dt, p.y + v.y * dt, p.z + v.z * dt);
+}
+
+// A node pass over the tiles; with `cover`, nodes whose cover byte has
+// `skip` set are left to the colour pass that predicts them itself.
 template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_predict(NodeView n, const vs_scene_params *__restrict__ scenes, TileView tv,
+                                                   const uint8_t *__restrict__ cover, uint32_t skip)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_scene_params &sc = scenes[tile.scene];
+        const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t k = tile.begin + i;
+            if (cover && (cover[k] & skip)) continue;
+            if (n.w[k] == 0.0f) continue;
+            V3 v = ld3(n.vx, n.vy, n.vz, k), p = ld3(n.px, n.py, n.pz, k), q;
+            predict_node(p, q, v, gx, gy, gz, dt);
+            n.qx[k] = q.x;
+            n.qy[k] = q.y;
+            n.qz[k] = q.z;
+            n.px[k] = p.x;
+            n.py[k] = p.y;
+            n.pz[k] = p.z;
+            n.vx[k] = v.x;
+            n.vy[k] = v.y;
+            n.vz[k] = v.z;
+        }
+    }
+}
+
+// resolvePlane (finalize.wgsl:29-42) for an axis-aligned normal +-e_A.
+template <int A, int SIGN>
+__device__ __forceinline__ V3 plane(V3 p, V3 prev, float depth, float friction)
+{
+    if (!(depth > 0.0f)) return p;
+    float c[3] = { p.x, p.y, p.z };
+    c[A] = SIGN > 0 ? c[A] + depth : c[A] - depth;
+    V3 q = mk(c[0], c[1], c[2]);
+    const V3 delta = sub(q, prev);
+    // tangent = delta - n * dot(delta, n): the A component cancels exactly.
+    float tg[3] = { delta.x, delta.y, delta.z };
+    tg[A] = 0.0f;
+    const V3 tangent = mk(tg[0], tg[1], tg[2]);
+    const float len = sqrtf(dot(tangent, tangent));
+    if (len > 1e-9f) {
+        const float lim = (friction * depth) / len;
+        q = sub(q, mul(tangent, lim < 1.0f ? lim : 1.0f));
+    }
+    return q;
+}
+
+// The uniforms finalize reads for one body of one scene.
+struct Tank {
+    float lx, ly, lz, hx, hz, fr, damp, dt;
+};
+
+__device__ __forceinline__ Tank tank_of(const vs_scene_params &sc, const vs_body &b)
+{
+    return Tank{ sc.tank_min[0], sc.tank_min[1], sc.tank_min[2], sc.tank_max[0], sc.tank_max[2], b.friction, b.damp,
+                 sc.dt };
+}
+
+// finalize.wgsl:44-74 for one node with w != 0: the five tank planes (no
+// ceiling), then the damped velocity.
+__device__ __forceinline__ void finalize_node(V3 &p, V3 prev, V3 &v, const Tank &k)
+{
+    p = plane<1, 1>(p, prev, k.ly - p.y, k.fr);
+    p = plane<0, 1>(p, prev, k.lx - p.x, k.fr);
+    p = plane<0, -1>(p, prev, p.x - k.hx, k.fr);
+    p = plane<2, 1>(p, prev, k.lz - p.z, k.fr);
+    p = plane<2, -1>(p, prev, p.z - k.hz, k.fr);
+    v = mk(((p.x - prev.x) / k.dt) * k.damp, ((p.y - prev.y) / k.dt) * k.damp, ((p.z - prev.z) / k.dt) * k.damp);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_finalize(NodeView n, const vs_body *__restrict__ bodies,
+                                                    const vs_scene_params *__restrict__ scenes, TileView tv,
+                                                    const uint8_t *__restrict__ cover, uint32_t skip)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const Tank tk = tank_of(scenes[tile.scene], bodies[tile.body]);
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t k = tile.begin + i;
+            if (cover && (cover[k] & skip)) continue;
+            if (n.w[k] == 0.0f) continue;
+            const V3 prev = ld3(n.qx, n.qy, n.qz, k);
+            V3 p = ld3(n.px, n.py, n.pz, k), v;
+            finalize_node(p, prev, v, tk);
+            n.px[k] = p.x;
+            n.py[k] = p.y;
+            n.pz[k] = p.z;
+            n.vx[k] = v.x;
+            n.vy[k] = v.y;
+            n.vz[k] = v.z;
+        }
+    }
+}
+
+// ------------------------------------------------------------- elastic
+// FUSE selects what the pass does around the projection:
+//   FUSE_NONE      the colour pass of project.wgsl;
+//   FUSE_PREDICT   predict (predict.wgsl) each corner first. The cells of one
+//                  colour own disjoint corner sets, so colour 0 can predict
+//                  the nodes it covers itself; a node pass predicts the rest;
+//   FUSE_FINALIZE  finalize (finalize.wgsl) each corner after it, for the last
+//                  colour when nothing (contact, stretch sweeps) comes between.
+// Every node is still predicted and finalized once, with the same arithmetic.
+enum { FUSE_NONE = 0, FUSE_PREDICT = 1, FUSE_FINALIZE = 2 };
+
+template <int BLOCK, int FUSE>
 __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__restrict__ bodies,
                                                    const vs_scene_params *__restrict__ scenes,
                                                    const uint32_t *__restrict__ cells, TileView tv,
@@ -214,7 +333,8 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
     for (uint32_t t = r.first; t < r.end; t += r.stride) {
         const vs_tile tile = tv.tiles[t];
         const vs_body bd = bodies[tile.body];
-        const float dt = scenes[bd.scene].dt;
+        const vs_scene_params &sc = scenes[bd.scene];
+        const float dt = sc.dt;
         const float maxStretch = bd.max_stretch;
         if (limit_only && !(maxStretch > 1.0f)) continue;
         const float invDt2 = 1.0f / (dt * dt);
@@ -238,14 +358,42 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const F2 x = ld2(n.px, off[k]), y = ld2(n.py, off[k]), z = ld2(n.pz, off[k]);
-                const F2 u = ld2(n.qx, off[k]), v = ld2(n.qy, off[k]), t = ld2(n.qz, off[k]);
                 const F2 m = ld2(n.w, off[k]);
                 p[2 * k] = mk(x.x, y.x, z.x);
                 p[2 * k + 1] = mk(x.y, y.y, z.y);
-                q[2 * k] = mk(u.x, v.x, t.x);
-                q[2 * k + 1] = mk(u.y, v.y, t.y);
                 w[2 * k] = m.x;
                 w[2 * k + 1] = m.y;
+                if (FUSE == FUSE_PREDICT) {
+                    // A node with w = 0 is not predicted and keeps its prev;
+                    // the others' prev becomes their position.
+                    F2 u = { p[2 * k].x, p[2 * k + 1].x }, v = { p[2 * k].y, p[2 * k + 1].y },
+                       s = { p[2 * k].z, p[2 * k + 1].z };
+                    if (m.x == 0.0f || m.y == 0.0f) {
+                        u = ld2(n.qx, off[k]);
+                        v = ld2(n.qy, off[k]);
+                        s = ld2(n.qz, off[k]);
+                    }
+                    q[2 * k] = mk(u.x, v.x, s.x);
+                    q[2 * k + 1] = mk(u.y, v.y, s.y);
+                    const F2 vx = ld2(n.vx, off[k]), vy = ld2(n.vy, off[k]), vz = ld2(n.vz, off[k]);
+                    V3 vel[2] = { mk(vx.x, vy.x, vz.x), mk(vx.y, vy.y, vz.y) };
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        if (w[2 * k + h] != 0.0f)
+                            predict_node(p[2 * k + h], q[2 * k + h], vel[h], sc.gravity[0], sc.gravity[1],
+                                         sc.gravity[2], dt);
+                    }
+                    st2(n.qx, off[k], q[2 * k].x, q[2 * k + 1].x);
+                    st2(n.qy, off[k], q[2 * k].y, q[2 * k + 1].y);
+                    st2(n.qz, off[k], q[2 * k].z, q[2 * k + 1].z);
+                    st2(n.vx, off[k], vel[0].x, vel[1].x);
+                    st2(n.vy, off[k], vel[0].y, vel[1].y);
+                    st2(n.vz, off[k], vel[0].z, vel[1].z);
+                } else {
+                    const F2 u = ld2(n.qx, off[k]), v = ld2(n.qy, off[k]), s = ld2(n.qz, off[k]);
+                    q[2 * k] = mk(u.x, v.x, s.x);
+                    q[2 * k + 1] = mk(u.y, v.y, s.y);
+                }
             }
             if (!limit_only) {
                 project_tet<0>(p, q, w, rs, bd.gamma, aD, aH, relax);
@@ -275,92 +423,33 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
                     }
                 }
             }
+            if (FUSE == FUSE_FINALIZE) {
+                const Tank tk = tank_of(sc, bd);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    // A node with w = 0 keeps its velocity: read it only for
+                    // pairs that hold one, to write the pair back whole.
+                    V3 vel[2] = { mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f) };
+                    if (w[2 * k] == 0.0f || w[2 * k + 1] == 0.0f) {
+                        const F2 vx = ld2(n.vx, off[k]), vy = ld2(n.vy, off[k]), vz = ld2(n.vz, off[k]);
+                        vel[0] = mk(vx.x, vy.x, vz.x);
+                        vel[1] = mk(vx.y, vy.y, vz.y);
+                    }
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        if (w[2 * k + h] != 0.0f) finalize_node(p[2 * k + h], q[2 * k + h], vel[h], tk);
+                    }
+                    st2(n.vx, off[k], vel[0].x, vel[1].x);
+                    st2(n.vy, off[k], vel[0].y, vel[1].y);
+                    st2(n.vz, off[k], vel[0].z, vel[1].z);
+                }
+            }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 st2(n.px, off[k], p[2 * k].x, p[2 * k + 1].x);
                 st2(n.py, off[k], p[2 * k].y, p[2 * k + 1].y);
                 st2(n.pz, off[k], p[2 * k].z, p[2 * k + 1].z);
             }
-        }
-    }
-}
-
-// ------------------------------------------------------ predict/finalize
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_predict(NodeView n, const vs_scene_params *__restrict__ scenes, TileView tv)
-{
-    const Range r = tile_range(*tv.count, 0);
-    for (uint32_t t = r.first; t < r.end; t += r.stride) {
-        const vs_tile tile = tv.tiles[t];
-        const vs_scene_params &sc = scenes[tile.scene];
-        const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
-        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
-            const uint32_t k = tile.begin + i;
-            if (n.w[k] == 0.0f) continue;
-            const float vx = n.vx[k] + gx * dt, vy = n.vy[k] + gy * dt, vz = n.vz[k] + gz * dt;
-            const float px = n.px[k], py = n.py[k], pz = n.pz[k];
-            n.qx[k] = px;
-            n.qy[k] = py;
-            n.qz[k] = pz;
-            n.px[k] = px + vx * dt;
-            n.py[k] = py + vy * dt;
-            n.pz[k] = pz + vz * dt;
-            n.vx[k] = vx;
-            n.vy[k] = vy;
-            n.vz[k] = vz;
-        }
-    }
-}
-
-// resolvePlane (finalize.wgsl:29-42) for an axis-aligned normal +-e_A.
-template <int A, int SIGN>
-__device__ __forceinline__ V3 plane(V3 p, V3 prev, float depth, float friction)
-{
-    if (!(depth > 0.0f)) return p;
-    float c[3] = { p.x, p.y, p.z };
-    c[A] = SIGN > 0 ? c[A] + depth : c[A] - depth;
-    V3 q = mk(c[0], c[1], c[2]);
-    const V3 delta = sub(q, prev);
-    // tangent = delta - n * dot(delta, n): the A component cancels exactly.
-    float tg[3] = { delta.x, delta.y, delta.z };
-    tg[A] = 0.0f;
-    const V3 tangent = mk(tg[0], tg[1], tg[2]);
-    const float len = sqrtf(dot(tangent, tangent));
-    if (len > 1e-9f) {
-        const float lim = (friction * depth) / len;
-        q = sub(q, mul(tangent, lim < 1.0f ? lim : 1.0f));
-    }
-    return q;
-}
-
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_finalize(NodeView n, const vs_body *__restrict__ bodies,
-                                                    const vs_scene_params *__restrict__ scenes, TileView tv)
-{
-    const Range r = tile_range(*tv.count, 0);
-    for (uint32_t t = r.first; t < r.end; t += r.stride) {
-        const vs_tile tile = tv.tiles[t];
-        const vs_scene_params &sc = scenes[tile.scene];
-        const float dt = sc.dt;
-        const float fr = bodies[tile.body].friction, damp = bodies[tile.body].damp;
-        const float lx = sc.tank_min[0], ly = sc.tank_min[1], lz = sc.tank_min[2];
-        const float hx = sc.tank_max[0], hz = sc.tank_max[2];
-        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
-            const uint32_t k = tile.begin + i;
-            if (n.w[k] == 0.0f) continue;
-            const V3 prev = ld3(n.qx, n.qy, n.qz, k);
-            V3 p = ld3(n.px, n.py, n.pz, k);
-            p = plane<1, 1>(p, prev, ly - p.y, fr);
-            p = plane<0, 1>(p, prev, lx - p.x, fr);
-            p = plane<0, -1>(p, prev, p.x - hx, fr);
-            p = plane<2, 1>(p, prev, lz - p.z, fr);
-            p = plane<2, -1>(p, prev, p.z - hz, fr);
-            n.px[k] = p.x;
-            n.py[k] = p.y;
-            n.pz[k] = p.z;
-            n.vx[k] = ((p.x - prev.x) / dt) * damp;
-            n.vy[k] = ((p.y - prev.y) / dt) * damp;
-            n.vz[k] = ((p.z - prev.z) / dt) * damp;
         }
     }
 }
@@ -763,21 +852,40 @@ __global__ __launch_bounds__(BLOCK) void k_splats(const float *__restrict__ px, 
         return hipGetLastError();                                                                              \
     } while (0)
 
-hipError_t launch_predict(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, TileView t)
+hipError_t launch_predict(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, TileView t,
+                          const uint8_t *cover, uint32_t skip)
 {
-    VSIM_DISPATCH_BLOCK(k_predict, sh, s, n, scenes, t);
+    VSIM_DISPATCH_BLOCK(k_predict, sh, s, n, scenes, t, cover, skip);
 }
 
 hipError_t launch_finalize(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies, const vs_scene_params *scenes,
-                           TileView t)
+                           TileView t, const uint8_t *cover, uint32_t skip)
 {
-    VSIM_DISPATCH_BLOCK(k_finalize, sh, s, n, bodies, scenes, t);
+    VSIM_DISPATCH_BLOCK(k_finalize, sh, s, n, bodies, scenes, t, cover, skip);
 }
 
-hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies, const vs_scene_params *scenes,
+namespace {
+template <int FUSE>
+hipError_t elastic_blocks(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies, const vs_scene_params *scenes,
                           const uint32_t *cells, TileView t, uint32_t limit_only)
 {
-    VSIM_DISPATCH_BLOCK(k_elastic, sh, s, n, bodies, scenes, cells, t, limit_only, sh.xcd);
+    switch (sh.block) {
+    case 64: k_elastic<64, FUSE><<<sh.grid, 64, 0, s>>>(n, bodies, scenes, cells, t, limit_only, sh.xcd); break;
+    case 128: k_elastic<128, FUSE><<<sh.grid, 128, 0, s>>>(n, bodies, scenes, cells, t, limit_only, sh.xcd); break;
+    case 256: k_elastic<256, FUSE><<<sh.grid, 256, 0, s>>>(n, bodies, scenes, cells, t, limit_only, sh.xcd); break;
+    case 512: k_elastic<512, FUSE><<<sh.grid, 512, 0, s>>>(n, bodies, scenes, cells, t, limit_only, sh.xcd); break;
+    default: return hipErrorInvalidConfiguration;
+    }
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies, const vs_scene_params *scenes,
+                          const uint32_t *cells, TileView t, uint32_t limit_only, int fuse)
+{
+    if (fuse == FUSE_PREDICT) return elastic_blocks<FUSE_PREDICT>(s, sh, n, bodies, scenes, cells, t, limit_only);
+    if (fuse == FUSE_FINALIZE) return elastic_blocks<FUSE_FINALIZE>(s, sh, n, bodies, scenes, cells, t, limit_only);
+    return elastic_blocks<FUSE_NONE>(s, sh, n, bodies, scenes, cells, t, limit_only);
 }
 
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,
