@@ -396,6 +396,81 @@ int vs_server_result(const vs_server *s, uint32_t request_id, float *pos);
 /* Seconds since run() began: arrival, admission and completion of a request. */
 int vs_server_times(const vs_server *s, uint32_t request_id, double *arrival, double *admitted, double *finished);
 
+/* ------------------------------------------------------ ingest
+ *
+ * Capture formats into occupancy grids, the input of the device lattice build
+ * (vs_body_desc.occupancy). The grids match the reference's voxelisers cell
+ * for cell: every float expression is evaluated in double, in the order the
+ * JS writes it, with the rotation matrix rounded to f32 as mat3FromQuat
+ * stores it (mat3.js:71-86). Math.hypot is evaluated as the square root of
+ * the sum of squares.
+ *
+ * These calls allocate their own device buffers for the call's duration and
+ * return host results allocated with malloc; free them with the matching
+ * *_free. vs_ingest_error() names the last failure on this thread in the
+ * reference's words. */
+
+/* OccupancyGrid (voxelize.js:21-29): dims, origin, cell edge h, one byte per
+ * cell, x fastest, 1 = material. */
+typedef struct vs_occupancy {
+    uint32_t dims[3];
+    uint32_t pad0;
+    double origin[3];
+    double h;
+    uint8_t *cells;
+} vs_occupancy;
+
+#define VS_SHAPE_SPHERE 0 /* size[0] = radius (shapes.js:60-67)                */
+#define VS_SHAPE_BOX 1    /* size = half extents (shapes.js:89-108)            */
+#define VS_SHAPE_TORUS 2  /* size[0] major, size[1] minor radius (shapes.js:32-43) */
+
+/* An analytic shape, translated by `offset` (translateShape, shapes.js:148-156). */
+typedef struct vs_shape {
+    int32_t kind, pad0;
+    double size[3];
+    double offset[3];
+} vs_shape;
+
+/* voxelizeCloud (cloudVolume.js:95-121): the ellipsoids at `sigma` standard
+ * deviations stamped into a grid over their bounds, then (fill != 0) sealed
+ * by `seal` dilations, flooded from the faces, eroded back and re-stamped.
+ * Reference defaults: padding 2, sigma 2, seal 2, fill true. Rotations are
+ * xyzw unit quaternions; scales in metres. */
+int vs_voxelize_cloud(int device, const float *positions, const float *rotations, const float *scales, uint64_t count,
+                      double element_size, uint32_t padding, double sigma, uint32_t seal, int32_t fill,
+                      vs_occupancy *out);
+/* voxelizeMesh (voxelize.js:174-203): two of three axis-parity votes or a
+ * surface crossing, then (close != 0) a morphological close. `bounds` is
+ * min[3], max[3]; NULL uses the vertices' bounds (meshBounds,
+ * restGeometry.js:29-40). Reference defaults: padding 2, close true. */
+int vs_voxelize_mesh(int device, const float *positions, uint64_t vertices, const uint32_t *indices, uint64_t triangles,
+                     const double *bounds, double element_size, uint32_t padding, int32_t close, vs_occupancy *out);
+/* voxelizeSdf (voxelize.js:118-148) of an analytic shape. */
+int vs_voxelize_shape(int device, const vs_shape *shape, double element_size, uint32_t padding, vs_occupancy *out);
+void vs_occupancy_free(vs_occupancy *grid);
+const char *vs_ingest_error(void);
+
+/* readPly (apps/simulator/src/assets/splats/ply.js:65-104): the first element
+ * of a binary little-endian PLY held in memory. `names` (n_names of them)
+ * selects the properties to keep; each becomes one malloc'd float column,
+ * columns[i] for names[i], whatever its stored type. *count receives the
+ * record count. */
+int vs_ply_read(const void *bytes, uint64_t size, const char *const *names, uint32_t n_names, uint64_t *count,
+                float **columns);
+/* The first element's property names, space-separated, into `list`. */
+int vs_ply_properties(const void *bytes, uint64_t size, char *list, uint64_t list_bytes, uint64_t *count);
+
+/* SplatCloud (apps/simulator/src/assets/splats/cloud.js:52-81): positions,
+ * xyzw unit rotations with w >= 0, scales in metres, opacity in [0, 1],
+ * colours from the degree-0 harmonic. Arrays are malloc'd. */
+typedef struct vs_cloud {
+    uint64_t count;
+    float *positions, *rotations, *scales, *opacity, *colors;
+} vs_cloud;
+/* readPly with the fourteen required properties, then cloudFromPly. */
+int vs_cloud_from_ply(const void *bytes, uint64_t size, vs_cloud *out);
+void vs_cloud_free(vs_cloud *cloud);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,17 @@ class OSplats(C.Structure):
     ]
 
 
+class OShape(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad0", C.c_int32), ("size", C.c_double * 3), ("offset", C.c_double * 3)]
+
+
+SHAPES = {"sphere": 0, "box": 1, "torus": 2}
+
+
+class OGrid(C.Structure):
+    _fields_ = [("dims", C.c_uint32 * 3), ("origin", C.c_double * 3), ("h", C.c_double)]
+
+
 def load(precision="f32"):
     if precision in _libs:
         return _libs[precision]
@@ -74,6 +85,17 @@ def load(precision="f32"):
     lib.o_elastic_color.argtypes = [sp, C.c_int, C.c_int]
     lib.o_deform_splats.argtypes = [sp, C.POINTER(OSplats)]
     lib.o_ellipsoid_axes.argtypes = [C.c_void_p] * 3
+    lib.o_voxelize_cloud.argtypes = [C.c_void_p] * 3 + [C.c_uint64, C.c_double, C.c_uint32, C.c_double, C.c_uint32,
+                                                         C.c_int, C.POINTER(OGrid), C.POINTER(C.c_void_p)]
+    lib.o_voxelize_mesh.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_double,
+                                    C.c_uint32, C.c_int, C.POINTER(OGrid), C.POINTER(C.c_void_p)]
+    lib.o_grid_for_bounds.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_uint32, C.POINTER(OGrid)]
+    lib.o_neighbourhood.argtypes = [C.POINTER(OGrid), C.c_void_p, C.c_void_p, C.c_int]
+    lib.o_fill_enclosed.argtypes = [C.POINTER(OGrid), C.c_void_p]
+    lib.o_fill_enclosed.restype = C.c_uint64
+    lib.o_free.argtypes = [C.c_void_p]
+    lib.o_voxelize_shape.argtypes = [C.POINTER(OShape), C.c_double, C.c_uint32, C.POINTER(OGrid),
+                                     C.POINTER(C.c_void_p)]
     lib.o_pick_ray.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
     lib.o_pick_ray.restype = C.c_int64
     lib.o_grip_radius.argtypes = [C.c_void_p] * 4 + [C.c_uint32] * 3 + [C.c_double, C.c_uint32] + [C.c_void_p] * 3
@@ -115,6 +137,108 @@ def grip_radius(pos, mass, anchor, begin, end, radius, limit):
     k = load("f64").o_grip_radius(*ptrs, m.ctypes.data, begin, end, anchor, float(radius), limit,
                                   nodes.ctypes.data, offs.ctypes.data, C.byref(found))
     return nodes[:k].copy(), offs[:k].copy(), found.value
+
+
+class Grid:
+    """An occupancy grid (OccupancyGrid, voxelize.js:1-16): dims (x, y, z),
+    f64 origin and edge, and one byte per cell with x fastest."""
+
+    def __init__(self, dims, origin, h, occupancy):
+        self.dims, self.origin, self.h = tuple(int(d) for d in dims), tuple(float(o) for o in origin), float(h)
+        self.occupancy = occupancy
+
+    def volume(self):
+        return int(self.occupancy.sum()) * self.h ** 3
+
+    def at(self, point):
+        """Occupancy of the cell holding `point` (locate, sample.js:42-60)."""
+        c = [min(max(int(np.floor((point[a] - self.origin[a]) / self.h)), 0), self.dims[a] - 1) for a in range(3)]
+        return int(self.occupancy[c[0] + self.dims[0] * (c[1] + self.dims[1] * c[2])])
+
+    def array(self):
+        """(z, y, x) view."""
+        return self.occupancy.reshape(self.dims[2], self.dims[1], self.dims[0])
+
+
+def _grid_from(g, ptr):
+    n = g.dims[0] * g.dims[1] * g.dims[2]
+    occ = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_uint8)), shape=(n,)).copy()
+    load("f64").o_free(ptr)
+    return Grid(list(g.dims), list(g.origin), g.h, occ)
+
+
+VOXEL_ERRORS = {
+    -1: "voxelizeCloud: a cloud with no Gaussians has no volume",
+    -2: "voxelizeCloud: a cloud with no extent has no volume",
+    -3: "gridForBounds: elementSize must be a positive length in metres",
+    -4: "gridForBounds: the bounds need more cells than one grid holds",
+    -5: "voxelizeMesh: vertex coordinates must be finite and indices inside the vertices",
+}
+
+
+def _raise(rc):
+    if rc:
+        raise ValueError(VOXEL_ERRORS[rc])
+
+
+def voxelize_cloud(positions, rotations, scales, element_size, padding=2, sigma=2.0, seal=2, fill=True):
+    """voxelizeCloud (cloudVolume.js:95-121) over xyz / xyzw / xyz arrays."""
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1)
+    r = np.ascontiguousarray(rotations, np.float32).reshape(-1)
+    s = np.ascontiguousarray(scales, np.float32).reshape(-1)
+    g = OGrid()
+    ptr = C.c_void_p()
+    rc = load("f64").o_voxelize_cloud(p.ctypes.data, r.ctypes.data, s.ctypes.data, len(p) // 3, element_size,
+                                      padding, sigma, seal, int(fill), C.byref(g), C.byref(ptr))
+    _raise(rc)
+    return _grid_from(g, ptr)
+
+
+def voxelize_mesh(positions, indices, element_size, padding=2, close=True, bounds=None):
+    """voxelizeMesh (voxelize.js:174-203) over `bounds` ((min, max)) or
+    meshBounds."""
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1)
+    ix = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    b = None if bounds is None else np.ascontiguousarray(np.concatenate(bounds), np.float64)
+    g = OGrid()
+    ptr = C.c_void_p()
+    rc = load("f64").o_voxelize_mesh(p.ctypes.data, len(p) // 3, ix.ctypes.data, len(ix) // 3,
+                                     None if b is None else b.ctypes.data, element_size, padding, int(close),
+                                     C.byref(g), C.byref(ptr))
+    _raise(rc)
+    return _grid_from(g, ptr)
+
+
+def voxelize_shape(kind, size, element_size, offset=(0.0, 0.0, 0.0), padding=2):
+    """voxelizeSdf (voxelize.js:120-148) of sphere (radius) / box (half
+    extents) / torus (major, minor)."""
+    sh = OShape(kind=SHAPES[kind])
+    sh.size[:] = list(size) + [0.0] * (3 - len(size))
+    sh.offset[:] = offset
+    g = OGrid()
+    ptr = C.c_void_p()
+    _raise(load("f64").o_voxelize_shape(C.byref(sh), element_size, padding, C.byref(g), C.byref(ptr)))
+    return _grid_from(g, ptr)
+
+
+def grid_for_bounds(mn, mx, element_size, padding=2):
+    g = OGrid()
+    lo = np.ascontiguousarray(mn, np.float64)
+    hi = np.ascontiguousarray(mx, np.float64)
+    _raise(load("f64").o_grid_for_bounds(lo.ctypes.data, hi.ctypes.data, element_size, padding, C.byref(g)))
+    return Grid(list(g.dims), list(g.origin), g.h, np.zeros(g.dims[0] * g.dims[1] * g.dims[2], np.uint8))
+
+
+def close_grid(grid):
+    """morphologicalClose (voxelize.js:386-394), in place."""
+    g = OGrid()
+    g.dims[:] = grid.dims
+    g.origin[:] = grid.origin
+    g.h = grid.h
+    tmp = np.empty_like(grid.occupancy)
+    lib = load("f64")
+    lib.o_neighbourhood(C.byref(g), grid.occupancy.ctypes.data, tmp.ctypes.data, 1)
+    lib.o_neighbourhood(C.byref(g), tmp.ctypes.data, grid.occupancy.ctypes.data, 0)
 
 
 def ellipsoid_axes(cov6, precision="f64"):

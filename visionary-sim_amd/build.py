@@ -33,7 +33,7 @@ FP_FLAGS = [
 ]
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
-SOURCES = ["kernels.hip", "lattice.hip", "grab.hip", "backend.cpp", "world.cpp", "server.cpp"]
+SOURCES = ["kernels.hip", "lattice.hip", "grab.hip", "voxel.hip", "backend.cpp", "world.cpp", "server.cpp", "ply.cpp"]
 
 
 def _run(cmds):
@@ -61,7 +61,7 @@ def build_lib():
         if not os.path.exists(path):
             continue
         obj = os.path.join(BUILD, src + ".o")
-        lang = ["-x", "hip", "--offload-arch=" + ARCH] if src.endswith(".hip") else ["-x", "hip", "--offload-arch=" + ARCH]
+        lang = ["-x", "hip", "--offload-arch=" + ARCH]
         cmds.append([HIPCC] + lang + COMMON + FP_FLAGS + ["-c", path, "-o", obj])
         objs.append(obj)
     _run(cmds)

@@ -9,6 +9,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "../../include/vsim.h"
 
 namespace vsim {
@@ -134,5 +136,8 @@ hipError_t launch_grip(hipStream_t s, uint32_t grid, const float *px, const floa
                        uint32_t cap, uint32_t limit, void *out);
 hipError_t launch_splats(hipStream_t s, Shape sh, const float *px, const float *py, const float *pz,
                          const vs_body *bodies, SplatView sp, TileView t);
+
+// Records the message vs_ingest_error() returns on this thread; returns code.
+int ingest_fail(int code, const std::string &message);
 
 }  // namespace vsim

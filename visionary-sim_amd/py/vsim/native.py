@@ -124,11 +124,30 @@ EXPORTS = [
     "vs_world_autotune", "vs_world_profile", "vs_world_time_pass",
     "vs_server_create", "vs_server_destroy", "vs_server_error", "vs_server_submit", "vs_server_run",
     "vs_server_latencies", "vs_server_result", "vs_server_times",
+    "vs_voxelize_cloud", "vs_voxelize_mesh", "vs_voxelize_shape", "vs_occupancy_free", "vs_ingest_error",
+    "vs_ply_read", "vs_ply_properties", "vs_cloud_from_ply", "vs_cloud_free",
 ]
 
 SPLAT_COV, SPLAT_AXES = 1, 2  # VS_SPLAT_* output bits of a frame's `splats`
 
 PASS_KINDS = ["predict", "drive", "elastic", "bin", "sort", "collide", "settle", "finalize", "splat"]
+
+class VsOccupancy(C.Structure):
+    _fields_ = [("dims", C.c_uint32 * 3), ("pad0", C.c_uint32), ("origin", C.c_double * 3), ("h", C.c_double),
+                ("cells", C.POINTER(C.c_uint8))]
+
+
+SHAPE_SPHERE, SHAPE_BOX, SHAPE_TORUS = 0, 1, 2
+
+
+class VsShape(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad0", C.c_int32), ("size", C.c_double * 3), ("offset", C.c_double * 3)]
+
+
+class VsCloud(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("positions", C.POINTER(C.c_float)), ("rotations", C.POINTER(C.c_float)),
+                ("scales", C.POINTER(C.c_float)), ("opacity", C.POINTER(C.c_float)), ("colors", C.POINTER(C.c_float))]
+
 
 _lib = None
 
@@ -184,6 +203,16 @@ def load(path=LIB_PATH):
         "vs_server_latencies": (C.c_int, [p, p, u64, C.POINTER(u64)]),
         "vs_server_result": (C.c_int, [p, u32, p]),
         "vs_server_times": (C.c_int, [p, u32, p, p, p]),
+        "vs_voxelize_cloud": (C.c_int, [C.c_int, p, p, p, u64, C.c_double, u32, C.c_double, u32, i32,
+                                        C.POINTER(VsOccupancy)]),
+        "vs_voxelize_mesh": (C.c_int, [C.c_int, p, u64, p, u64, p, C.c_double, u32, i32, C.POINTER(VsOccupancy)]),
+        "vs_voxelize_shape": (C.c_int, [C.c_int, C.POINTER(VsShape), C.c_double, u32, C.POINTER(VsOccupancy)]),
+        "vs_occupancy_free": (None, [C.POINTER(VsOccupancy)]),
+        "vs_ingest_error": (C.c_char_p, []),
+        "vs_ply_read": (C.c_int, [p, u64, C.POINTER(C.c_char_p), u32, C.POINTER(u64), C.POINTER(C.POINTER(f))]),
+        "vs_ply_properties": (C.c_int, [p, u64, C.c_char_p, u64, C.POINTER(u64)]),
+        "vs_cloud_from_ply": (C.c_int, [p, u64, C.POINTER(VsCloud)]),
+        "vs_cloud_free": (None, [C.POINTER(VsCloud)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
