@@ -424,11 +424,15 @@ typedef struct vs_occupancy {
 #define VS_SHAPE_BOX 1    /* size = half extents (shapes.js:89-108)            */
 #define VS_SHAPE_TORUS 2  /* size[0] major, size[1] minor radius (shapes.js:32-43) */
 
-/* An analytic shape, translated by `offset` (translateShape, shapes.js:148-156). */
+/* An analytic shape moved by `n_offsets` (at most 4) translations
+ * (translateShape, shapes.js:148-156), offset[0] the first applied: its sdf
+ * subtracts offset[n_offsets - 1] from the query point first, and its bounds
+ * add offset[0] first, as nested translateShape calls evaluate. */
 typedef struct vs_shape {
-    int32_t kind, pad0;
+    int32_t kind;
+    uint32_t n_offsets;
     double size[3];
-    double offset[3];
+    double offset[4][3];
 } vs_shape;
 
 /* voxelizeCloud (cloudVolume.js:95-121): the ellipsoids at `sigma` standard
@@ -467,6 +471,11 @@ typedef struct vs_cloud {
     uint64_t count;
     float *positions, *rotations, *scales, *opacity, *colors;
 } vs_cloud;
+/* cloudFromPly over the columns of an element already read: columns[i] holds
+ * the property REQUIRED_PROPERTIES[i] (cloud.js:28-34) names, in that order:
+ * x y z f_dc_0 f_dc_1 f_dc_2 opacity scale_0 scale_1 scale_2 rot_0 rot_1
+ * rot_2 rot_3. */
+int vs_cloud_from_columns(uint64_t count, const float *const *columns, vs_cloud *out);
 /* readPly with the fourteen required properties, then cloudFromPly. */
 int vs_cloud_from_ply(const void *bytes, uint64_t size, vs_cloud *out);
 void vs_cloud_free(vs_cloud *cloud);

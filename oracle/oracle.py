@@ -60,7 +60,8 @@ class OSplats(C.Structure):
 
 
 class OShape(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("pad0", C.c_int32), ("size", C.c_double * 3), ("offset", C.c_double * 3)]
+    _fields_ = [("kind", C.c_int32), ("n_offsets", C.c_uint32), ("size", C.c_double * 3),
+                ("offset", (C.c_double * 3) * 4)]
 
 
 SHAPES = {"sphere": 0, "box": 1, "torus": 2}
@@ -210,11 +211,15 @@ def voxelize_mesh(positions, indices, element_size, padding=2, close=True, bound
 
 
 def voxelize_shape(kind, size, element_size, offset=(0.0, 0.0, 0.0), padding=2):
-    """voxelizeSdf (voxelize.js:120-148) of sphere (radius) / box (half
-    extents) / torus (major, minor)."""
+    """voxelizeSdf (voxelize.js:118-148) of sphere (radius) / box (half
+    extents) / torus (major, minor). `offset` is one translation or a list of
+    up to four, applied in order (translateShape of translateShape ...)."""
     sh = OShape(kind=SHAPES[kind])
     sh.size[:] = list(size) + [0.0] * (3 - len(size))
-    sh.offset[:] = offset
+    offsets = [offset] if np.ndim(offset) == 1 else list(offset)
+    sh.n_offsets = len(offsets)
+    for t, o in enumerate(offsets):
+        sh.offset[t][:] = [float(v) for v in o]
     g = OGrid()
     ptr = C.c_void_p()
     _raise(load("f64").o_voxelize_shape(C.byref(sh), element_size, padding, C.byref(g), C.byref(ptr)))

@@ -403,20 +403,25 @@ int o_voxelize_mesh(const float *pos, uint64_t verts, const uint32_t *idx, uint6
 /* The analytic shapes of apps/simulator/src/assets/shapes.js as signed
  * distances: kind 0 sphere (size[0] = radius, :60-67), 1 box (size = half
  * extents, :89-108), 2 torus (size[0] major, size[1] minor radius, :32-43);
- * `offset` translates (translateShape, :148-156). */
+ * then `n_offsets` translations (translateShape, :148-156), offset[0] the
+ * first one applied to the shape. A translated shape's sdf subtracts the
+ * last translation from the query point first. */
 typedef struct {
-    int32_t kind, pad0;
+    int32_t kind;
+    uint32_t n_offsets;
     double size[3];
-    double offset[3];
+    double offset[4][3];
 } o_shape;
 
 static double hypot2(double a, double b) { return sqrt(a * a + b * b); }
 
 double o_shape_sdf(const o_shape *s, double x, double y, double z)
 {
-    x -= s->offset[0];
-    y -= s->offset[1];
-    z -= s->offset[2];
+    for (uint32_t t = s->n_offsets; t-- > 0;) {
+        x -= s->offset[t][0];
+        y -= s->offset[t][1];
+        z -= s->offset[t][2];
+    }
     if (s->kind == 0) return hypot3(x, y, z) - s->size[0];
     if (s->kind == 1) {
         const double dx = fabs(x) - s->size[0], dy = fabs(y) - s->size[1], dz = fabs(z) - s->size[2];
@@ -443,8 +448,12 @@ void o_shape_bounds(const o_shape *s, double mn[3], double mx[3])
         hi[0] = outer, hi[1] = s->size[1], hi[2] = outer;
     }
     for (int a = 0; a < 3; a++) {
-        mn[a] = lo[a] + s->offset[a];
-        mx[a] = hi[a] + s->offset[a];
+        mn[a] = lo[a];
+        mx[a] = hi[a];
+        for (uint32_t t = 0; t < s->n_offsets; t++) {
+            mn[a] += s->offset[t][a];
+            mx[a] += s->offset[t][a];
+        }
     }
 }
 
@@ -453,6 +462,7 @@ void o_shape_bounds(const o_shape *s, double mn[3], double mx[3])
  * or any corner is inside. */
 int o_voxelize_shape(const o_shape *s, double h, uint32_t padding, o_grid *g, uint8_t **occ)
 {
+    if (s->n_offsets > 4) return -3;
     double mn[3], mx[3];
     o_shape_bounds(s, mn, mx);
     const int rc = o_grid_for_bounds(mn, mx, h, padding, g);

@@ -45,7 +45,16 @@ export function fakeAddon() {
             }
         },
         synchronize() {},
-        launchSize() { return 112; }
+        launchSize() { return 112; },
+        // Voxelisers: the call is recorded and a 2 x 1 x 1 block comes back.
+        voxelCalls: [],
+        voxelizeMesh(...args) { return api.voxelGrid('mesh', args); },
+        voxelizeShape(...args) { return api.voxelGrid('shape', args); },
+        voxelizeCloud(...args) { return api.voxelGrid('cloud', args); },
+        voxelGrid(what, args) {
+            api.voxelCalls.push({ what, args });
+            return { dims: [2, 1, 1], origin: [0, 0, 0], h: 0.01, occupancy: Uint8Array.from([1, 1]) };
+        }
     };
     return api;
 }

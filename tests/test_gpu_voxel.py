@@ -39,6 +39,8 @@ def random_rotations(rng, n):
     ("box", [0.21, 0.013, 0.4], 0.0117, (-1, 2, 0.5), 1),
     ("torus", [0.08, 0.03], 0.22 / 28, (0, 0, 0), 2),
     ("torus", [0.3, 0.11], 0.006, (0.01, 0.02, -0.03), 2),
+    ("torus", [0.3, 0.11], 0.0071, [(0.01, 0.02, -0.03), (0.3, -0.7, 0.11), (-0.05, 0.0, 0.9)], 2),
+    ("box", [0.2, 0.1, 0.3], 0.013, [(0.1, 0.2, 0.3), (0.7, -0.1, 0.05)], 2),
 ])
 def test_shape_equals_oracle(kind, size, h, offset, padding):
     dev = ingest.voxelize_shape(kind, size, h, offset=offset, padding=padding)

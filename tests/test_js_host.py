@@ -124,3 +124,59 @@ def test_js_simulator_matches_oracle(tmp_path, case):
         assert got["hit"]["position"] == [float(v) for v in pos[:, anchor]]
         assert got["hit"]["held"] == len(nodes)
         assert np.max(np.abs(axis - o.out_axis)) / np.max(np.abs(o.out_axis)) <= 1e-5
+
+
+@pytest.mark.gpu
+def test_js_capture_voxelised_on_the_device(tmp_path):
+    # A PLY capture through the JS host: the device grid and the lattice the
+    # JS host builds on it equal the oracle's grid and the Python lattice, and
+    # the frames stepped on it equal the f32 oracle's.
+    import math
+    from _geometry import sphere_shell
+    from oracle import oracle as O
+    from oracle.oracle import OracleScene
+    from test_ply import build_ply
+    from vsim import ingest
+    rng = np.random.default_rng(12)
+    pts = sphere_shell(0.05, 2500)
+    n = len(pts)
+    cols = {"x": pts[:, 0], "y": pts[:, 1], "z": pts[:, 2], "opacity": rng.normal(size=n),
+            "f_dc_0": rng.normal(size=n), "f_dc_1": rng.normal(size=n), "f_dc_2": rng.normal(size=n),
+            "scale_0": np.full(n, math.log(0.003)), "scale_1": np.full(n, math.log(0.002)),
+            "scale_2": np.full(n, math.log(0.001)), "rot_0": rng.normal(size=n), "rot_1": rng.normal(size=n),
+            "rot_2": rng.normal(size=n), "rot_3": rng.normal(size=n)}
+    ply = tmp_path / "capture.ply"
+    data = build_ply([(k, "float", v) for k, v in cols.items()])
+    ply.write_bytes(data)
+    cloud = ingest.cloud_from_ply(data)
+    h = 0.009
+    grid = O.voxelize_cloud(cloud.positions, cloud.rotations, cloud.scales, h)
+    mat = scenes.Material(**scenes.C1_MATERIAL)
+    body = scenes.Body(scenes.build_lattice(grid.array(), grid.origin, grid.h, mat.density), mat)
+    lo = [grid.origin[a] - 0.05 for a in range(3)]
+    hi = [grid.origin[a] + grid.dims[a] * grid.h + 0.05 for a in range(3)]
+    sc = scenes.Scene([body], dt=float(np.float32(1 / 600)), tank_min=tuple(lo), tank_max=tuple(hi), name="jscap")
+    frames = 3
+    spec = dict(ply=str(ply), elementSize=h, tank=dict(min=lo, max=hi), dt=sc.dt, substeps=sc.substeps, frames=frames,
+                material=dict(youngsModulus=mat.youngs_modulus, poissonRatio=mat.poisson_ratio, density=mat.density,
+                              friction=mat.friction, damping=mat.damping, relaxation=mat.relaxation,
+                              maxStretch=mat.max_stretch))
+    path = tmp_path / "spec.json"
+    path.write_text(json.dumps(spec))
+    out = str(tmp_path / "out")
+    r = run_node("tests/js/capture_gpu.mjs", str(path), out)
+    print(r.stdout, r.stderr[-3000:])
+    assert r.returncode == 0
+    meta = json.load(open(os.path.join(out, "meta.json")))
+    assert tuple(meta["cellDims"]) == grid.dims and tuple(meta["origin"]) == grid.origin and meta["h"] == grid.h
+    lat = body.lattice
+    assert np.array_equal(load(out, "cells.bin", np.uint32), lat.cells)
+    assert np.array_equal(load(out, "rest.bin").reshape(-1, 3), lat.rest)
+    assert np.array_equal(load(out, "w.bin"), sc.arrays()["w"])
+    o = OracleScene(sc, "f32")
+    for _ in range(frames):
+        o.frame(sc.substeps)
+    pos = load(out, "pos.bin").reshape(3, -1)
+    err = np.max(np.abs(pos - o.p)) / np.max(np.abs(o.p))
+    print(f"capture: {lat.node_count} nodes, exact {np.mean(pos == o.p):.6f} rel {err:.3e}")
+    assert err <= 1e-5

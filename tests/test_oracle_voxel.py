@@ -200,3 +200,16 @@ def test_mesh_bounds_argument_lays_out_the_grid():
     # a face diagonal (their sample points round differently off a new origin).
     diff = int((wide.array()[:, :, 5:] != own.array()).sum())
     assert not wide.array()[:, :, :5].any() and diff <= 0.01 * own.occupancy.sum()
+
+
+def test_translation_chain():
+    # Offsets exact in binary: two translations are one by their sum.
+    one = O.voxelize_shape("torus", [0.25, 0.125], 0.0625, offset=(0.75, -0.5, 0.25))
+    two = O.voxelize_shape("torus", [0.25, 0.125], 0.0625, offset=[(0.5, -0.25, 0.0), (0.25, -0.25, 0.25)])
+    assert one.dims == two.dims and one.origin == two.origin and np.array_equal(one.occupancy, two.occupancy)
+    # A chain is applied outermost first to the query point: with offsets
+    # whose sum rounds, the chain's origin is the nested sum's.
+    chain = [(0.1, 0.2, 0.3), (0.7, -0.1, 0.05)]
+    g = O.voxelize_shape("sphere", [0.1], 0.01, offset=chain)
+    assert abs(g.origin[0] - ((-0.1 + 0.1) + 0.7 - 0.02 - 0.005)) < 1e-15
+    assert g.occupancy.sum() > 0

@@ -158,3 +158,19 @@ def test_cloud_requires_the_fourteen_properties():
     v = {n: np.zeros(1) for n in ingest.REQUIRED_PROPERTIES if n != "rot_3"}
     with pytest.raises(ValueError, match="has no property rot_3"):
         ingest.cloud_from_ply(build_ply([(n, "float", a) for n, a in v.items()]))
+
+
+def test_a_header_count_past_the_file_is_truncation():
+    # 2^60 records of 15 bytes: the byte count wraps a 64-bit integer.
+    data = build_ply(PROPS).replace(b"element vertex 3", b"element vertex 1152921504606846976")
+    with pytest.raises(ValueError, match="truncated"):
+        ingest.read_ply(data)
+
+
+def test_cloud_from_columns_equals_cloud_from_ply():
+    rng = np.random.default_rng(9)
+    v = {n: rng.normal(size=40).astype(np.float32) for n in ingest.REQUIRED_PROPERTIES}
+    a = ingest.cloud_from_ply(cloud_ply(v))
+    b = ingest.cloud_from_columns(40, v)
+    for name in ("positions", "rotations", "scales", "opacity", "colors"):
+        assert np.array_equal(getattr(a, name), getattr(b, name))

@@ -8,7 +8,10 @@
 // the activations (exp scale, sigmoid opacity, degree-0 colour, unit
 // quaternion with a non-negative real part) are evaluated in double and
 // rounded to f32 on store, like the JS.
+#include <algorithm>
+#include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -142,7 +145,7 @@ int parse_header(const uint8_t *bytes, uint64_t size, Element &element, uint64_t
             if (w.size() > 2) {
                 char *stop = nullptr;
                 const double n = std::strtod(w[2].c_str(), &stop);
-                usable = *stop == 0 && n >= 0 && n == std::floor(n) && n < 9.007199254740992e15;
+                usable = *stop == 0 && n >= 0 && n == std::floor(n) && n < 18446744073709551616.0;
                 element.count = usable ? (uint64_t)n : 0;
             }
         } else if (w[0] == "property" && have) {
@@ -183,11 +186,16 @@ int read_columns(const uint8_t *bytes, uint64_t size, const char *const *names, 
         offset[p] = stride;
         stride += element.properties[p].size;
     }
-    const uint64_t needed = header + stride * element.count;
-    if (size < needed)
+    // The reference sums in doubles; in integers the product can wrap, so
+    // compare without forming it.
+    const bool fits = size >= header && (element.count == 0 || stride <= (size - header) / element.count);
+    if (!fits) {
+        char needed[32];
+        std::snprintf(needed, sizeof needed, "%.17g", (double)header + (double)stride * (double)element.count);
         return ingest_fail(VS_E_INVALID, "PLY is truncated: " + std::to_string(element.count) + " records of " +
-                                             std::to_string(stride) + " bytes need " + std::to_string(needed) +
-                                             " bytes, file has " + std::to_string(size));
+                                             std::to_string(stride) + " bytes need " + needed + " bytes, file has " +
+                                             std::to_string(size));
+    }
     for (uint32_t i = 0; i < n_names; i++) columns[i] = nullptr;
     for (uint32_t i = 0; i < n_names; i++) {
         columns[i] = static_cast<float *>(std::malloc(std::max<uint64_t>(element.count, 1) * sizeof(float)));
@@ -263,16 +271,12 @@ void vs_cloud_free(vs_cloud *c)
     *c = vs_cloud{};
 }
 
-int vs_cloud_from_ply(const void *bytes, uint64_t size, vs_cloud *out)
+int vs_cloud_from_columns(uint64_t n, const float *const *columns, vs_cloud *out)
 {
-    if (!out || (!bytes && size)) return ingest_fail(VS_E_INVALID, "missing argument");
+    if (!out || (n && !columns)) return ingest_fail(VS_E_INVALID, "missing argument");
     *out = vs_cloud{};
-    float *col[14];
-    uint64_t n = 0;
-    int rc = read_columns(static_cast<const uint8_t *>(bytes), size, REQUIRED, 14, &n, col);
-    if (rc != VS_OK) return rc;
-    const float *x = col[0], *y = col[1], *z = col[2], *dc0 = col[3], *dc1 = col[4], *dc2 = col[5], *op = col[6],
-                *s0 = col[7], *s1 = col[8], *s2 = col[9], *r0 = col[10], *r1 = col[11], *r2 = col[12], *r3 = col[13];
+    for (int c = 0; n && c < 14; c++)
+        if (!columns[c]) return ingest_fail(VS_E_INVALID, std::string("missing column ") + REQUIRED[c]);
     const size_t m = std::max<uint64_t>(n, 1);
     out->count = n;
     out->positions = static_cast<float *>(std::malloc(12 * m));
@@ -282,31 +286,46 @@ int vs_cloud_from_ply(const void *bytes, uint64_t size, vs_cloud *out)
     out->colors = static_cast<float *>(std::malloc(12 * m));
     if (!out->positions || !out->rotations || !out->scales || !out->opacity || !out->colors) {
         vs_cloud_free(out);
-        rc = ingest_fail(VS_E_NOMEM, "cloud does not fit host memory");
-    } else {
-        for (uint64_t i = 0; i < n; i++) {
-            out->positions[3 * i] = x[i];
-            out->positions[3 * i + 1] = y[i];
-            out->positions[3 * i + 2] = z[i];
-            out->scales[3 * i] = (float)std::exp((double)s0[i]);
-            out->scales[3 * i + 1] = (float)std::exp((double)s1[i]);
-            out->scales[3 * i + 2] = (float)std::exp((double)s2[i]);
-            out->opacity[i] = (float)sigmoid(op[i]);
-            out->colors[3 * i] = (float)((double)dc0[i] * SH_C0 + 0.5);
-            out->colors[3 * i + 1] = (float)((double)dc1[i] * SH_C0 + 0.5);
-            out->colors[3 * i + 2] = (float)((double)dc2[i] * SH_C0 + 0.5);
-            // writeUnitQuat (cloud.js:99-108): rot_0 is the real part; the
-            // result is xyzw, normalised, real part >= 0, identity when zero.
-            const double qx = r1[i], qy = r2[i], qz = r3[i], qw = r0[i];
-            const double length = std::sqrt(qx * qx + qy * qy + qz * qz + qw * qw);
-            const double sign = qw < 0 ? -1 : 1;
-            const double scale = length > 0 ? sign / length : 0;
-            out->rotations[4 * i] = (float)(qx * scale);
-            out->rotations[4 * i + 1] = (float)(qy * scale);
-            out->rotations[4 * i + 2] = (float)(qz * scale);
-            out->rotations[4 * i + 3] = (float)(length > 0 ? qw * scale : 1);
-        }
+        return ingest_fail(VS_E_NOMEM, "cloud does not fit host memory");
     }
+    if (n == 0) return VS_OK;
+    const float *x = columns[0], *y = columns[1], *z = columns[2], *dc0 = columns[3], *dc1 = columns[4],
+                *dc2 = columns[5], *op = columns[6], *s0 = columns[7], *s1 = columns[8], *s2 = columns[9],
+                *r0 = columns[10], *r1 = columns[11], *r2 = columns[12], *r3 = columns[13];
+    for (uint64_t i = 0; i < n; i++) {
+        out->positions[3 * i] = x[i];
+        out->positions[3 * i + 1] = y[i];
+        out->positions[3 * i + 2] = z[i];
+        out->scales[3 * i] = (float)std::exp((double)s0[i]);
+        out->scales[3 * i + 1] = (float)std::exp((double)s1[i]);
+        out->scales[3 * i + 2] = (float)std::exp((double)s2[i]);
+        out->opacity[i] = (float)sigmoid(op[i]);
+        out->colors[3 * i] = (float)((double)dc0[i] * SH_C0 + 0.5);
+        out->colors[3 * i + 1] = (float)((double)dc1[i] * SH_C0 + 0.5);
+        out->colors[3 * i + 2] = (float)((double)dc2[i] * SH_C0 + 0.5);
+        // writeUnitQuat (cloud.js:99-108): rot_0 is the real part; the result
+        // is xyzw, normalised, real part >= 0, the identity when zero.
+        const double qx = r1[i], qy = r2[i], qz = r3[i], qw = r0[i];
+        const double length = std::sqrt(qx * qx + qy * qy + qz * qz + qw * qw);
+        const double sign = qw < 0 ? -1 : 1;
+        const double scale = length > 0 ? sign / length : 0;
+        out->rotations[4 * i] = (float)(qx * scale);
+        out->rotations[4 * i + 1] = (float)(qy * scale);
+        out->rotations[4 * i + 2] = (float)(qz * scale);
+        out->rotations[4 * i + 3] = (float)(length > 0 ? qw * scale : 1);
+    }
+    return VS_OK;
+}
+
+int vs_cloud_from_ply(const void *bytes, uint64_t size, vs_cloud *out)
+{
+    if (!out || (!bytes && size)) return ingest_fail(VS_E_INVALID, "missing argument");
+    *out = vs_cloud{};
+    float *col[14];
+    uint64_t n = 0;
+    int rc = read_columns(static_cast<const uint8_t *>(bytes), size, REQUIRED, 14, &n, col);
+    if (rc != VS_OK) return rc;
+    rc = vs_cloud_from_columns(n, col, out);
     for (float *c : col) std::free(c);
     return rc;
 }

@@ -571,9 +571,11 @@ __global__ __launch_bounds__(BLOCK) void k_mesh_vote(const uint8_t *votes, const
 // Signed distances of the analytic shapes (shapes.js:32-108), translated.
 __device__ double shape_sdf(const vs_shape &s, double x, double y, double z)
 {
-    x -= s.offset[0];
-    y -= s.offset[1];
-    z -= s.offset[2];
+    for (uint32_t t = s.n_offsets; t-- > 0;) {
+        x -= s.offset[t][0];
+        y -= s.offset[t][1];
+        z -= s.offset[t][2];
+    }
     if (s.kind == VS_SHAPE_SPHERE) return hypot3(x, y, z) - s.size[0];
     if (s.kind == VS_SHAPE_BOX) {
         const double dx = fabs(x) - s.size[0], dy = fabs(y) - s.size[1], dz = fabs(z) - s.size[2];
@@ -963,7 +965,7 @@ int vs_voxelize_mesh(int device, const float *positions, uint64_t vertices, cons
 
 int vs_voxelize_shape(int device, const vs_shape *shape, double element_size, uint32_t padding, vs_occupancy *out)
 {
-    if (!out || !shape || shape->kind < VS_SHAPE_SPHERE || shape->kind > VS_SHAPE_TORUS)
+    if (!out || !shape || shape->kind < VS_SHAPE_SPHERE || shape->kind > VS_SHAPE_TORUS || shape->n_offsets > 4)
         return fail(VS_E_INVALID, "unknown shape");
     *out = vs_occupancy{};
     double lo[3], hi[3];
@@ -978,8 +980,12 @@ int vs_voxelize_shape(int device, const vs_shape *shape, double element_size, ui
     }
     double mn[3], mx[3];
     for (int a = 0; a < 3; a++) {
-        mn[a] = lo[a] + shape->offset[a];
-        mx[a] = hi[a] + shape->offset[a];
+        mn[a] = lo[a];
+        mx[a] = hi[a];
+        for (uint32_t t = 0; t < shape->n_offsets; t++) {
+            mn[a] += shape->offset[t][a];
+            mx[a] += shape->offset[t][a];
+        }
     }
     Grid g;
     const int grid_rc = grid_for_bounds(mn, mx, element_size, padding, g);

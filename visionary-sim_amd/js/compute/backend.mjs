@@ -163,6 +163,8 @@ export class ComputeBackend {
     constructor(device = 0, native = null) {
         this.native = native || addon();
         this.handle = this.native.create(device);
+        /** HIP device index (the voxelisers run there too). */
+        this.device = typeof device === 'number' ? device : 0;
         this.allocations = new Set();
         this.uniformSets = new Set();
         this.destroyed = false;

@@ -3,6 +3,9 @@
 // lattice/lattice.js:44-239 and voxelLattice.js:17-112). Host setup only; it
 // fixes the order of every GPU input.
 
+import { meshBounds, voxelizeMesh, voxelizeSdf } from './voxelize.mjs';
+import { voxelizeCloud } from './cloudVolume.mjs';
+
 export const COLOR_COUNT = 8;
 /** Contact sphere radius as a fraction of the cell (voxelLattice.js:17). */
 export const CONTACT_RADIUS_CELLS = 0.5;
@@ -129,12 +132,30 @@ export class VoxelLattice {
     }
 }
 
-/** Rest geometry kinds this host discretises: an occupancy grid or a block. */
-export function proxyFor(body) {
-    const { source, material } = body;
-    let grid;
-    if (source.kind === 'lattice') grid = source.grid;
-    else if (source.kind === 'block') grid = blockGrid(source.cells, source.origin, source.h);
-    else throw new Error(`no way to discretise rest geometry of kind "${source.kind}"`);
-    return new VoxelLattice(buildLattice(grid, { density: material.density }));
+/**
+ * How each kind of rest geometry becomes an occupancy grid (voxelLattice.js:
+ * 208-212): meshes, analytic shapes and Gaussian clouds are voxelised on the
+ * backend's device; an occupancy grid or a block is taken as given.
+ */
+const VOXELIZERS = {
+    mesh: (source, options) => voxelizeMesh(source.mesh, meshBounds(source.mesh.positions), options),
+    shape: (source, options) => voxelizeSdf(source.shape, options),
+    cloud: (source, options) => voxelizeCloud(source.cloud, options),
+    lattice: source => source.grid,
+    block: source => blockGrid(source.cells, source.origin, source.h)
+};
+
+/**
+ * proxyFor (voxelLattice.js:218-227): the body's lattice, cells
+ * `body.elementSize` across.
+ * @param {object} body - A SoftBody.
+ * @param {object} [backend] - The ComputeBackend whose device and addon voxelise.
+ */
+export function proxyFor(body, backend = null) {
+    const { source, material, elementSize } = body;
+    const voxelize = VOXELIZERS[source.kind];
+    if (!voxelize) throw new Error(`no way to discretise rest geometry of kind "${source.kind}"`);
+    const options = { elementSize };
+    if (backend) Object.assign(options, { device: backend.device, native: backend.native });
+    return new VoxelLattice(buildLattice(voxelize(source, options), { density: material.density }));
 }

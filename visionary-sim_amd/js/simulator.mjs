@@ -18,15 +18,21 @@ import { DEFAULT_GRIPPED_PART_RADIUS, GRIP_STRATEGIES, GrabQueries, bodyOfNode }
 
 const DEFAULT_PARAMS = { gravity: [0, -9.81, 0], paused: false };
 
-/** A body as an author describes it (body/softBody.js:37-60). */
+/**
+ * A body as an author describes it (body/softBody.js:37-60). `source` is
+ * rest geometry: { kind: 'mesh', mesh }, { kind: 'shape', shape },
+ * { kind: 'cloud', cloud } (voxelised on the device at `elementSize`), or an
+ * occupancy grid ({ kind: 'lattice', grid }) or block ({ kind: 'block', ... }).
+ */
 export class SoftBody {
-    constructor({ name, source, material = {}, staticRegion = null, velocity = null }) {
+    constructor({ name, source, material = {}, staticRegion = null, velocity = null, elementSize = 0.01 }) {
         this.name = name;
         this.source = source;
         this.material = Object.assign({}, DEFAULT_MATERIAL, material);
         validateMaxStretch(this.material.maxStretch);
         this.staticRegion = staticRegion;
         this.velocity = velocity;
+        this.elementSize = elementSize;
     }
 }
 
@@ -56,7 +62,7 @@ export class Simulator {
         try {
             this.tank = tank;
             this.params = Object.assign({}, DEFAULT_PARAMS);
-            this.simBodies = bodies.map(body => ({ body, proxy: proxyFor(body) }));
+            this.simBodies = bodies.map(body => ({ body, proxy: proxyFor(body, backend) }));
             this.uniforms = backend.uniforms();
             this.particles = new Particles(backend, this.simBodies.map(({ body, proxy }) => Object.assign({}, proxy.dof, {
                 staticNodes: staticNodesInRegion(proxy.dof.restPositions, body.staticRegion),

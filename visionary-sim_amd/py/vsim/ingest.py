@@ -107,11 +107,17 @@ def voxelize_mesh(positions, indices, element_size, padding=2, close=True, bound
 
 def voxelize_shape(kind, size, element_size, offset=(0.0, 0.0, 0.0), padding=2, device=0):
     """voxelizeSdf of an analytic shape: "sphere" (radius), "box" (half
-    extents) or "torus" (major, minor radius), translated by `offset`."""
+    extents) or "torus" (major, minor radius), translated by `offset`: one
+    translation, or a list of up to four applied in order."""
     lib = native.load()
     sh = native.VsShape(kind=SHAPES[kind])
     sh.size[:] = [float(v) for v in size] + [0.0] * (3 - len(size))
-    sh.offset[:] = [float(v) for v in offset]
+    offsets = [offset] if np.ndim(offset) == 1 else list(offset)
+    if len(offsets) > 4:
+        raise ValueError("voxelizeSdf: at most four translations")
+    sh.n_offsets = len(offsets)
+    for t, o in enumerate(offsets):
+        sh.offset[t][:] = [float(v) for v in o]
     occ = native.VsOccupancy()
     if lib.vs_voxelize_shape(device, C.byref(sh), float(element_size), int(padding), C.byref(occ)) != native.VS_OK:
         _fail(lib)
@@ -174,6 +180,24 @@ def cloud_from_ply(data):
     c = native.VsCloud()
     if lib.vs_cloud_from_ply(data, len(data), C.byref(c)) != native.VS_OK:
         _fail(lib)
+    return _take_cloud(lib, c)
+
+
+def cloud_from_columns(count, properties):
+    """cloudFromPly (cloud.js:52-81) over an element already read:
+    `properties` maps each required name to a float32 array of `count`."""
+    lib = native.load()
+    cols = [np.ascontiguousarray(properties[n], np.float32) for n in REQUIRED_PROPERTIES]
+    if any(len(a) != count for a in cols):
+        raise ValueError("cloudFromPly: every property needs `count` values")
+    ptrs = (C.POINTER(C.c_float) * 14)(*[native.ptr(a) for a in cols])
+    c = native.VsCloud()
+    if lib.vs_cloud_from_columns(count, ptrs, C.byref(c)) != native.VS_OK:
+        _fail(lib)
+    return _take_cloud(lib, c)
+
+
+def _take_cloud(lib, c):
     n = c.count
     try:
         arr = lambda ptr, w: np.ctypeslib.as_array(ptr, shape=(n * w,)).reshape(n, w).copy() if n else \

@@ -125,7 +125,7 @@ EXPORTS = [
     "vs_server_create", "vs_server_destroy", "vs_server_error", "vs_server_submit", "vs_server_run",
     "vs_server_latencies", "vs_server_result", "vs_server_times",
     "vs_voxelize_cloud", "vs_voxelize_mesh", "vs_voxelize_shape", "vs_occupancy_free", "vs_ingest_error",
-    "vs_ply_read", "vs_ply_properties", "vs_cloud_from_ply", "vs_cloud_free",
+    "vs_ply_read", "vs_ply_properties", "vs_cloud_from_columns", "vs_cloud_from_ply", "vs_cloud_free",
 ]
 
 SPLAT_COV, SPLAT_AXES = 1, 2  # VS_SPLAT_* output bits of a frame's `splats`
@@ -141,7 +141,8 @@ SHAPE_SPHERE, SHAPE_BOX, SHAPE_TORUS = 0, 1, 2
 
 
 class VsShape(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("pad0", C.c_int32), ("size", C.c_double * 3), ("offset", C.c_double * 3)]
+    _fields_ = [("kind", C.c_int32), ("n_offsets", C.c_uint32), ("size", C.c_double * 3),
+                ("offset", (C.c_double * 3) * 4)]
 
 
 class VsCloud(C.Structure):
@@ -211,6 +212,7 @@ def load(path=LIB_PATH):
         "vs_ingest_error": (C.c_char_p, []),
         "vs_ply_read": (C.c_int, [p, u64, C.POINTER(C.c_char_p), u32, C.POINTER(u64), C.POINTER(C.POINTER(f))]),
         "vs_ply_properties": (C.c_int, [p, u64, C.c_char_p, u64, C.POINTER(u64)]),
+        "vs_cloud_from_columns": (C.c_int, [u64, C.POINTER(C.POINTER(f)), C.POINTER(VsCloud)]),
         "vs_cloud_from_ply": (C.c_int, [p, u64, C.POINTER(VsCloud)]),
         "vs_cloud_free": (None, [C.POINTER(VsCloud)]),
     }
