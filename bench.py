@@ -125,6 +125,78 @@ def serve_leg(count, budget_nodes):
             "max_active": rep["max_active"], "iterations": rep["iterations"], "completed": rep["completed"]}
 
 
+def capture_cloud(n, seed=1):
+    """A synthetic capture: n Gaussians on a noisy ellipsoid shell (0.5 x 0.36 x
+    0.6 m), random orientations, log-normal scales around 2 mm."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos = (d * (1 + rng.normal(scale=0.02, size=(n, 1))) * [0.25, 0.18, 0.3]).astype(np.float32)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    q[q[:, 3] < 0] *= -1
+    s = np.exp(rng.normal(math.log(0.002), 0.5, size=(n, 3))).astype(np.float32)
+    return pos, q.astype(np.float32), s
+
+
+def torus_mesh(major, minor, around, ring):
+    """A closed torus surface of 2 * around * ring triangles."""
+    u, v = np.meshgrid(np.arange(around) / around * 2 * np.pi, np.arange(ring) / ring * 2 * np.pi, indexing="ij")
+    r = major + minor * np.cos(u)
+    p = np.stack([r * np.cos(v), minor * np.sin(u), r * np.sin(v)], -1).reshape(-1, 3).astype(np.float32)
+    i, j = np.meshgrid(np.arange(around), np.arange(ring), indexing="ij")
+    a, b = i * ring + j, ((i + 1) % around) * ring + j
+    c, d = ((i + 1) % around) * ring + (j + 1) % ring, i * ring + (j + 1) % ring
+    tri = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3), np.stack([a, c, d], -1).reshape(-1, 3)])
+    return p, tri.astype(np.uint32).reshape(-1)
+
+
+def ingest_leg(cpu):
+    """The device voxelisers on capture-sized inputs (best of 3 calls, host
+    arrays in and out, PCIe included), and the oracle's single-core time on
+    the same algorithm and inputs."""
+    from vsim import ingest
+
+    def best(fn, reps=3):
+        fn()
+        t = []
+        for _ in range(reps):
+            a = time.perf_counter()
+            g = fn()
+            t.append(time.perf_counter() - a)
+        return min(t), g
+
+    out = {}
+    p, q, s = capture_cloud(1_000_000)
+    t, g = best(lambda: ingest.voxelize_cloud(p, q, s, 0.004))
+    out["cloud"] = {"gaussians": len(p), "element_size": 0.004, "grid": list(g.dims), "occupied": g.count(),
+                    "gpu_ms": t * 1e3, "gaussians_per_s": len(p) / t}
+    mp, mix = torus_mesh(0.3, 0.1, 256, 512)
+    t, g = best(lambda: ingest.voxelize_mesh(mp, mix, 0.002))
+    out["mesh"] = {"triangles": len(mix) // 3, "element_size": 0.002, "grid": list(g.dims), "occupied": g.count(),
+                   "gpu_ms": t * 1e3, "triangles_per_s": len(mix) // 3 / t}
+    t, g = best(lambda: ingest.voxelize_shape("torus", [0.3, 0.1], 0.002))
+    out["shape"] = {"kind": "torus", "element_size": 0.002, "grid": list(g.dims), "gpu_ms": t * 1e3,
+                    "cells_per_s": int(np.prod(g.dims)) / t}
+    if cpu:
+        from oracle import oracle as O
+        k = len(p)
+        a = time.perf_counter()
+        O.voxelize_cloud(p, q, s, 0.004)
+        tc = time.perf_counter() - a
+        a = time.perf_counter()
+        O.voxelize_mesh(mp, mix, 0.002)
+        tm = time.perf_counter() - a
+        a = time.perf_counter()
+        O.voxelize_shape("torus", [0.3, 0.1], 0.002)
+        ts = time.perf_counter() - a
+        out["cpu_oracle"] = {"cores": 1, "kind": "port",
+                             "cloud": {"gaussians": k, "ms": tc * 1e3, "gaussians_per_s": k / tc},
+                             "mesh": {"ms": tm * 1e3, "triangles_per_s": len(mix) // 3 / tm},
+                             "shape": {"ms": ts * 1e3}}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +216,9 @@ def main():
                     help="scene requests of the serving leg (Poisson 50/s, 4k-500k nodes, 120 frames); 0 skips it")
     ap.add_argument("--serve-budget-nodes", type=float, default=6e6,
                     help="node capacity of the serving world (its HBM budget for node state)")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the voxeliser leg")
+    ap.add_argument("--ingest-only", action="store_true",
+                    help="run only the voxeliser leg (GPU side) and print its JSON")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a device-to-device copy of 1 GiB (5 times), the reference kernel "
                          "for scaling FETCH_SIZE/WRITE_SIZE counters to bytes")
@@ -157,6 +232,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
+        return
+    if args.ingest_only:
+        print(json.dumps(ingest_leg(cpu=False)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); scenes shard across GPUs with no "
@@ -256,6 +334,11 @@ def main():
         serving = serve_leg(args.serve_requests, int(args.serve_budget_nodes))
         log(f"[bench] serving: {serving}")
 
+    ingest_out = None
+    if not args.no_ingest:
+        ingest_out = ingest_leg(cpu=not args.no_cpu)
+        log(f"[bench] ingest: {ingest_out}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scs, args.cpu_budget)
@@ -287,6 +370,7 @@ def main():
                      "avg_launch_us_with_events": prof["elastic"][0] / prof["elastic"][1] * 1e3},
         "splat_pass_us": splat_us,
         "serving": serving,
+        "ingest": ingest_out,
         "cpu_baseline": cpu,
     }
     if tune:

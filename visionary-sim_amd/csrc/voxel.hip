@@ -166,7 +166,10 @@ __device__ Stamp stamp_of(const float *pos, const float *rot, const float *scale
     return st;
 }
 
-// Units of Gaussian i: at least one, which also stamps the centre.
+// Units of Gaussian i: one per UNIT cells of its box and at least one, the
+// first also stamping the centre. (A direct one-lane-per-Gaussian pass for
+// the small boxes measured slower: a capture's scales spread wide enough
+// that its lanes diverge.)
 __device__ __forceinline__ uint64_t stamp_units(const float *pos, const float *rot, const float *scale, uint64_t i,
                                                 double sigma, const Grid &g)
 {
@@ -186,50 +189,58 @@ __device__ __forceinline__ uint64_t owner(const uint64_t *start, uint64_t items,
     return lo;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_cloud_stamp(const float *pos, const float *rot, const float *scale,
-                                                       uint64_t count, double sigma, Grid g, const uint64_t *start,
-                                                       uint64_t units, uint8_t *occ)
+// Tests cells [first, last) of a Gaussian's box (x fastest), and stamps the
+// centre's cell when `first` is 0.
+__device__ void stamp_cells(const Stamp &st, const Grid &g, uint64_t first, uint64_t last, uint8_t *occ)
 {
     const uint64_t n = cells_of(g);
+    if (first == 0) {
+        int64_t cc[3];
+        for (int a = 0; a < 3; a++) cc[a] = clampi(floor((st.c[a] - g.origin[a]) / g.h), 0, (int64_t)g.dims[a] - 1);
+        occ[cell_at(g, cc[0], cc[1], cc[2])] = 1;
+    }
+    for (uint64_t b = first; b < last; b++) {
+        const int64_t ii = st.lo[0] + (int64_t)(b % (uint64_t)st.size[0]);
+        const int64_t j = st.lo[1] + (int64_t)((b / (uint64_t)st.size[0]) % (uint64_t)st.size[1]);
+        const int64_t k = st.lo[2] + (int64_t)(b / ((uint64_t)st.size[0] * (uint64_t)st.size[1]));
+        // On a grid under three cells thick the box can reach index -1;
+        // like a typed array, an index outside the grid is skipped.
+        const int64_t lin = ii + (int64_t)g.dims[0] * (j + (int64_t)g.dims[1] * k);
+        if (lin < 0 || (uint64_t)lin >= n) continue;
+        const size_t cell = (size_t)lin;
+        if (occ[cell]) continue;
+        for (int s = 0; s < 9; s++) {
+            const double ox = s == 8 ? 0.5 : (double)(s & 1), oy = s == 8 ? 0.5 : (double)((s >> 1) & 1),
+                         oz = s == 8 ? 0.5 : (double)((s >> 2) & 1);
+            const double dx = g.origin[0] + ((double)ii + ox) * g.h - st.c[0];
+            const double dy = g.origin[1] + ((double)j + oy) * g.h - st.c[1];
+            const double dz = g.origin[2] + ((double)k + oz) * g.h - st.c[2];
+            double sum = 0;
+            for (int a = 0; a < 3; a++) {
+                const double t = ((double)st.col[a * 3] * dx + (double)st.col[a * 3 + 1] * dy +
+                                  (double)st.col[a * 3 + 2] * dz) / st.radius[a];
+                sum += t * t;
+            }
+            if (sum <= 1) {
+                occ[cell] = 1;
+                break;
+            }
+        }
+    }
+}
+
+// UNIT cells of one Gaussian's box per lane (the owner found by binary
+// search over the Gaussians' first units).
+__global__ __launch_bounds__(BLOCK) void k_cloud_stamp(const float *pos, const float *rot, const float *scale,
+                                                             uint64_t count, double sigma, Grid g,
+                                                             const uint64_t *start, uint64_t units, uint8_t *occ)
+{
     for (uint64_t unit = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; unit < units;
          unit += (uint64_t)gridDim.x * BLOCK) {
         const uint64_t i = owner(start, count, unit);
         const Stamp st = stamp_of(pos, rot, scale, i, sigma, g);
         const uint64_t first = (unit - start[i]) * UNIT;
-        if (first == 0) {
-            int64_t cc[3];
-            for (int a = 0; a < 3; a++) cc[a] = clampi(floor((st.c[a] - g.origin[a]) / g.h), 0, (int64_t)g.dims[a] - 1);
-            occ[cell_at(g, cc[0], cc[1], cc[2])] = 1;
-        }
-        const uint64_t last = first + UNIT < st.cells ? first + UNIT : st.cells;
-        for (uint64_t b = first; b < last; b++) {
-            const int64_t ii = st.lo[0] + (int64_t)(b % (uint64_t)st.size[0]);
-            const int64_t j = st.lo[1] + (int64_t)((b / (uint64_t)st.size[0]) % (uint64_t)st.size[1]);
-            const int64_t k = st.lo[2] + (int64_t)(b / ((uint64_t)st.size[0] * (uint64_t)st.size[1]));
-            // On a grid under three cells thick the box can reach index -1;
-            // like a typed array, an index outside the grid is skipped.
-            const int64_t lin = ii + (int64_t)g.dims[0] * (j + (int64_t)g.dims[1] * k);
-            if (lin < 0 || (uint64_t)lin >= n) continue;
-            const size_t cell = (size_t)lin;
-            if (occ[cell]) continue;
-            for (int s = 0; s < 9; s++) {
-                const double ox = s == 8 ? 0.5 : (double)(s & 1), oy = s == 8 ? 0.5 : (double)((s >> 1) & 1),
-                             oz = s == 8 ? 0.5 : (double)((s >> 2) & 1);
-                const double dx = g.origin[0] + ((double)ii + ox) * g.h - st.c[0];
-                const double dy = g.origin[1] + ((double)j + oy) * g.h - st.c[1];
-                const double dz = g.origin[2] + ((double)k + oz) * g.h - st.c[2];
-                double sum = 0;
-                for (int a = 0; a < 3; a++) {
-                    const double t = ((double)st.col[a * 3] * dx + (double)st.col[a * 3 + 1] * dy +
-                                      (double)st.col[a * 3 + 2] * dz) / st.radius[a];
-                    sum += t * t;
-                }
-                if (sum <= 1) {
-                    occ[cell] = 1;
-                    break;
-                }
-            }
-        }
+        stamp_cells(st, g, first, first + UNIT < st.cells ? first + UNIT : st.cells, occ);
     }
 }
 
@@ -279,7 +290,10 @@ __global__ __launch_bounds__(BLOCK) void k_flood_seed(Grid g, const uint8_t *occ
 __global__ __launch_bounds__(BLOCK) void k_flood_sweep(Grid g, const uint8_t *occ, uint8_t *reached, int axis,
                                                        int forward, uint32_t *changed)
 {
-    const int u = (axis + 1) % 3, v = (axis + 2) % 3;
+    // Neighbouring lanes take neighbouring lines along the lower of the other
+    // two axes (x unless the sweep runs along x), so their bytes share cache
+    // lines.
+    const int u = axis == 0 ? 1 : 0, v = axis == 2 ? 1 : 2;
     const size_t lines = (size_t)g.dims[u] * g.dims[v];
     for (size_t l = blockIdx.x * (size_t)BLOCK + threadIdx.x; l < lines; l += (size_t)gridDim.x * BLOCK) {
         int64_t ijk[3];
@@ -434,6 +448,15 @@ __global__ __launch_bounds__(BLOCK) void k_add_offsets(uint64_t *start, uint64_t
         start[e] += tile_offset[e / BLOCK];
 }
 
+// Where column (cu2, cv2) of `axis` keeps its crossings. Only memory layout:
+// x runs fastest where x is one of the column's axes (along y the columns
+// are (z, x)), so the lanes that walk neighbouring columns share cache lines.
+__device__ __forceinline__ size_t column_of(const Grid &g, int axis, int64_t cu2, int64_t cv2)
+{
+    const int u = (axis + 1) % 3, v = (axis + 2) % 3;
+    return axis == 1 ? (size_t)cv2 + (size_t)g.dims[v] * (size_t)cu2 : (size_t)cu2 + (size_t)g.dims[u] * (size_t)cv2;
+}
+
 // The ray tests of accumulateAxis (voxelize.js:308-326). offset == nullptr:
 // count crossings per column; else write each into its column's segment
 // (the column's offset plus a slot from the fill counter).
@@ -459,7 +482,7 @@ __global__ __launch_bounds__(BLOCK) void k_mesh_crossings(const float *pos, cons
             if (l1 < 0) continue;
             const double l2 = 1 - l0 - l1;
             if (l2 < 0) continue;
-            const size_t col = (size_t)cu2 + (size_t)g.dims[u] * (size_t)cv2;
+            const size_t col = column_of(g, axis, cu2, cv2);
             const uint32_t slot = atomicAdd(&count[col], 1u);
             if (offset) hits[offset[col] + slot] = l0 * r.aw + l1 * r.bw + l2 * r.cw;
         }
@@ -517,8 +540,14 @@ __global__ __launch_bounds__(BLOCK) void k_mesh_columns(Grid g, int axis, const 
         double *h = hits + offset[col];
         sort_hits(h, n);
         int64_t ijk[3];
-        ijk[u] = (int64_t)(col % g.dims[u]);
-        ijk[v] = (int64_t)(col / g.dims[u]);
+        // column_of, inverted
+        if (axis == 1) {
+            ijk[v] = (int64_t)(col % g.dims[v]);
+            ijk[u] = (int64_t)(col / g.dims[v]);
+        } else {
+            ijk[u] = (int64_t)(col % g.dims[u]);
+            ijk[v] = (int64_t)(col / g.dims[u]);
+        }
         for (uint32_t e = 0; e < n; e++) {
             const double cw2 = floor((h[e] - g.origin[w]) / g.h);
             if (cw2 >= 0 && cw2 < g.dims[w]) {
