@@ -209,9 +209,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--fuse", action="store_true",
-                    help="fold predict / finalize into the first / last colour pass (same results; on this "
-                         "workload's even-sided blocks the leftover face nodes make it slower)")
+    ap.add_argument("--separate-node-passes", action="store_true",
+                    help="run each substep's finalize and the next one's predict as two node passes (same "
+                         "results; the default merges them inside a frame)")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="scene requests of the serving leg (Poisson 50/s, 4k-500k nodes, 120 frames); 0 skips it")
     ap.add_argument("--serve-budget-nodes", type=float, default=6e6,
@@ -262,7 +262,7 @@ def main():
     log(f"[bench] {len(scs)} scenes: {nodes} nodes, {cells} cells, {splats} splats (generated in "
         f"{time.perf_counter() - t_gen:.1f} s)")
 
-    world = make_world(scs, fuse=int(args.fuse))
+    world = make_world(scs, separate_node_passes=int(args.separate_node_passes))
     ids = []
     for sc in scs:
         ids.append(world.add(sc))

@@ -222,9 +222,10 @@ typedef struct vs_world_config {
     uint32_t block_cells;   /* threads per workgroup of the colour passes      */
     uint32_t persistent;    /* workgroups of a persistent launch (0 = 1 per tile) */
     uint32_t xcd_chunked;   /* 1: give each XCD a contiguous run of tiles      */
-    uint32_t fuse;          /* 1: fold predict into the first colour pass and
-                               finalize into the last where nothing runs between
-                               (same results; node passes cover the rest)     */
+    uint32_t separate_node_passes; /* 1: keep every substep's finalize and the
+                               next substep's predict as two node passes; 0 (the
+                               default) runs them as one pass inside a frame
+                               (same arithmetic, same results)                 */
 } vs_world_config;
 
 /* One body of a scene, in the reference's host layout: the dense node grid
@@ -341,7 +342,8 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out);
 #define VS_P_SETTLE 6
 #define VS_P_FINALIZE 7
 #define VS_P_SPLAT 8
-#define VS_P_COUNT 9
+#define VS_P_FINALIZE_PREDICT 9 /* a substep's finalize and the next one's predict */
+#define VS_P_COUNT 10
 /* One frame of every admitted scene, batched but not captured, with HIP
  * events around every launch: ms[k] is the summed duration of the launches of
  * kind k, launches[k] their number. Synchronous. */
@@ -353,8 +355,7 @@ int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, 
  * so the simulation is left where it was. Synchronous. */
 int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, uint32_t *launches);
 /* Re-times the colour-pass launch shapes on the admitted scenes and keeps the
- * fastest (block size x cells per tile x persistent grid x XCD mapping), then
- * the fastest shape with the fused schedule (vs_world_config.fuse). */
+ * fastest (block size x cells per tile x persistent grid x XCD mapping). */
 int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t report_bytes);
 
 /* ------------------------------------------------------ the server */

@@ -234,11 +234,13 @@ def _sparse_scene(seed, shape=(9, 12, 10), fill=0.65, pinned=False):
     return occupied_scene(np.random.default_rng(seed).random(shape) < fill, pinned=pinned, name=f"sp{seed}")
 
 
-@pytest.mark.parametrize("case", ["c1", "odd_block", "sparse", "sparse_pinned", "hard_drive"])
-def test_fused_frames_equal_unfused_and_oracle(case):
-    # Predict folded into colour 0 and finalize into colour 7, the node passes
-    # covering what those colours do not: the same frames, bit for bit.
+@pytest.mark.parametrize("case", ["c1", "odd_block", "sparse", "sparse_pinned", "sparse_device", "hard_drive",
+                                  "compliant_drive", "contact", "stretch"])
+def test_merged_node_passes_equal_separate_and_oracle(case):
+    # A substep's finalize and the next one's predict as one node pass: the
+    # same frames, bit for bit, as the two passes, and the oracle's frames.
     from oracle.oracle import OracleScene
+    device = False
     if case == "c1":
         sc = scenes.c1_scene(splats=0)
     elif case == "odd_block":
@@ -247,23 +249,34 @@ def test_fused_frames_equal_unfused_and_oracle(case):
         sc = _sparse_scene(11)
     elif case == "sparse_pinned":
         sc = _sparse_scene(12, shape=(8, 9, 14), pinned=True)
-    else:
+    elif case == "sparse_device":
+        sc, device = _sparse_scene(13, shape=(11, 7, 9)), True
+    elif case in ("hard_drive", "compliant_drive"):
         sc = scenes.jelly_block(10, seed=5, pinned=False, tank_margin=0.3, name="hd")
-        sc.drive = scenes.handle_drive(sc, seed=6, law=0)
+        sc.drive = scenes.handle_drive(sc, seed=6, law=0 if case == "hard_drive" else 1)
+    elif case == "contact":
+        sc = scenes.contact_stack(count=3, side=4)
+    else:
+        sc = scenes.jelly_block(6, seed=8, name="st")
+        sc.bodies[0].material.max_stretch = 1.05
     out = []
-    for fuse in (1, 0):
-        w = world_for(sc, fuse=fuse)
-        sid = w.add(sc)
+    for separate in (0, 1):
+        w = world_for(sc, separate_node_passes=separate)
+        sid = w.add(sc, device_lattice=device)
         for f in range(3):
             if sc.drive is not None:
                 w.set_target(sid, *sc.drive.frame_target(f, sc.substeps * sc.dt))
             w.frame(sc.substeps, False, f == 1)
         out.append(w.read_nodes(sid))
+        prof = w.profile(sc.substeps)
+        assert prof["finalize+predict"][1] == (0 if separate else sc.substeps - 1)
+        assert prof["predict"][1] == (sc.substeps if separate else 1)
         w.close()
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
-    o = OracleScene(sc, "f32")
-    for f in range(3):
-        o.frame(sc.substeps)
-    print(f"{case}: exact vs oracle {np.mean(out[0][0] == o.p):.6f}")
-    assert rel_err(out[0][0], o.p) <= RTOL
-    assert rel_err(out[0][1], o.v) <= RTOL
+    if sc.drive is None:
+        o = OracleScene(sc, "f32")
+        for f in range(3):
+            o.frame(sc.substeps)
+        print(f"{case}: exact vs oracle {np.mean(out[0][0] == o.p):.6f}")
+        assert rel_err(out[0][0], o.p) <= RTOL
+        assert rel_err(out[0][1], o.v) <= RTOL

@@ -67,11 +67,6 @@ struct SplatView {
     uint32_t mode;     // VS_SPLAT_COV | VS_SPLAT_AXES
 };
 
-// Cover byte of a node: which fused colour passes own it (the node is a
-// corner of an occupied cell of that colour).
-constexpr uint8_t COVER_FIRST = 1;  // colour 0
-constexpr uint8_t COVER_LAST = 2;   // colour 7
-
 // One body's lattice build on the device (lattice.hip).
 struct LatticeView {
     const uint8_t *occ;          // cell_dims product bytes, x fastest
@@ -88,7 +83,6 @@ struct LatticeView {
     uint32_t *cells;             // the body's range of the cell pool
     uint32_t *s_node, *s_body;   // the body's range of the surface planes
     float *s_radius;
-    uint8_t *cover;              // world cover bytes (COVER_FIRST | COVER_LAST)
 };
 
 struct Shape {
@@ -99,18 +93,14 @@ struct Shape {
 
 // Host launchers; each enqueues one kernel on `stream`. They return the HIP
 // launch status.
-// Node passes; with `cover`, nodes whose cover byte has a bit of `skip` set
-// are skipped (a fused colour pass handles them).
-hipError_t launch_predict(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, TileView t,
-                          const uint8_t *cover = nullptr, uint32_t skip = 0);
+hipError_t launch_predict(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, TileView t);
 hipError_t launch_finalize(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
-                           const vs_scene_params *scenes, TileView t, const uint8_t *cover = nullptr,
-                           uint32_t skip = 0);
-// `fuse`: 0 the colour pass alone; 1 predict each corner first (colour 0);
-// 2 finalize each corner after (the last colour).
+                           const vs_scene_params *scenes, TileView t);
+// One substep's finalize and the next one's predict in one node pass.
+hipError_t launch_finalize_predict(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
+                                  const vs_scene_params *scenes, TileView t);
 hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
-                          const vs_scene_params *scenes, const uint32_t *cells, TileView t, uint32_t limit_only,
-                          int fuse = 0);
+                          const vs_scene_params *scenes, const uint32_t *cells, TileView t, uint32_t limit_only);
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes,
                               SurfaceView sv, GridView g, TileView t);
 hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv,

@@ -212,11 +212,8 @@ __device__ __forceinline__ void predict_node(V3 &p, V3 &q, V3 &v, float gx, floa
     p = mk(p.x + v.x * dt, p.y + v.y * dt, p.z + v.z * dt);
 }
 
-// A node pass over the tiles; with `cover`, nodes whose cover byte has
-// `skip` set are left to the colour pass that predicts them itself.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_predict(NodeView n, const vs_scene_params *__restrict__ scenes, TileView tv,
-                                                   const uint8_t *__restrict__ cover, uint32_t skip)
+__global__ __launch_bounds__(BLOCK) void k_predict(NodeView n, const vs_scene_params *__restrict__ scenes, TileView tv)
 {
     const Range r = tile_range(*tv.count, 0);
     for (uint32_t t = r.first; t < r.end; t += r.stride) {
@@ -225,7 +222,6 @@ __global__ __launch_bounds__(BLOCK) void k_predict(NodeView n, const vs_scene_pa
         const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
         for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
             const uint32_t k = tile.begin + i;
-            if (cover && (cover[k] & skip)) continue;
             if (n.w[k] == 0.0f) continue;
             V3 v = ld3(n.vx, n.vy, n.vz, k), p = ld3(n.px, n.py, n.pz, k), q;
             predict_node(p, q, v, gx, gy, gz, dt);
@@ -288,8 +284,7 @@ __device__ __forceinline__ void finalize_node(V3 &p, V3 prev, V3 &v, const Tank 
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_finalize(NodeView n, const vs_body *__restrict__ bodies,
-                                                    const vs_scene_params *__restrict__ scenes, TileView tv,
-                                                    const uint8_t *__restrict__ cover, uint32_t skip)
+                                                    const vs_scene_params *__restrict__ scenes, TileView tv)
 {
     const Range r = tile_range(*tv.count, 0);
     for (uint32_t t = r.first; t < r.end; t += r.stride) {
@@ -297,7 +292,6 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(NodeView n, const vs_body *_
         const Tank tk = tank_of(scenes[tile.scene], bodies[tile.body]);
         for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
             const uint32_t k = tile.begin + i;
-            if (cover && (cover[k] & skip)) continue;
             if (n.w[k] == 0.0f) continue;
             const V3 prev = ld3(n.qx, n.qy, n.qz, k);
             V3 p = ld3(n.px, n.py, n.pz, k), v;
@@ -312,18 +306,40 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(NodeView n, const vs_body *_
     }
 }
 
-// ------------------------------------------------------------- elastic
-// FUSE selects what the pass does around the projection:
-//   FUSE_NONE      the colour pass of project.wgsl;
-//   FUSE_PREDICT   predict (predict.wgsl) each corner first. The cells of one
-//                  colour own disjoint corner sets, so colour 0 can predict
-//                  the nodes it covers itself; a node pass predicts the rest;
-//   FUSE_FINALIZE  finalize (finalize.wgsl) each corner after it, for the last
-//                  colour when nothing (contact, stretch sweeps) comes between.
-// Every node is still predicted and finalized once, with the same arithmetic.
-enum { FUSE_NONE = 0, FUSE_PREDICT = 1, FUSE_FINALIZE = 2 };
+// The last finalize of one substep and the predict of the next, per node in
+// one pass (nothing runs between them in Stepper.sequence): the node passes'
+// arithmetic in their order, 64 B a node instead of 52 + 64.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_finalize_predict(NodeView n, const vs_body *__restrict__ bodies,
+                                                            const vs_scene_params *__restrict__ scenes, TileView tv)
+{
+    const Range r = tile_range(*tv.count, 0);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_scene_params &sc = scenes[tile.scene];
+        const Tank tk = tank_of(sc, bodies[tile.body]);
+        const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
+        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
+            const uint32_t k = tile.begin + i;
+            if (n.w[k] == 0.0f) continue;
+            V3 q = ld3(n.qx, n.qy, n.qz, k), p = ld3(n.px, n.py, n.pz, k), v;
+            finalize_node(p, q, v, tk);
+            predict_node(p, q, v, gx, gy, gz, dt);
+            n.qx[k] = q.x;
+            n.qy[k] = q.y;
+            n.qz[k] = q.z;
+            n.px[k] = p.x;
+            n.py[k] = p.y;
+            n.pz[k] = p.z;
+            n.vx[k] = v.x;
+            n.vy[k] = v.y;
+            n.vz[k] = v.z;
+        }
+    }
+}
 
-template <int BLOCK, int FUSE>
+// ------------------------------------------------------------- elastic
+template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__restrict__ bodies,
                                                    const vs_scene_params *__restrict__ scenes,
                                                    const uint32_t *__restrict__ cells, TileView tv,
@@ -359,41 +375,13 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
             for (int k = 0; k < 4; k++) {
                 const F2 x = ld2(n.px, off[k]), y = ld2(n.py, off[k]), z = ld2(n.pz, off[k]);
                 const F2 m = ld2(n.w, off[k]);
+                const F2 u = ld2(n.qx, off[k]), v = ld2(n.qy, off[k]), s = ld2(n.qz, off[k]);
                 p[2 * k] = mk(x.x, y.x, z.x);
                 p[2 * k + 1] = mk(x.y, y.y, z.y);
                 w[2 * k] = m.x;
                 w[2 * k + 1] = m.y;
-                if (FUSE == FUSE_PREDICT) {
-                    // A node with w = 0 is not predicted and keeps its prev;
-                    // the others' prev becomes their position.
-                    F2 u = { p[2 * k].x, p[2 * k + 1].x }, v = { p[2 * k].y, p[2 * k + 1].y },
-                       s = { p[2 * k].z, p[2 * k + 1].z };
-                    if (m.x == 0.0f || m.y == 0.0f) {
-                        u = ld2(n.qx, off[k]);
-                        v = ld2(n.qy, off[k]);
-                        s = ld2(n.qz, off[k]);
-                    }
-                    q[2 * k] = mk(u.x, v.x, s.x);
-                    q[2 * k + 1] = mk(u.y, v.y, s.y);
-                    const F2 vx = ld2(n.vx, off[k]), vy = ld2(n.vy, off[k]), vz = ld2(n.vz, off[k]);
-                    V3 vel[2] = { mk(vx.x, vy.x, vz.x), mk(vx.y, vy.y, vz.y) };
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        if (w[2 * k + h] != 0.0f)
-                            predict_node(p[2 * k + h], q[2 * k + h], vel[h], sc.gravity[0], sc.gravity[1],
-                                         sc.gravity[2], dt);
-                    }
-                    st2(n.qx, off[k], q[2 * k].x, q[2 * k + 1].x);
-                    st2(n.qy, off[k], q[2 * k].y, q[2 * k + 1].y);
-                    st2(n.qz, off[k], q[2 * k].z, q[2 * k + 1].z);
-                    st2(n.vx, off[k], vel[0].x, vel[1].x);
-                    st2(n.vy, off[k], vel[0].y, vel[1].y);
-                    st2(n.vz, off[k], vel[0].z, vel[1].z);
-                } else {
-                    const F2 u = ld2(n.qx, off[k]), v = ld2(n.qy, off[k]), s = ld2(n.qz, off[k]);
-                    q[2 * k] = mk(u.x, v.x, s.x);
-                    q[2 * k + 1] = mk(u.y, v.y, s.y);
-                }
+                q[2 * k] = mk(u.x, v.x, s.x);
+                q[2 * k + 1] = mk(u.y, v.y, s.y);
             }
             if (!limit_only) {
                 project_tet<0>(p, q, w, rs, bd.gamma, aD, aH, relax);
@@ -421,27 +409,6 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
                             }
                         }
                     }
-                }
-            }
-            if (FUSE == FUSE_FINALIZE) {
-                const Tank tk = tank_of(sc, bd);
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    // A node with w = 0 keeps its velocity: read it only for
-                    // pairs that hold one, to write the pair back whole.
-                    V3 vel[2] = { mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f) };
-                    if (w[2 * k] == 0.0f || w[2 * k + 1] == 0.0f) {
-                        const F2 vx = ld2(n.vx, off[k]), vy = ld2(n.vy, off[k]), vz = ld2(n.vz, off[k]);
-                        vel[0] = mk(vx.x, vy.x, vz.x);
-                        vel[1] = mk(vx.y, vy.y, vz.y);
-                    }
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        if (w[2 * k + h] != 0.0f) finalize_node(p[2 * k + h], q[2 * k + h], vel[h], tk);
-                    }
-                    st2(n.vx, off[k], vel[0].x, vel[1].x);
-                    st2(n.vy, off[k], vel[0].y, vel[1].y);
-                    st2(n.vz, off[k], vel[0].z, vel[1].z);
                 }
             }
 #pragma unroll
@@ -852,40 +819,27 @@ __global__ __launch_bounds__(BLOCK) void k_splats(const float *__restrict__ px, 
         return hipGetLastError();                                                                              \
     } while (0)
 
-hipError_t launch_predict(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, TileView t,
-                          const uint8_t *cover, uint32_t skip)
+hipError_t launch_predict(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, TileView t)
 {
-    VSIM_DISPATCH_BLOCK(k_predict, sh, s, n, scenes, t, cover, skip);
+    VSIM_DISPATCH_BLOCK(k_predict, sh, s, n, scenes, t);
 }
 
 hipError_t launch_finalize(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies, const vs_scene_params *scenes,
-                           TileView t, const uint8_t *cover, uint32_t skip)
+                           TileView t)
 {
-    VSIM_DISPATCH_BLOCK(k_finalize, sh, s, n, bodies, scenes, t, cover, skip);
+    VSIM_DISPATCH_BLOCK(k_finalize, sh, s, n, bodies, scenes, t);
 }
 
-namespace {
-template <int FUSE>
-hipError_t elastic_blocks(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies, const vs_scene_params *scenes,
-                          const uint32_t *cells, TileView t, uint32_t limit_only)
+hipError_t launch_finalize_predict(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
+                                  const vs_scene_params *scenes, TileView t)
 {
-    switch (sh.block) {
-    case 64: k_elastic<64, FUSE><<<sh.grid, 64, 0, s>>>(n, bodies, scenes, cells, t, limit_only, sh.xcd); break;
-    case 128: k_elastic<128, FUSE><<<sh.grid, 128, 0, s>>>(n, bodies, scenes, cells, t, limit_only, sh.xcd); break;
-    case 256: k_elastic<256, FUSE><<<sh.grid, 256, 0, s>>>(n, bodies, scenes, cells, t, limit_only, sh.xcd); break;
-    case 512: k_elastic<512, FUSE><<<sh.grid, 512, 0, s>>>(n, bodies, scenes, cells, t, limit_only, sh.xcd); break;
-    default: return hipErrorInvalidConfiguration;
-    }
-    return hipGetLastError();
+    VSIM_DISPATCH_BLOCK(k_finalize_predict, sh, s, n, bodies, scenes, t);
 }
-}  // namespace
 
 hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies, const vs_scene_params *scenes,
-                          const uint32_t *cells, TileView t, uint32_t limit_only, int fuse)
+                          const uint32_t *cells, TileView t, uint32_t limit_only)
 {
-    if (fuse == FUSE_PREDICT) return elastic_blocks<FUSE_PREDICT>(s, sh, n, bodies, scenes, cells, t, limit_only);
-    if (fuse == FUSE_FINALIZE) return elastic_blocks<FUSE_FINALIZE>(s, sh, n, bodies, scenes, cells, t, limit_only);
-    return elastic_blocks<FUSE_NONE>(s, sh, n, bodies, scenes, cells, t, limit_only);
+    VSIM_DISPATCH_BLOCK(k_elastic, sh, s, n, bodies, scenes, cells, t, limit_only, sh.xcd);
 }
 
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,

@@ -151,24 +151,6 @@ __global__ __launch_bounds__(TILE) void k_lattice_nodes(LatticeView L, NodeView 
         n.vy[g] = L.velocity[1];
         n.vz[g] = L.velocity[2];
         const_cast<float *>(n.w)[g] = L.wtab[incident(L.occ, cd, i, j, k)];
-        // Which fused colour passes own the node: the one incident cell of
-        // colour 0 (all parities even) or 7 (all odd) exists and is occupied.
-        uint8_t cover = 0;
-        const uint32_t nijk[3] = { i, j, k };
-#pragma unroll
-        for (uint32_t c = 0; c < 8; c += 7) {
-            uint32_t ci[3];
-            bool inside = true;
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                const uint32_t par = (c >> a) & 1u;
-                const uint32_t back = (nijk[a] ^ par) & 1u;
-                inside = inside && nijk[a] >= back && nijk[a] - back < cd[a];
-                ci[a] = nijk[a] - back;
-            }
-            if (inside && L.occ[ci[0] + cd[0] * (ci[1] + cd[1] * ci[2])] != 0) cover |= c == 0 ? COVER_FIRST : COVER_LAST;
-        }
-        L.cover[g] = cover;
     }
 }
 

@@ -106,9 +106,6 @@ struct SceneRec {
     uint64_t drive_begin = 0, drive_count = 0;
     uint64_t splat_begin = 0, splat_count = 0;
     bool contact = false, stretch = false;
-    // A compliant drive moves predicted positions, so predict cannot move
-    // into the first colour pass while one runs.
-    bool compliant() const { return drive_count && params.drive_law != 0; }
     std::vector<BodyRec> bodies;
     std::vector<uint32_t> splat_perm;  // stored splat -> caller's splat index
 };
@@ -143,7 +140,6 @@ struct vs_world {
     uint32_t *d_grid = nullptr;
     uint32_t *d_drive = nullptr;
     float *d_splat_in = nullptr, *d_splat_out = nullptr;
-    uint8_t *d_cover = nullptr;  // per node: COVER_FIRST | COVER_LAST
     int32_t *d_pairlog = nullptr;
     uint32_t pair_cap = 0;
 
@@ -161,7 +157,7 @@ struct vs_world {
     std::vector<vs_body> h_bodies;
 
     std::vector<SceneRec> recs;
-    bool tables_dirty = true, params_dirty = true, any_stretch = false, any_contact = false, any_compliant = false;
+    bool tables_dirty = true, params_dirty = true, any_stretch = false, any_contact = false;
     std::vector<std::vector<char>> pending;  // host copies kept alive until uploaded
 
     // Captured frame.
@@ -266,7 +262,6 @@ int rebuild_tables(vs_world *w)
     const uint32_t S = w->cfg.max_scenes;
     w->any_stretch = false;
     w->any_contact = false;
-    w->any_compliant = false;
     for (int p = 0; p < NPASS; p++) {
         vs_tile *tiles = reinterpret_cast<vs_tile *>(w->h_tiles[p].p + 4);
         uint64_t n = 0;
@@ -303,7 +298,6 @@ int rebuild_tables(vs_world *w)
                 if (!ok) return fail(w, VS_E_NOMEM, "tile table overflow");
                 w->any_stretch |= r.stretch;
                 w->any_contact |= r.contact;
-                w->any_compliant |= r.compliant();
             }
             w->scene_ntiles[p][s] = (uint32_t)(n - start);
             w->h_scene_counts.p[(size_t)p * S + s] = (uint32_t)(n - start);
@@ -386,18 +380,10 @@ struct Profile {
     }
 };
 
-// Predict moves into colour 0 and finalize into colour 7 when nothing runs
-// between them and those passes: no tensile sweeps after colour 7, no
-// contact before finalize, no compliant drive after predict.
-bool fused(const vs_world *w, bool stretch, bool contact, bool compliant)
-{
-    return w->cfg.fuse && !stretch && !contact && !compliant;
-}
-
 // Enqueues one launch of a pass over the whole table (scene < 0) or one
 // scene's slice: `color` picks the elastic colour, `limit_only` the
 // stretch-only sweeps. A pass with no tiles enqueues nothing.
-int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_only, int fuse = 0)
+int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_only)
 {
     using namespace vsim;
     auto shape = [&](int p) { return Shape{ grid_for(w, p, scene), pass_block(w, p), scene < 0 ? w->cfg.xcd_chunked : 0u }; };
@@ -407,10 +393,7 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
     switch (kind) {
     case VS_P_PREDICT:
         sh = shape(P_NODE);
-        // Fused: only the nodes the first colour pass does not predict.
-        if (sh.grid)
-            HIPW(w, launch_predict(st, sh, n, w->d_scenes, tile_view(w, P_NODE, scene), fuse ? w->d_cover : nullptr,
-                                   vsim::COVER_FIRST));
+        if (sh.grid) HIPW(w, launch_predict(st, sh, n, w->d_scenes, tile_view(w, P_NODE, scene)));
         return VS_OK;
     case VS_P_DRIVE:
         sh = shape(P_DRIVE);
@@ -420,14 +403,16 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
         sh = shape(P_CELL + color);
         if (sh.grid)
             HIPW(w, launch_elastic(st, sh, n, w->d_bodies, w->d_scenes, w->d_cells, tile_view(w, P_CELL + color, scene),
-                                   limit_only, fuse));
+                                   limit_only));
         return VS_OK;
     case VS_P_FINALIZE:
         sh = shape(P_NODE);
-        // Fused: only the nodes the last colour pass does not finalize.
+        if (sh.grid) HIPW(w, launch_finalize(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_NODE, scene)));
+        return VS_OK;
+    case VS_P_FINALIZE_PREDICT:
+        sh = shape(P_NODE);
         if (sh.grid)
-            HIPW(w, launch_finalize(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_NODE, scene),
-                                    fuse ? w->d_cover : nullptr, vsim::COVER_LAST));
+            HIPW(w, launch_finalize_predict(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_NODE, scene)));
         return VS_OK;
     case VS_P_SPLAT:
         sh = shape(P_SPLAT);
@@ -463,48 +448,46 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
 int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Profile *prof = nullptr)
 {
     if (splats) w->splat_mode = splats;
-    bool stretch = w->any_stretch, contact = w->any_contact, compliant = w->any_compliant;
+    bool stretch = w->any_stretch, contact = w->any_contact;
     if (scene >= 0) {
         stretch = w->recs[scene].stretch;
         contact = w->recs[scene].contact;
-        compliant = w->recs[scene].compliant();
     }
-    const bool fuse = fused(w, stretch, contact, compliant);
+    const bool merge = !w->cfg.separate_node_passes;
     hipStream_t st = w->stream;
     // With a profile, events bracket every launch of a non-empty pass.
-    auto pass = [&](int kind, int color, uint32_t limit_only, int f) -> int {
+    auto pass = [&](int kind, int color, uint32_t limit_only) -> int {
         const int p = kind == VS_P_ELASTIC ? P_CELL + color
                       : kind == VS_P_DRIVE ? P_DRIVE
                       : kind == VS_P_SPLAT ? P_SPLAT
-                      : (kind == VS_P_PREDICT || kind == VS_P_FINALIZE) ? P_NODE
-                                                                         : P_SURF;
+                      : (kind == VS_P_PREDICT || kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_NODE
+                                                                                                         : P_SURF;
         const bool timed = prof && grid_for(w, p, scene) != 0;
         if (timed) {
             prof->marks.emplace_back(kind, prof->used);
             HIPW(w, hipEventRecord(prof->next(), st));
         }
-        if (int rc = enqueue_pass(w, kind, color, scene, limit_only, f)) return rc;
+        if (int rc = enqueue_pass(w, kind, color, scene, limit_only)) return rc;
         if (timed) HIPW(w, hipEventRecord(prof->next(), st));
         return VS_OK;
     };
-    // Fused frames fold predict into colour 0 and finalize into colour 7
-    // (each node still predicted and finalized once); node passes cover the
-    // nodes no occupied cell of those colours touches.
+    // Nothing runs between one substep's finalize and the next one's
+    // predict, so inside a frame the two share one node pass.
     for (uint32_t s = 0; s < substeps; s++) {
-        if (int rc = pass(VS_P_PREDICT, 0, 0, fuse)) return rc;
-        if (int rc = pass(VS_P_DRIVE, 0, 0, 0)) return rc;
+        if (s == 0 || !merge)
+            if (int rc = pass(VS_P_PREDICT, 0, 0)) return rc;
+        if (int rc = pass(VS_P_DRIVE, 0, 0)) return rc;
         for (int sweep = 0; sweep < (stretch ? 4 : 1); sweep++)
-            for (int c = 0; c < 8; c++) {
-                const int f = !fuse ? 0 : c == 0 ? 1 : c == 7 ? 2 : 0;
-                if (int rc = pass(VS_P_ELASTIC, c, sweep > 0 ? 1u : 0u, f)) return rc;
-            }
+            for (int c = 0; c < 8; c++)
+                if (int rc = pass(VS_P_ELASTIC, c, sweep > 0 ? 1u : 0u)) return rc;
         if (contact)
             for (int kind : { VS_P_BIN, VS_P_SORT, VS_P_COLLIDE, VS_P_SETTLE })
-                if (int rc = pass(kind, 0, 0, 0)) return rc;
-        if (int rc = pass(VS_P_FINALIZE, 0, 0, fuse)) return rc;
+                if (int rc = pass(kind, 0, 0)) return rc;
+        const bool last = s + 1 == substeps;
+        if (int rc = pass(merge && !last ? VS_P_FINALIZE_PREDICT : VS_P_FINALIZE, 0, 0)) return rc;
     }
     if (splats)
-        if (int rc = pass(VS_P_SPLAT, 0, 0, 0)) return rc;
+        if (int rc = pass(VS_P_SPLAT, 0, 0)) return rc;
     return VS_OK;
 }
 
@@ -631,7 +614,6 @@ static int world_init(vs_world *w)
     ALLOC(W->d_drive, sizeof(uint32_t) * DRIVE_PLANES * c.max_driven);
     ALLOC(W->d_splat_in, sizeof(float) * 10 * c.max_splats);
     ALLOC(W->d_splat_out, sizeof(float) * SPLAT_OUT_PLANES * c.max_splats);
-    ALLOC(W->d_cover, c.max_nodes);
     ALLOC(W->d_scene_counts, sizeof(uint32_t) * NPASS * c.max_scenes);
     HIPW(W, hipMemset(W->d_nodes, 0, sizeof(float) * NODE_PLANES * c.max_nodes));
     HIPW(W, hipMemset(W->d_grid, 0, sizeof(uint32_t) * c.max_buckets));
@@ -698,8 +680,7 @@ void vs_world_destroy(vs_world *w)
     for (hipEvent_t ev : { w->upload_done, w->tables_done, w->params_done, w->tune[0], w->tune[1] })
         if (ev) (void)hipEventDestroy(ev);
     void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf,
-                     w->d_grid,  w->d_drive, w->d_splat_in, w->d_splat_out, w->d_pairlog, w->d_scene_counts,
-                     w->d_cover };
+                     w->d_grid,  w->d_drive, w->d_splat_in, w->d_splat_out, w->d_pairlog, w->d_scene_counts };
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int p = 0; p < NPASS; p++)
@@ -923,7 +904,6 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
         if (pl.device) {
             vsim::LatticeView L = lattice_view(bd, pl, node_at, r.body_begin + b);
             L.cells = w->d_cells + cell_at;
-            L.cover = w->d_cover;
             if (surface) {
                 L.s_node = w->d_surf + surf_at;
                 L.s_body = w->d_surf + S + surf_at;
@@ -970,26 +950,6 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
             }
             const float *hw = keep(w, std::vector<float>(bd.inv_mass, bd.inv_mass + nn));
             HIPW(w, hipMemcpyAsync(w->d_nodes + 9 * N + node_at, hw, nn * 4, hipMemcpyHostToDevice, up));
-            // Cover bytes: corners of the occupied cells of colour 0 and 7.
-            std::vector<uint8_t> cover(nn, 0);
-            const uint32_t ndx = nd[0], ndxy = nd[0] * nd[1];
-            uint64_t first = 0;
-            for (int c = 0; c < 8; c++) {
-                if (c == 0 || c == 7) {
-                    const uint8_t bit = c == 0 ? vsim::COVER_FIRST : vsim::COVER_LAST;
-                    for (uint64_t i = first; i < first + bd.color_count[c]; i++) {
-                        const uint32_t lin = bd.cells[i];
-                        const uint32_t cx = lin % bd.cell_dims[0], cy = (lin / bd.cell_dims[0]) % bd.cell_dims[1],
-                                       cz = lin / (bd.cell_dims[0] * bd.cell_dims[1]);
-                        const uint64_t n0 = cx + (uint64_t)ndx * cy + (uint64_t)ndxy * cz;
-                        for (int k = 0; k < 8; k++)
-                            cover[n0 + (k & 1) + (k & 2 ? ndx : 0) + (k & 4 ? ndxy : 0)] |= bit;
-                    }
-                }
-                first += bd.color_count[c];
-            }
-            const uint8_t *hcov = keep(w, std::move(cover));
-            HIPW(w, hipMemcpyAsync(w->d_cover + node_at, hcov, nn, hipMemcpyHostToDevice, up));
             if (bd.n_cells) {
                 const uint32_t *hc = keep(w, std::vector<uint32_t>(bd.cells, bd.cells + bd.n_cells));
                 HIPW(w, hipMemcpyAsync(w->d_cells + cell_at, hc, (size_t)bd.n_cells * 4, hipMemcpyHostToDevice, up));
@@ -1136,7 +1096,8 @@ int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph
     const uint32_t mode = splat_mode(splats);
     if (!graph) return enqueue_frame(w, substeps, mode, -1);
     std::vector<uint32_t> key = { substeps, mode, (uint32_t)w->any_stretch, (uint32_t)w->any_contact,
-                                  (uint32_t)w->any_compliant, w->cfg.fuse, w->pair_cap, w->block_cells,
+                                  w->cfg.separate_node_passes, w->pair_cap,
+                                  w->block_cells,
                                   w->cfg.xcd_chunked };
     for (int p = 0; p < NPASS; p++) key.push_back(grid_for(w, p, -1));
     if (!w->exec || key != w->graph_key) {
@@ -1364,8 +1325,8 @@ int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, ui
     const int p = kind == VS_P_ELASTIC ? P_CELL
                   : kind == VS_P_DRIVE ? P_DRIVE
                   : kind == VS_P_SPLAT ? P_SPLAT
-                  : (kind == VS_P_PREDICT || kind == VS_P_FINALIZE) ? P_NODE
-                                                                     : P_SURF;
+                  : (kind == VS_P_PREDICT || kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_NODE
+                                                                                                     : P_SURF;
     uint32_t per_round = 0;
     for (int c = 0; c < (kind == VS_P_ELASTIC ? 8 : 1); c++) per_round += grid_for(w, p + c, -1) != 0;
     *ms = 0.f;
@@ -1409,7 +1370,7 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
     HIPW(w, hipMalloc(&snap, bytes));
     HIPW(w, hipMemcpyAsync(snap, w->d_nodes, bytes, hipMemcpyDeviceToDevice, w->stream));
     struct Cand {
-        uint32_t block, tile, persistent, xcd, fuse;
+        uint32_t block, tile, persistent, xcd;
         float ms;
     };
     std::vector<Cand> cands;
@@ -1418,15 +1379,14 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
             for (uint32_t persistent : { 0u, 2048u })
                 for (uint32_t xcd : { 0u, 1u }) {
                     if (xcd && !persistent) continue;
-                    cands.push_back(Cand{ block, block * mult, persistent, xcd, 0u, 0.f });
+                    cands.push_back(Cand{ block, block * mult, persistent, xcd, 0.f });
                 }
-    const Cand saved{ w->block_cells, w->tile_cells, w->cfg.persistent, w->cfg.xcd_chunked, w->cfg.fuse, 0.f };
+    const Cand saved{ w->block_cells, w->tile_cells, w->cfg.persistent, w->cfg.xcd_chunked, 0.f };
     auto apply = [&](const Cand &c) {
         w->block_cells = c.block;
         w->tile_cells = c.tile;
         w->cfg.persistent = c.persistent;
         w->cfg.xcd_chunked = c.xcd;
-        w->cfg.fuse = c.fuse;
         w->tables_dirty = true;
     };
     auto time = [&](Cand &c) -> int {
@@ -1445,17 +1405,6 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
     int rc = VS_OK;
     for (Cand &c : cands)
         if ((rc = time(c))) break;
-    if (!rc) {
-        // The fastest launch shape, once more with predict and finalize folded
-        // into the first and last colour passes.
-        const Cand *best = &cands[0];
-        for (const Cand &c : cands)
-            if (c.ms < best->ms) best = &c;
-        Cand fused = *best;
-        fused.fuse = 1;
-        cands.push_back(fused);
-        rc = time(cands.back());
-    }
     HIPW(w, hipMemcpyAsync(w->d_nodes, snap, bytes, hipMemcpyDeviceToDevice, w->stream));
     HIPW(w, hipStreamSynchronize(w->stream));
     HIPW(w, hipFree(snap));
@@ -1470,8 +1419,8 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
     if (report && report_bytes) {
         auto one = [](const Cand &c) {
             char buf[192];
-            std::snprintf(buf, sizeof buf, "{\"block\":%u,\"tile\":%u,\"persistent\":%u,\"xcd\":%u,\"fuse\":%u,\"ms\":%.4f}",
-                          c.block, c.tile, c.persistent, c.xcd, c.fuse, c.ms);
+            std::snprintf(buf, sizeof buf, "{\"block\":%u,\"tile\":%u,\"persistent\":%u,\"xcd\":%u,\"ms\":%.4f}",
+                          c.block, c.tile, c.persistent, c.xcd, c.ms);
             return std::string(buf);
         };
         std::string out = "{\"candidates\":[";
