@@ -212,6 +212,9 @@ def main():
     ap.add_argument("--separate-node-passes", action="store_true",
                     help="run each substep's finalize and the next one's predict as two node passes (same "
                          "results; the default merges them inside a frame)")
+    ap.add_argument("--separate-colours", action="store_true",
+                    help="one launch per colour for every body (same results; the default solves the colours "
+                         "of dense lattices in pairs, by rows of cells, one launch per pair)")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="scene requests of the serving leg (Poisson 50/s, 4k-500k nodes, 120 frames); 0 skips it")
     ap.add_argument("--serve-budget-nodes", type=float, default=6e6,
@@ -262,7 +265,8 @@ def main():
     log(f"[bench] {len(scs)} scenes: {nodes} nodes, {cells} cells, {splats} splats (generated in "
         f"{time.perf_counter() - t_gen:.1f} s)")
 
-    world = make_world(scs, separate_node_passes=int(args.separate_node_passes))
+    world = make_world(scs, separate_node_passes=int(args.separate_node_passes),
+                       separate_colours=int(args.separate_colours))
     ids = []
     for sc in scs:
         ids.append(world.add(sc))
@@ -299,7 +303,10 @@ def main():
     # world's stream, HIP events only at the two ends (state restored after)
     el_ms, el_n = world.time_pass("elastic", substeps)
     el_avg_s = el_ms / el_n / 1e3
-    el_bytes = cells * ELASTIC_BYTES_PER_CELL / 8
+    # One sweep is 8 colour launches, or 4 colour-pair launches (rows of a
+    # dense lattice read no cell list: 320 B a cell and colour).
+    per_sweep = el_n // substeps
+    el_bytes = cells * (ELASTIC_BYTES_PER_CELL if per_sweep == 8 else ELASTIC_BYTES_PER_CELL - 4) / per_sweep
     achieved = el_bytes / el_avg_s / 1e9
     log(f"[bench] colour pass: {el_n} launches, {el_avg_s * 1e6:.2f} us avg, {achieved:.0f} GB/s algorithmic")
     # ---- the splat pass alone, covariance output and with the engine format

@@ -226,6 +226,10 @@ typedef struct vs_world_config {
                                next substep's predict as two node passes; 0 (the
                                default) runs them as one pass inside a frame
                                (same arithmetic, same results)                 */
+    uint32_t separate_colours; /* 1: one launch per colour for every body; 0
+                               (the default) runs the colours of a dense body in
+                               pairs (2c, 2c+1) over whole x-rows of cells, one
+                               launch per pair (same arithmetic, same results) */
 } vs_world_config;
 
 /* One body of a scene, in the reference's host layout: the dense node grid

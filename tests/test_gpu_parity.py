@@ -280,3 +280,73 @@ def test_merged_node_passes_equal_separate_and_oracle(case):
         print(f"{case}: exact vs oracle {np.mean(out[0][0] == o.p):.6f}")
         assert rel_err(out[0][0], o.p) <= RTOL
         assert rel_err(out[0][1], o.v) <= RTOL
+
+
+def _pair_case(case):
+    if case == "c1":
+        return [scenes.c1_scene(splats=0)]
+    if case == "odd_block":
+        return [scenes.jelly_block((5, 6, 7), seed=3, name="odd")]
+    if case == "thin":
+        # one cell along x (a row of one lane), odd and even y/z counts
+        return [scenes.jelly_block((1, 3, 2), seed=4, name="thin"), scenes.jelly_block((2, 1, 5), seed=5, name="thin2")]
+    if case == "wide_rows":
+        # rows of 36 and 66 lanes: the second does not fit a 64-lane
+        # workgroup and runs per colour beside the first
+        return [scenes.jelly_block((71, 3, 4), h=0.004, seed=6, name="w71"),
+                scenes.jelly_block((130, 2, 3), h=0.002, seed=7, name="w130")]
+    if case == "contact":
+        return [scenes.contact_stack(count=3, side=4)]
+    if case == "stretch":
+        sc = scenes.jelly_block((6, 5, 7), seed=8, name="st")
+        sc.bodies[0].material.max_stretch = 1.05
+        return [sc]
+    if case == "hard_drive":
+        sc = scenes.jelly_block(10, seed=5, pinned=False, tank_margin=0.3, name="hd")
+        sc.drive = scenes.handle_drive(sc, seed=6, law=0)
+        return [sc]
+    if case == "mixed":
+        # dense blocks (pairs) and a sparse lattice (per colour) in one batch
+        return [scenes.jelly_block((9, 8, 7), seed=9, name="m1"), _sparse_scene(21),
+                scenes.jelly_block(12, seed=10, name="m2")]
+    raise ValueError(case)
+
+
+@pytest.mark.parametrize("block", [64, 256])
+@pytest.mark.parametrize("case", ["c1", "odd_block", "thin", "wide_rows", "contact", "stretch", "hard_drive", "mixed"])
+def test_colour_pairs_equal_separate_colours_and_oracle(case, block):
+    # Colours 2c and 2c+1 of a dense lattice solved by rows in one launch:
+    # the same frames, bit for bit, as one launch per colour, batched, from a
+    # graph and plain; and within tolerance of the oracle.
+    from oracle.oracle import OracleScene
+    scs = _pair_case(case)
+    print(f"case {case} block {block}: dims {[b.lattice.cell_dims for s in scs for b in s.bodies]}", flush=True)
+    results = {}
+    for separate in (1, 0):
+        for plain in (False, True):
+            w = world_for(*scs, separate_colours=separate, block_cells=block)
+            sids = [w.add(s) for s in scs]
+            for f in range(2):
+                for s, sid in zip(scs, sids):
+                    if s.drive is not None:
+                        w.set_target(sid, *s.drive.frame_target(f, s.substeps * s.dt))
+                if plain:
+                    for s, sid in zip(scs, sids):
+                        w.frame_scene(sid, s.substeps, False)
+                else:
+                    w.frame(scs[0].substeps, False, f == 1)
+            results[(separate, plain)] = [w.read_nodes(sid) for sid in sids]
+            w.close()
+    ref = results[(1, False)]
+    for key, got in results.items():
+        for (p, v), (rp, rv) in zip(got, ref):
+            assert np.array_equal(p, rp) and np.array_equal(v, rv), key
+    for s, (p, v) in zip(scs, results[(0, False)]):
+        if s.drive is not None:
+            continue
+        o = OracleScene(s, "f32")
+        for f in range(2):
+            o.frame(s.substeps)
+        print(f"{case}/{s.name}: exact vs oracle {np.mean(p == o.p):.6f}")
+        assert rel_err(p, o.p) <= RTOL
+        assert rel_err(v, o.v) <= RTOL

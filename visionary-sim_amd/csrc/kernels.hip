@@ -339,6 +339,56 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_predict(NodeView n, const vs
 }
 
 // ------------------------------------------------------------- elastic
+// The constants a cell projection reads, per body and dt (project.wgsl:35-60,
+// xpbdCouple.wgsl:58-71).
+struct CellConsts {
+    float rs, gamma, aD, aH, relax, h, maxStretch;
+};
+
+__device__ __forceinline__ CellConsts cell_consts(const vs_body &bd, float dt)
+{
+    const float invDt2 = 1.0f / (dt * dt);
+    return CellConsts{ 1.0f / bd.h, bd.gamma, bd.alpha_dev * invDt2, bd.alpha_hyd * invDt2, bd.relaxation / dt, bd.h,
+                       bd.max_stretch };
+}
+
+// One cell: the six Kuhn tetrahedra in order, then the tensile limit
+// (project.wgsl:162-179) when maxStretch > 1. `limit_only` skips the tets
+// (the extra stretch sweeps of elasticCells.js:197-214).
+__device__ __forceinline__ void solve_cell(V3 (&p)[8], const V3 (&q)[8], const float (&w)[8], const CellConsts &k,
+                                           uint32_t limit_only)
+{
+    if (!limit_only) {
+        project_tet<0>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<1>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<2>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<3>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<4>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<5>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
+    }
+    if (k.maxStretch > 1.0f) {
+        for (int sweep = 0; sweep < 2; sweep++) {
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+#pragma unroll
+                for (int b = a + 1; b < 8; b++) {
+                    const V3 delta = sub(p[b], p[a]);
+                    const float len = sqrtf(dot(delta, delta));
+                    const float maximum = (edge_len(a, b) * k.h) * k.maxStretch;
+                    const float weight = w[a] + w[b];
+                    if (len > maximum && weight > 0.0f) {
+                        const V3 corr = mul(delta, (len - maximum) / (len * weight));
+                        p[a] = add(p[a], mul(corr, w[a]));
+                        p[b] = sub(p[b], mul(corr, w[b]));
+                    }
+                }
+            }
+        }
+    }
+}
+
+// One colour pass over a list of cells: one lane per cell, its eight corners
+// gathered as four x-pairs per plane, solved, and scattered back.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__restrict__ bodies,
                                                    const vs_scene_params *__restrict__ scenes,
@@ -350,14 +400,8 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
         const vs_tile tile = tv.tiles[t];
         const vs_body bd = bodies[tile.body];
         const vs_scene_params &sc = scenes[bd.scene];
-        const float dt = sc.dt;
-        const float maxStretch = bd.max_stretch;
-        if (limit_only && !(maxStretch > 1.0f)) continue;
-        const float invDt2 = 1.0f / (dt * dt);
-        const float aD = bd.alpha_dev * invDt2;
-        const float aH = bd.alpha_hyd * invDt2;
-        const float relax = bd.relaxation / dt;
-        const float rs = 1.0f / bd.h;
+        if (limit_only && !(bd.max_stretch > 1.0f)) continue;
+        const CellConsts kc = cell_consts(bd, sc.dt);
         const uint32_t cdx = bd.cell_dims[0], cdy = bd.cell_dims[1];
         const uint32_t ndx = bd.node_dims[0], ndxy = bd.node_dims[0] * bd.node_dims[1];
 
@@ -383,34 +427,7 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
                 q[2 * k] = mk(u.x, v.x, s.x);
                 q[2 * k + 1] = mk(u.y, v.y, s.y);
             }
-            if (!limit_only) {
-                project_tet<0>(p, q, w, rs, bd.gamma, aD, aH, relax);
-                project_tet<1>(p, q, w, rs, bd.gamma, aD, aH, relax);
-                project_tet<2>(p, q, w, rs, bd.gamma, aD, aH, relax);
-                project_tet<3>(p, q, w, rs, bd.gamma, aD, aH, relax);
-                project_tet<4>(p, q, w, rs, bd.gamma, aD, aH, relax);
-                project_tet<5>(p, q, w, rs, bd.gamma, aD, aH, relax);
-            }
-            if (maxStretch > 1.0f) {
-                // project.wgsl:162-179
-                for (int sweep = 0; sweep < 2; sweep++) {
-#pragma unroll
-                    for (int a = 0; a < 8; a++) {
-#pragma unroll
-                        for (int b = a + 1; b < 8; b++) {
-                            const V3 delta = sub(p[b], p[a]);
-                            const float len = sqrtf(dot(delta, delta));
-                            const float maximum = (edge_len(a, b) * bd.h) * maxStretch;
-                            const float weight = w[a] + w[b];
-                            if (len > maximum && weight > 0.0f) {
-                                const V3 corr = mul(delta, (len - maximum) / (len * weight));
-                                p[a] = add(p[a], mul(corr, w[a]));
-                                p[b] = sub(p[b], mul(corr, w[b]));
-                            }
-                        }
-                    }
-                }
-            }
+            solve_cell(p, q, w, kc, limit_only);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 st2(n.px, off[k], p[2 * k].x, p[2 * k + 1].x);
@@ -418,6 +435,148 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
                 st2(n.pz, off[k], p[2 * k].z, p[2 * k + 1].z);
             }
         }
+    }
+}
+
+// Two colours in one pass, over whole x-rows of cells of a dense lattice.
+//
+// Colours 2c and 2c+1 (c = 0..3) are the cells with y parity c&1 and z parity
+// c>>1, even x then odd x: the cells of one such row y, z. Cells of one colour
+// share no corner, and a row's odd cell shares its corners only with the two
+// even cells beside it in the same row, so solving every row's even cells and
+// then its odd cells is exactly the two colour passes in order.
+//
+// Lane m of a row holds the node pair x = 2m, 2m+1 of the row's four node
+// rows (y or y+1, z or z+1): an 8-byte access, the row's lanes together one
+// contiguous span per node row. It solves cell 2m (colour 2c), hands node 2m
+// to lane m-1 through LDS, takes node 2m+2 from lane m+1, solves cell 2m+1
+// (colour 2c+1) and hands node 2m+2 back. Every node of the lattice is read
+// and written once per pass instead of once per colour. Rows never span
+// workgroups: a tile is `count` whole rows starting at row `begin` of the
+// class, rows of L = cdx/2 + 1 lanes laid end to end.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_elastic_rows(NodeView n, const vs_body *__restrict__ bodies,
+                                                        const vs_scene_params *__restrict__ scenes, TileView tv,
+                                                        uint32_t cls, uint32_t limit_only, uint32_t xcd)
+{
+    // Per lane: start-of-substep position (12) and inverse mass (4) of its
+    // node 2m in the four node rows, then that node's position (12).
+    __shared__ float lds[28 * BLOCK];
+    const uint32_t py = cls & 1u, pz = cls >> 1;
+    const uint32_t tid = threadIdx.x;
+    const Range r = tile_range(*tv.count, xcd);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_body bd = bodies[tile.body];
+        const vs_scene_params &sc = scenes[bd.scene];
+        if (limit_only && !(bd.max_stretch > 1.0f)) continue;
+        const CellConsts kc = cell_consts(bd, sc.dt);
+        const uint32_t cdx = bd.cell_dims[0];
+        const uint32_t ndx = bd.node_dims[0], ndxy = bd.node_dims[0] * bd.node_dims[1];
+        const uint32_t L = cdx / 2u + 1u;
+        const uint32_t nyr = (bd.cell_dims[1] + 1u - py) / 2u;
+        const uint32_t row = tid / L, m = tid - row * L;
+        const bool on = row < tile.count;
+        const uint32_t x0 = 2u * m;
+        const bool pair = x0 + 1u <= cdx;      // node x0 + 1 is in the row
+        const bool cellA = on && x0 < cdx;     // cell x0, colour 2c
+        const bool cellB = on && x0 + 1u < cdx;  // cell x0 + 1, colour 2c + 1
+        uint32_t off[4] = { 0u, 0u, 0u, 0u };
+        V3 p[8], q[8];
+        float w[8];
+        if (on) {
+            const uint32_t rho = tile.begin + row;
+            const uint32_t y = py + 2u * (rho % nyr), z = pz + 2u * (rho / nyr);
+            const uint32_t n0 = bd.node_offset + x0 + ndx * y + ndxy * z;
+#pragma unroll
+            for (int k = 0; k < 4; k++) off[k] = 4u * (n0 + (k & 1 ? ndx : 0u) + (k & 2 ? ndxy : 0u));
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (on && pair) {
+                const F2 x = ld2(n.px, off[k]), y = ld2(n.py, off[k]), z = ld2(n.pz, off[k]);
+                const F2 mm = ld2(n.w, off[k]);
+                const F2 u = ld2(n.qx, off[k]), v = ld2(n.qy, off[k]), s = ld2(n.qz, off[k]);
+                p[2 * k] = mk(x.x, y.x, z.x);
+                p[2 * k + 1] = mk(x.y, y.y, z.y);
+                w[2 * k] = mm.x;
+                w[2 * k + 1] = mm.y;
+                q[2 * k] = mk(u.x, v.x, s.x);
+                q[2 * k + 1] = mk(u.y, v.y, s.y);
+            } else if (on) {
+                const uint32_t e = off[k] / 4u;
+                p[2 * k] = ld3(n.px, n.py, n.pz, e);
+                q[2 * k] = ld3(n.qx, n.qy, n.qz, e);
+                w[2 * k] = n.w[e];
+                p[2 * k + 1] = q[2 * k + 1] = mk(0.f, 0.f, 0.f);
+                w[2 * k + 1] = 0.f;
+            } else {
+                p[2 * k] = q[2 * k] = p[2 * k + 1] = q[2 * k + 1] = mk(0.f, 0.f, 0.f);
+                w[2 * k] = w[2 * k + 1] = 0.f;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            lds[(3 * k + 0) * BLOCK + tid] = q[2 * k].x;
+            lds[(3 * k + 1) * BLOCK + tid] = q[2 * k].y;
+            lds[(3 * k + 2) * BLOCK + tid] = q[2 * k].z;
+            lds[(12 + k) * BLOCK + tid] = w[2 * k];
+        }
+        if (cellA) solve_cell(p, q, w, kc, limit_only);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            lds[(16 + 3 * k + 0) * BLOCK + tid] = p[2 * k].x;
+            lds[(16 + 3 * k + 1) * BLOCK + tid] = p[2 * k].y;
+            lds[(16 + 3 * k + 2) * BLOCK + tid] = p[2 * k].z;
+        }
+        __syncthreads();
+        // Cell x0 + 1: its low corners are this lane's node x0 + 1, its high
+        // corners lane m+1's node x0 + 2 (same row whenever the cell exists).
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            p[2 * k] = p[2 * k + 1];
+            q[2 * k] = q[2 * k + 1];
+            w[2 * k] = w[2 * k + 1];
+        }
+        if (cellB) {
+            const uint32_t nb = tid + 1u;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                q[2 * k + 1] = mk(lds[(3 * k + 0) * BLOCK + nb], lds[(3 * k + 1) * BLOCK + nb], lds[(3 * k + 2) * BLOCK + nb]);
+                w[2 * k + 1] = lds[(12 + k) * BLOCK + nb];
+                p[2 * k + 1] = mk(lds[(16 + 3 * k + 0) * BLOCK + nb], lds[(16 + 3 * k + 1) * BLOCK + nb],
+                                  lds[(16 + 3 * k + 2) * BLOCK + nb]);
+            }
+            solve_cell(p, q, w, kc, limit_only);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                lds[(16 + 3 * k + 0) * BLOCK + nb] = p[2 * k + 1].x;
+                lds[(16 + 3 * k + 1) * BLOCK + nb] = p[2 * k + 1].y;
+                lds[(16 + 3 * k + 2) * BLOCK + nb] = p[2 * k + 1].z;
+            }
+        }
+        __syncthreads();
+        // Node x0 is final in this lane's slot: lane m-1 wrote it back after
+        // cell x0 - 1, or, at the row's start, this lane's own cell x0 did.
+        if (on) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float lx = lds[(16 + 3 * k + 0) * BLOCK + tid];
+                const float ly = lds[(16 + 3 * k + 1) * BLOCK + tid];
+                const float lz = lds[(16 + 3 * k + 2) * BLOCK + tid];
+                if (pair) {
+                    st2(n.px, off[k], lx, p[2 * k].x);
+                    st2(n.py, off[k], ly, p[2 * k].y);
+                    st2(n.pz, off[k], lz, p[2 * k].z);
+                } else {
+                    const uint32_t e = off[k] / 4u;
+                    n.px[e] = lx;
+                    n.py[e] = ly;
+                    n.pz[e] = lz;
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -840,6 +999,12 @@ hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bo
                           const uint32_t *cells, TileView t, uint32_t limit_only)
 {
     VSIM_DISPATCH_BLOCK(k_elastic, sh, s, n, bodies, scenes, cells, t, limit_only, sh.xcd);
+}
+
+hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
+                               const vs_scene_params *scenes, TileView t, uint32_t cls, uint32_t limit_only)
+{
+    VSIM_DISPATCH_BLOCK(k_elastic_rows, sh, s, n, bodies, scenes, t, cls, limit_only, sh.xcd);
 }
 
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,

@@ -33,7 +33,7 @@ using vsim::SurfaceView;
 using vsim::TileView;
 
 constexpr uint32_t BUCKET_SLOTS = 32;  // contacts.wgsl:75, bodyContact.js:35
-constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, NPASS = 12;
+constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, P_ROW = 12, NPASS = 16;
 constexpr int NODE_PLANES = 13;        // p xyz, prev xyz, vel xyz, w, rest xyz
 constexpr int SURF_PLANES = 9;
 constexpr int DRIVE_PLANES = 4;
@@ -93,6 +93,8 @@ struct BodyRec {
     uint64_t color_begin[8] = {}, color_count[8] = {};
     uint64_t splat_begin = 0, splat_count = 0;
     float damping = 0;
+    uint32_t cell_dims[3] = {};
+    bool dense = false;  // every cell of the lattice box occupied
 };
 
 struct SceneRec {
@@ -240,7 +242,7 @@ SplatView splat_view(vs_world *w)
 uint32_t pass_block(const vs_world *w, int p)
 {
     if (p == P_NODE) return w->block_node;
-    if (p >= P_CELL && p < P_CELL + 8) return w->block_cells;
+    if ((p >= P_CELL && p < P_CELL + 8) || p >= P_ROW) return w->block_cells;
     if (p == P_SURF) return w->block_surf;
     if (p == P_DRIVE) return w->block_drive;
     return w->block_splat;
@@ -254,6 +256,17 @@ uint32_t pass_tile(const vs_world *w, int p)
     if (p == P_DRIVE) return w->tile_drive;
     return w->tile_splat;
 }
+
+// A dense body whose rows fit one workgroup runs its colours in pairs, by
+// rows (k_elastic_rows); any other body runs one colour per pass.
+bool row_body(const vs_world *w, const BodyRec &b)
+{
+    return !w->cfg.separate_colours && b.dense && b.cell_dims[0] / 2u + 1u <= w->block_cells;
+}
+
+// The tile table of an elastic launch: colours 0..7, then the four row
+// classes (colours 2c and 2c+1) as 8..11.
+int elastic_table(int color) { return color < 8 ? P_CELL + color : P_ROW + (color - 8); }
 
 // Rebuilds every tile table from the admitted scenes, in scene-id order.
 int rebuild_tables(vs_world *w)
@@ -286,7 +299,28 @@ int rebuild_tables(vs_world *w)
                 } else if (p < P_CELL + 8) {
                     const int c = p - P_CELL;
                     for (size_t b = 0; b < r.bodies.size(); b++)
-                        ok &= emit(r.bodies[b].color_begin[c], r.bodies[b].color_count[c], (uint32_t)(r.body_begin + b));
+                        if (!row_body(w, r.bodies[b]))
+                            ok &= emit(r.bodies[b].color_begin[c], r.bodies[b].color_count[c],
+                                       (uint32_t)(r.body_begin + b));
+                } else if (p >= P_ROW) {
+                    // Whole rows of the class, as many as fit one workgroup.
+                    const uint32_t cls = (uint32_t)(p - P_ROW);
+                    for (size_t b = 0; b < r.bodies.size() && ok; b++) {
+                        const BodyRec &br = r.bodies[b];
+                        if (!row_body(w, br)) continue;
+                        const uint64_t ny = (br.cell_dims[1] + 1u - (cls & 1u)) / 2u;
+                        const uint64_t nz = (br.cell_dims[2] + 1u - (cls >> 1)) / 2u;
+                        const uint64_t per = w->block_cells / (br.cell_dims[0] / 2u + 1u);
+                        const uint64_t rows = ny * nz;
+                        for (uint64_t o = 0; o < rows; o += per) {
+                            if (n >= w->tile_cap[p]) {
+                                ok = false;
+                                break;
+                            }
+                            tiles[n++] = vs_tile{ (uint32_t)o, (uint32_t)std::min<uint64_t>(per, rows - o),
+                                                  (uint32_t)(r.body_begin + b), s };
+                        }
+                    }
                 } else if (p == P_SURF) {
                     if (r.contact) ok &= emit(r.surf_begin, r.surf_count, (uint32_t)r.body_begin);
                 } else if (p == P_DRIVE) {
@@ -400,8 +434,12 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
         if (sh.grid) HIPW(w, launch_drive(st, sh, n, w->d_scenes, drive_view(w), tile_view(w, P_DRIVE, scene), -1));
         return VS_OK;
     case VS_P_ELASTIC:
-        sh = shape(P_CELL + color);
-        if (sh.grid)
+        sh = shape(elastic_table(color));
+        if (!sh.grid) return VS_OK;
+        if (color >= 8)
+            HIPW(w, launch_elastic_rows(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, elastic_table(color), scene),
+                                        (uint32_t)(color - 8), limit_only));
+        else
             HIPW(w, launch_elastic(st, sh, n, w->d_bodies, w->d_scenes, w->d_cells, tile_view(w, P_CELL + color, scene),
                                    limit_only));
         return VS_OK;
@@ -457,7 +495,7 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
     hipStream_t st = w->stream;
     // With a profile, events bracket every launch of a non-empty pass.
     auto pass = [&](int kind, int color, uint32_t limit_only) -> int {
-        const int p = kind == VS_P_ELASTIC ? P_CELL + color
+        const int p = kind == VS_P_ELASTIC ? elastic_table(color)
                       : kind == VS_P_DRIVE ? P_DRIVE
                       : kind == VS_P_SPLAT ? P_SPLAT
                       : (kind == VS_P_PREDICT || kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_NODE
@@ -477,9 +515,12 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
         if (s == 0 || !merge)
             if (int rc = pass(VS_P_PREDICT, 0, 0)) return rc;
         if (int rc = pass(VS_P_DRIVE, 0, 0)) return rc;
+        // Colours in order; row class c (colours 2c, 2c+1 of the row bodies)
+        // goes with the two colour passes of the other bodies.
         for (int sweep = 0; sweep < (stretch ? 4 : 1); sweep++)
-            for (int c = 0; c < 8; c++)
-                if (int rc = pass(VS_P_ELASTIC, c, sweep > 0 ? 1u : 0u)) return rc;
+            for (int c = 0; c < 4; c++)
+                for (int e : { 8 + c, 2 * c, 2 * c + 1 })
+                    if (int rc = pass(VS_P_ELASTIC, e, sweep > 0 ? 1u : 0u)) return rc;
         if (contact)
             for (int kind : { VS_P_BIN, VS_P_SORT, VS_P_COLLIDE, VS_P_SETTLE })
                 if (int rc = pass(kind, 0, 0)) return rc;
@@ -634,6 +675,8 @@ static int world_init(vs_world *w)
     // Colour populations are uneven for small or odd lattices: size every
     // colour table for all cells (tiles are 16 bytes, cheap to over-provision).
     for (int k = 0; k < 8; k++) W->tile_cap[P_CELL + k] = c.max_cells / 64 + c.max_bodies + 64;
+    // A full row tile holds at least block/2 >= 32 cells of its class.
+    for (int k = 0; k < 4; k++) W->tile_cap[P_ROW + k] = c.max_cells / 32 + c.max_bodies + 64;
     for (int p = 0; p < NPASS; p++) {
         const size_t bytes = 16 + W->tile_cap[p] * sizeof(vs_tile);
         ALLOC(W->d_tiles[p], bytes);
@@ -882,6 +925,8 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
         br.node_begin = node_at;
         br.node_count = nn;
         br.damping = bd.damping;
+        for (int a = 0; a < 3; a++) br.cell_dims[a] = bd.cell_dims[a];
+        br.dense = (uint64_t)pl.n_cells == (uint64_t)bd.cell_dims[0] * bd.cell_dims[1] * bd.cell_dims[2];
         vs_body &vb = w->h_bodies[r.body_begin + b];
         vb = vs_body{};
         for (int a = 0; a < 3; a++) {
@@ -1098,7 +1143,7 @@ int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph
     std::vector<uint32_t> key = { substeps, mode, (uint32_t)w->any_stretch, (uint32_t)w->any_contact,
                                   w->cfg.separate_node_passes, w->pair_cap,
                                   w->block_cells,
-                                  w->cfg.xcd_chunked };
+                                  w->cfg.xcd_chunked, w->cfg.separate_colours };
     for (int p = 0; p < NPASS; p++) key.push_back(grid_for(w, p, -1));
     if (!w->exec || key != w->graph_key) {
         if (w->exec) HIPW(w, hipGraphExecDestroy(w->exec));
@@ -1327,8 +1372,10 @@ int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, ui
                   : kind == VS_P_SPLAT ? P_SPLAT
                   : (kind == VS_P_PREDICT || kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_NODE
                                                                                                      : P_SURF;
+    const int kinds = kind == VS_P_ELASTIC ? 12 : 1;  // elastic: colours 0..7 and row classes 8..11
+    auto table = [&](int c) { return kind == VS_P_ELASTIC ? elastic_table(c) : p; };
     uint32_t per_round = 0;
-    for (int c = 0; c < (kind == VS_P_ELASTIC ? 8 : 1); c++) per_round += grid_for(w, p + c, -1) != 0;
+    for (int c = 0; c < kinds; c++) per_round += grid_for(w, table(c), -1) != 0;
     *ms = 0.f;
     *launches = 0;
     if (per_round == 0) return VS_OK;
@@ -1342,7 +1389,7 @@ int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, ui
         HIPW(w, hipEventRecord(a, w->stream));
         int e = VS_OK;
         for (uint32_t r = 0; r < rounds && !e; r++)
-            for (int c = 0; c < (kind == VS_P_ELASTIC ? 8 : 1) && !e; c++) e = enqueue_pass(w, kind, c, -1, 0);
+            for (int c = 0; c < kinds && !e; c++) e = enqueue_pass(w, kind, c, -1, 0);
         HIPW(w, hipEventRecord(b, w->stream));
         // Restored whether or not every launch went in.
         HIPW(w, hipMemcpyAsync(w->d_nodes, snap, bytes, hipMemcpyDeviceToDevice, w->stream));
@@ -1370,23 +1417,28 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
     HIPW(w, hipMalloc(&snap, bytes));
     HIPW(w, hipMemcpyAsync(snap, w->d_nodes, bytes, hipMemcpyDeviceToDevice, w->stream));
     struct Cand {
-        uint32_t block, tile, persistent, xcd;
+        uint32_t block, tile, persistent, xcd, colours;
         float ms;
     };
+    // Launch shapes for one colour per pass; colour pairs by rows take a
+    // whole number of rows per workgroup, so only the block size, the grid
+    // and the tile order vary for them.
     std::vector<Cand> cands;
-    for (uint32_t block : { 64u, 128u, 256u, 512u })
-        for (uint32_t mult : { 1u, 2u, 4u })
-            for (uint32_t persistent : { 0u, 2048u })
-                for (uint32_t xcd : { 0u, 1u }) {
-                    if (xcd && !persistent) continue;
-                    cands.push_back(Cand{ block, block * mult, persistent, xcd, 0.f });
-                }
-    const Cand saved{ w->block_cells, w->tile_cells, w->cfg.persistent, w->cfg.xcd_chunked, 0.f };
+    for (uint32_t colours : { 0u, 1u })
+        for (uint32_t block : { 64u, 128u, 256u, 512u })
+            for (uint32_t mult : { 1u, 2u, 4u })
+                for (uint32_t persistent : { 0u, 2048u })
+                    for (uint32_t xcd : { 0u, 1u }) {
+                        if ((xcd && !persistent) || (!colours && mult > 1)) continue;
+                        cands.push_back(Cand{ block, block * mult, persistent, xcd, colours, 0.f });
+                    }
+    const Cand saved{ w->block_cells, w->tile_cells, w->cfg.persistent, w->cfg.xcd_chunked, w->cfg.separate_colours, 0.f };
     auto apply = [&](const Cand &c) {
         w->block_cells = c.block;
         w->tile_cells = c.tile;
         w->cfg.persistent = c.persistent;
         w->cfg.xcd_chunked = c.xcd;
+        w->cfg.separate_colours = c.colours;
         w->tables_dirty = true;
     };
     auto time = [&](Cand &c) -> int {
@@ -1418,9 +1470,10 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
     apply(*best);
     if (report && report_bytes) {
         auto one = [](const Cand &c) {
-            char buf[192];
-            std::snprintf(buf, sizeof buf, "{\"block\":%u,\"tile\":%u,\"persistent\":%u,\"xcd\":%u,\"ms\":%.4f}",
-                          c.block, c.tile, c.persistent, c.xcd, c.ms);
+            char buf[224];
+            std::snprintf(buf, sizeof buf,
+                          "{\"block\":%u,\"tile\":%u,\"persistent\":%u,\"xcd\":%u,\"separate_colours\":%u,\"ms\":%.4f}",
+                          c.block, c.tile, c.persistent, c.xcd, c.colours, c.ms);
             return std::string(buf);
         };
         std::string out = "{\"candidates\":[";
