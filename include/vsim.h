@@ -81,8 +81,13 @@ typedef struct vs_scene_params {
     uint32_t surf_base;    /* first surface record of this scene             */
     uint32_t surf_count;
     uint32_t bucket_slots; /* 32, BUCKET_SLOTS of contacts.wgsl:75           */
-    uint32_t flags;
+    uint32_t flags;        /* VS_SCENE_*                                     */
 } vs_scene_params;
+
+/* A scene whose finalize follows its last colour directly (no contact and
+ * no tensile-limit sweeps between them): the last colour-pair pass may
+ * finalize, and predict the next substep, for the nodes it writes. */
+#define VS_SCENE_FOLD 1u
 
 /* One unit of work for a launch: `count` consecutive items from `begin` in the
  * pass's item array (nodes, cells, surface records, driven records, splats),

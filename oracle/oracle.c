@@ -235,77 +235,91 @@ static void project_tet(v3 *p, const v3 *prev, const real *w, const int *tet, m3
 }
 
 /* The same projection in the evaluation order the GPU kernel uses. The
- * mathematics is project.wgsl's: because every Kuhn tetrahedron is an axis
- * path, F = Ds * Dm^-1 has the path's edge differences times 1/h as its
- * columns (axis by axis), and (dC/dF) * B^T gives each path corner the
- * difference of the per-axis gradients of the steps into and out of it
- * (d0 = -(d1 + d2 + d3) holds by construction). Products feeding a sum are
- * fused (one rounding), and the 2x2 block divides once. */
+ * mathematics is project.wgsl's, rewritten so that it needs no square root
+ * and no 3x3 scaling:
+ *
+ * - Every Kuhn tetrahedron is an axis path, so F = Ds * Dm^-1 = s * E, with
+ *   E the path's edge differences (axis by axis) and s = 1/h; the cofactor
+ *   columns are H = s^2 * K, K = (E1 x E2, E2 x E0, E0 x E1), and det F =
+ *   s^3 * det E.
+ * - (dC/dF) * B^T gives each path corner the difference of the per-axis
+ *   gradients into and out of it: d_i = c * e_i with c = s / |E| and e_i
+ *   the edge differences of E, h_i = s^3 * f_i with f_i those of K
+ *   (d0 = -(d1 + d2 + d3) holds by construction).
+ * - With u = c * lambda_D and v = s^3 * lambda_H the 2x2 block of
+ *   xpbdCouple.wgsl:58-71 becomes
+ *       [c^2 Ge st + aD,  c^2 Gef st ] [u]   [-(s^2 + r c^2 Re)      ]
+ *       [s^6 Gef st,      s^6 Gf st + aH] [v] = [-s^3 (C_H + r s^3 Rf)]
+ *   (G.. = sum w e.e, e.f, f.f; R. = sum e.x, f.x; st = 1 + r), whose
+ *   determinant is the original one, and the corner update is
+ *   w_i (e_i u + f_i v), the correction formed before it meets the position
+ *   (one rounding at the position's scale). Only c^2 = s^2 / |E|^2 enters,
+ *   and c * C_D = s^2.
+ *   A tetrahedron with |F| <= 1e-9 has no deviatoric gradient: c^2 = 0,
+ *   and then the first row and the coupling are taken as 0.
+ * Products feeding a sum are fused (one rounding), and the block divides
+ * once. */
 static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real s, real gamma, real aD, real aH,
                               real relax)
 {
     const int *tv = KUHN_TETS[t];
     real wt[4] = { w[tv[0]], w[tv[1]], w[tv[2]], w[tv[3]] };
     if (((wt[0] + wt[1]) + wt[2]) + wt[3] == (real)0) return;
-    v3 F[3];
-    for (int j = 0; j < 3; j++) F[AXIS[t][j]] = scale(sub(p[PATH[t][j + 1]], p[PATH[t][j]]), s);
-    real n2 = F[0].x * F[0].x;
-    n2 = FMA(F[0].y, F[0].y, n2);
-    n2 = FMA(F[0].z, F[0].z, n2);
+    v3 E[3];
+    for (int j = 0; j < 3; j++) E[AXIS[t][j]] = sub(p[PATH[t][j + 1]], p[PATH[t][j]]);
+    real e2 = E[0].x * E[0].x;
+    e2 = FMA(E[0].y, E[0].y, e2);
+    e2 = FMA(E[0].z, E[0].z, e2);
     for (int c = 1; c < 3; c++) {
-        n2 = FMA(F[c].x, F[c].x, n2);
-        n2 = FMA(F[c].y, F[c].y, n2);
-        n2 = FMA(F[c].z, F[c].z, n2);
+        e2 = FMA(E[c].x, E[c].x, e2);
+        e2 = FMA(E[c].y, E[c].y, e2);
+        e2 = FMA(E[c].z, E[c].z, e2);
     }
-    real cDev = SQRT(n2);
-    v3 H[3] = { crossf(F[1], F[2]), crossf(F[2], F[0]), crossf(F[0], F[1]) };
-    real cHyd = dotf(F[0], H[0]) - gamma;
-    real k = cDev > (real)(float)1e-9 ? (real)1 / cDev : (real)0;
-    real ks = k * s;
-    v3 gD[3], gH[3];
-    for (int a = 0; a < 3; a++) {
-        gD[a] = scale(F[a], ks);
-        gH[a] = scale(H[a], s);
-    }
-    v3 d[4], h[4], x[4];
+    v3 K[3] = { crossf(E[1], E[2]), crossf(E[2], E[0]), crossf(E[0], E[1]) };
+    real s2 = s * s, s3 = s2 * s, s6 = s3 * s3;
+    real cHyd = FMA(dotf(E[0], K[0]), s3, -gamma);
+    int dev = s2 * e2 > (real)(float)1e-18;
+    real c2 = dev ? s2 / e2 : (real)0;
+    v3 e[4], f[4], x[4];
     for (int i = 0; i < 4; i++) {
         int j = POS[t][i];
         if (j == 0) {
-            d[i] = neg(gD[AXIS[t][0]]);
-            h[i] = neg(gH[AXIS[t][0]]);
+            e[i] = neg(E[AXIS[t][0]]);
+            f[i] = neg(K[AXIS[t][0]]);
         } else if (j == 3) {
-            d[i] = gD[AXIS[t][2]];
-            h[i] = gH[AXIS[t][2]];
+            e[i] = E[AXIS[t][2]];
+            f[i] = K[AXIS[t][2]];
         } else {
-            d[i] = sub(gD[AXIS[t][j - 1]], gD[AXIS[t][j]]);
-            h[i] = sub(gH[AXIS[t][j - 1]], gH[AXIS[t][j]]);
+            e[i] = sub(E[AXIS[t][j - 1]], E[AXIS[t][j]]);
+            f[i] = sub(K[AXIS[t][j - 1]], K[AXIS[t][j]]);
         }
         x[i] = sub(p[tv[i]], q[tv[i]]);
     }
-    real gx = 0, gy = 0, gz = 0, rD = 0, rH = 0;
+    real ge = 0, gef = 0, gf = 0, re = 0, rf = 0;
     for (int i = 0; i < 4; i++) {
-        gx = FMA(wt[i], dotf(d[i], d[i]), gx);
-        gy = FMA(wt[i], dotf(d[i], h[i]), gy);
-        gz = FMA(wt[i], dotf(h[i], h[i]), gz);
-        rD = dotacc(d[i], x[i], rD);
-        rH = dotacc(h[i], x[i], rH);
+        ge = FMA(wt[i], dotf(e[i], e[i]), ge);
+        gef = FMA(wt[i], dotf(e[i], f[i]), gef);
+        gf = FMA(wt[i], dotf(f[i], f[i]), gf);
+        re = dotacc(e[i], x[i], re);
+        rf = dotacc(f[i], x[i], rf);
     }
     real st = (real)1 + relax;
-    real a00 = FMA(gx, st, aD);
-    real a01 = gy * st;
-    real a11 = FMA(gz, st, aH);
-    real det = FMA(a00, a11, -(a01 * a01));
+    real A = FMA(c2 * ge, st, aD);
+    real B = (c2 * gef) * st;
+    real C = dev ? (s6 * gef) * st : (real)0;
+    real D = FMA(s6 * gf, st, aH);
+    real det = FMA(A, D, -(B * C));
     if (det < (real)(float)1e-20) return;
-    real bD = -FMA(rD, relax, cDev);
-    real bH = -FMA(rH, relax, cHyd);
+    real P = dev ? -FMA(relax, c2 * re, s2) : (real)0;
+    real Q = -(s3 * FMA(relax, s3 * rf, cHyd));
     real inv = (real)1 / det;
-    real lD = FMA(a11, bD, -(a01 * bH)) * inv;
-    real lH = FMA(a00, bH, -(a01 * bD)) * inv;
+    real u = FMA(D, P, -(B * Q)) * inv;
+    real v = FMA(A, Q, -(C * P)) * inv;
     for (int i = 0; i < 4; i++) {
-        v3 u = scale(d[i], lD);
-        u = mk(FMA(h[i].x, lH, u.x), FMA(h[i].y, lH, u.y), FMA(h[i].z, lH, u.z));
+        v3 g = scale(e[i], u);
+        g = mk(FMA(f[i].x, v, g.x), FMA(f[i].y, v, g.y), FMA(f[i].z, v, g.z));
         v3 *pi = &p[tv[i]];
-        *pi = mk(FMA(u.x, wt[i], pi->x), FMA(u.y, wt[i], pi->y), FMA(u.z, wt[i], pi->z));
+        *pi = mk(FMA(g.x, wt[i], pi->x), FMA(g.y, wt[i], pi->y), FMA(g.z, wt[i], pi->z));
     }
 }
 

@@ -309,11 +309,21 @@ def _pair_case(case):
         # dense blocks (pairs) and a sparse lattice (per colour) in one batch
         return [scenes.jelly_block((9, 8, 7), seed=9, name="m1"), _sparse_scene(21),
                 scenes.jelly_block(12, seed=10, name="m2")]
+    if case == "mixed_fold":
+        # scenes whose last colour pair finalizes and predicts its nodes
+        # beside scenes with contact or tensile sweeps after the colours
+        stack = scenes.contact_stack(count=2, side=5)
+        stack.substeps = 10
+        st = scenes.jelly_block((7, 4, 6), seed=12, name="st2")
+        st.bodies[0].material.max_stretch = 1.02
+        return [scenes.jelly_block((9, 8, 7), seed=11, name="f1"), stack, st,
+                scenes.jelly_block((4, 9, 3), seed=13, pinned=False, name="f2")]
     raise ValueError(case)
 
 
 @pytest.mark.parametrize("block", [64, 256])
-@pytest.mark.parametrize("case", ["c1", "odd_block", "thin", "wide_rows", "contact", "stretch", "hard_drive", "mixed"])
+@pytest.mark.parametrize("case", ["c1", "odd_block", "thin", "wide_rows", "contact", "stretch", "hard_drive", "mixed",
+                                  "mixed_fold"])
 def test_colour_pairs_equal_separate_colours_and_oracle(case, block):
     # Colours 2c and 2c+1 of a dense lattice solved by rows in one launch:
     # the same frames, bit for bit, as one launch per colour, batched, from a

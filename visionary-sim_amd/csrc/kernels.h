@@ -103,9 +103,12 @@ hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bo
                           const vs_scene_params *scenes, const uint32_t *cells, TileView t, uint32_t limit_only);
 // Colours 2*cls and 2*cls + 1 over whole x-rows of dense lattices; a tile is
 // `count` rows of the class from row `begin` (row = a + ny*b for y = (cls&1)
-// + 2a, z = (cls>>1) + 2b, ny = the class's rows along y).
+// + 2a, z = (cls>>1) + 2b, ny = the class's rows along y). `node_mode` 1
+// finalizes and predicts, 2 finalizes, the nodes written, in scenes flagged
+// VS_SCENE_FOLD.
 hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
-                               const vs_scene_params *scenes, TileView t, uint32_t cls, uint32_t limit_only);
+                               const vs_scene_params *scenes, TileView t, uint32_t cls, uint32_t limit_only,
+                               uint32_t node_mode);
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes,
                               SurfaceView sv, GridView g, TileView t);
 hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv,

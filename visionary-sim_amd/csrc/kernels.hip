@@ -74,81 +74,83 @@ __device__ __forceinline__ V3 crossf(V3 a, V3 b)
 }
 
 // One tetrahedron of project.wgsl:75-153 (the stable Neo-Hookean pair solved
-// as the 2x2 block of xpbdCouple.wgsl:58-71), in the fused evaluation order
-// the oracle's project_tet_fused defines.
+// as the 2x2 block of xpbdCouple.wgsl:58-71), in the evaluation order the
+// oracle's project_tet_fused defines. With E the path's edge differences
+// (F = s E, s = 1/h), K = cof(E) (cof F = s^2 K), c = s/|E| and the scaled
+// multipliers u = c lambda_D, v = s^3 lambda_H, the block is
+//     [c^2 Ge st + aD,  c^2 Gef st    ] [u]   [-(s^2 + r c^2 Re)     ]
+//     [s^6 Gef st,      s^6 Gf st + aH] [v] = [-s^3 (C_H + r s^3 Rf)]
+// over the corner differences e of E and f of K (G.. weighted Gram sums,
+// R. rates, st = 1 + r), and corner i moves by w_i (e_i u + f_i v). Same
+// mathematics and determinant as the WGSL form; no square root, and no 3x3
+// scaling of F or of the gradients.
 template <int T>
-__device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const float (&w)[8], float s,
-                                            float gamma, float aD, float aH, float relax)
+__device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const float (&w)[8], float s2, float s3,
+                                            float s6, float gamma, float aD, float aH, float relax)
 {
     constexpr Tet tet = TETS[T];
     const float wt[4] = { w[tet.v[0]], w[tet.v[1]], w[tet.v[2]], w[tet.v[3]] };
     if (((wt[0] + wt[1]) + wt[2]) + wt[3] == 0.0f) return;
 
-    V3 F[3];
+    V3 E[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) F[tet.axis[j]] = mul(sub(p[tet.path[j + 1]], p[tet.path[j]]), s);
-    float n2 = F[0].x * F[0].x;
-    n2 = fmaf(F[0].y, F[0].y, n2);
-    n2 = fmaf(F[0].z, F[0].z, n2);
+    for (int j = 0; j < 3; j++) E[tet.axis[j]] = sub(p[tet.path[j + 1]], p[tet.path[j]]);
+    float e2 = E[0].x * E[0].x;
+    e2 = fmaf(E[0].y, E[0].y, e2);
+    e2 = fmaf(E[0].z, E[0].z, e2);
 #pragma unroll
     for (int c = 1; c < 3; c++) {
-        n2 = fmaf(F[c].x, F[c].x, n2);
-        n2 = fmaf(F[c].y, F[c].y, n2);
-        n2 = fmaf(F[c].z, F[c].z, n2);
+        e2 = fmaf(E[c].x, E[c].x, e2);
+        e2 = fmaf(E[c].y, E[c].y, e2);
+        e2 = fmaf(E[c].z, E[c].z, e2);
     }
-    const float cDev = sqrtf(n2);
-    const V3 H[3] = { crossf(F[1], F[2]), crossf(F[2], F[0]), crossf(F[0], F[1]) };
-    const float cHyd = dotf(F[0], H[0]) - gamma;
-    const float k = cDev > 1e-9f ? 1.0f / cDev : 0.0f;
-    const float ks = k * s;
-    V3 gD[3], gH[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        gD[a] = mul(F[a], ks);
-        gH[a] = mul(H[a], s);
-    }
-    V3 d[4], h[4], x[4];
+    const V3 K[3] = { crossf(E[1], E[2]), crossf(E[2], E[0]), crossf(E[0], E[1]) };
+    const float cHyd = fmaf(dotf(E[0], K[0]), s3, -gamma);
+    const bool dev = s2 * e2 > 1e-18f;
+    const float c2 = dev ? s2 / e2 : 0.0f;
+    V3 e[4], f[4], x[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int j = tet.pos[i];
         if (j == 0) {
-            d[i] = neg(gD[tet.axis[0]]);
-            h[i] = neg(gH[tet.axis[0]]);
+            e[i] = neg(E[tet.axis[0]]);
+            f[i] = neg(K[tet.axis[0]]);
         } else if (j == 3) {
-            d[i] = gD[tet.axis[2]];
-            h[i] = gH[tet.axis[2]];
+            e[i] = E[tet.axis[2]];
+            f[i] = K[tet.axis[2]];
         } else {
-            d[i] = sub(gD[tet.axis[j - 1]], gD[tet.axis[j]]);
-            h[i] = sub(gH[tet.axis[j - 1]], gH[tet.axis[j]]);
+            e[i] = sub(E[tet.axis[j - 1]], E[tet.axis[j]]);
+            f[i] = sub(K[tet.axis[j - 1]], K[tet.axis[j]]);
         }
         x[i] = sub(p[tet.v[i]], q[tet.v[i]]);
     }
-    float gx = 0.f, gy = 0.f, gz = 0.f, rD = 0.f, rH = 0.f;
+    float ge = 0.f, gef = 0.f, gf = 0.f, re = 0.f, rf = 0.f;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        gx = fmaf(wt[i], dotf(d[i], d[i]), gx);
-        gy = fmaf(wt[i], dotf(d[i], h[i]), gy);
-        gz = fmaf(wt[i], dotf(h[i], h[i]), gz);
-        rD = dotacc(d[i], x[i], rD);
-        rH = dotacc(h[i], x[i], rH);
+        ge = fmaf(wt[i], dotf(e[i], e[i]), ge);
+        gef = fmaf(wt[i], dotf(e[i], f[i]), gef);
+        gf = fmaf(wt[i], dotf(f[i], f[i]), gf);
+        re = dotacc(e[i], x[i], re);
+        rf = dotacc(f[i], x[i], rf);
     }
     const float st = 1.0f + relax;
-    const float a00 = fmaf(gx, st, aD);
-    const float a01 = gy * st;
-    const float a11 = fmaf(gz, st, aH);
-    const float det = fmaf(a00, a11, -(a01 * a01));
+    const float A = fmaf(c2 * ge, st, aD);
+    const float B = (c2 * gef) * st;
+    const float C = dev ? (s6 * gef) * st : 0.0f;
+    const float D = fmaf(s6 * gf, st, aH);
+    const float det = fmaf(A, D, -(B * C));
     if (det < 1e-20f) return;
-    const float bD = -fmaf(rD, relax, cDev);
-    const float bH = -fmaf(rH, relax, cHyd);
+    const float P = dev ? -fmaf(relax, c2 * re, s2) : 0.0f;
+    const float Q = -(s3 * fmaf(relax, s3 * rf, cHyd));
     const float inv = 1.0f / det;
-    const float lD = fmaf(a11, bD, -(a01 * bH)) * inv;
-    const float lH = fmaf(a00, bH, -(a01 * bD)) * inv;
+    const float u = fmaf(D, P, -(B * Q)) * inv;
+    const float v = fmaf(A, Q, -(C * P)) * inv;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        V3 u = mul(d[i], lD);
-        u = mk(fmaf(h[i].x, lH, u.x), fmaf(h[i].y, lH, u.y), fmaf(h[i].z, lH, u.z));
+        V3 g = mul(e[i], u);
+        g = mk(fmaf(f[i].x, v, g.x), fmaf(f[i].y, v, g.y), fmaf(f[i].z, v, g.z));
         V3 &pi = p[tet.v[i]];
-        pi = mk(fmaf(u.x, wt[i], pi.x), fmaf(u.y, wt[i], pi.y), fmaf(u.z, wt[i], pi.z));
+        pi = mk(fmaf(g.x, wt[i], pi.x), fmaf(g.y, wt[i], pi.y), fmaf(g.z, wt[i], pi.z));
     }
 }
 
@@ -308,7 +310,10 @@ __global__ __launch_bounds__(BLOCK) void k_finalize(NodeView n, const vs_body *_
 
 // The last finalize of one substep and the predict of the next, per node in
 // one pass (nothing runs between them in Stepper.sequence): the node passes'
-// arithmetic in their order, 64 B a node instead of 52 + 64.
+// arithmetic in their order. The velocity is not stored: every pass up to the
+// next finalize, which overwrites it, reads positions only (predict reads it
+// only at a frame's start, after a plain finalize). 52 B a node instead of
+// 52 + 64.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_finalize_predict(NodeView n, const vs_body *__restrict__ bodies,
                                                             const vs_scene_params *__restrict__ scenes, TileView tv)
@@ -331,9 +336,6 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_predict(NodeView n, const vs
             n.px[k] = p.x;
             n.py[k] = p.y;
             n.pz[k] = p.z;
-            n.vx[k] = v.x;
-            n.vy[k] = v.y;
-            n.vz[k] = v.z;
         }
     }
 }
@@ -342,14 +344,15 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_predict(NodeView n, const vs
 // The constants a cell projection reads, per body and dt (project.wgsl:35-60,
 // xpbdCouple.wgsl:58-71).
 struct CellConsts {
-    float rs, gamma, aD, aH, relax, h, maxStretch;
+    float rs2, rs3, rs6, gamma, aD, aH, relax, h, maxStretch;  // rsN = (1/h)^N, rounded step by step
 };
 
 __device__ __forceinline__ CellConsts cell_consts(const vs_body &bd, float dt)
 {
     const float invDt2 = 1.0f / (dt * dt);
-    return CellConsts{ 1.0f / bd.h, bd.gamma, bd.alpha_dev * invDt2, bd.alpha_hyd * invDt2, bd.relaxation / dt, bd.h,
-                       bd.max_stretch };
+    const float s = 1.0f / bd.h, s2 = s * s, s3 = s2 * s;
+    return CellConsts{ s2, s3, s3 * s3, bd.gamma, bd.alpha_dev * invDt2, bd.alpha_hyd * invDt2,
+                       bd.relaxation / dt, bd.h, bd.max_stretch };
 }
 
 // One cell: the six Kuhn tetrahedra in order, then the tensile limit
@@ -359,12 +362,12 @@ __device__ __forceinline__ void solve_cell(V3 (&p)[8], const V3 (&q)[8], const f
                                            uint32_t limit_only)
 {
     if (!limit_only) {
-        project_tet<0>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
-        project_tet<1>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
-        project_tet<2>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
-        project_tet<3>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
-        project_tet<4>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
-        project_tet<5>(p, q, w, k.rs, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<0>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<1>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<2>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<3>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<4>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<5>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
     }
     if (k.maxStretch > 1.0f) {
         for (int sweep = 0; sweep < 2; sweep++) {
@@ -454,10 +457,15 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
 // and written once per pass instead of once per colour. Rows never span
 // workgroups: a tile is `count` whole rows starting at row `begin` of the
 // class, rows of L = cdx/2 + 1 lanes laid end to end.
+//
+// `node_mode` (class 3, the last colours, in a scene flagged VS_SCENE_FOLD):
+// after the colours, each node the lane writes is finalized (1, 2) and then
+// predicted for the next substep (1), as the node passes would do next.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_elastic_rows(NodeView n, const vs_body *__restrict__ bodies,
                                                         const vs_scene_params *__restrict__ scenes, TileView tv,
-                                                        uint32_t cls, uint32_t limit_only, uint32_t xcd)
+                                                        uint32_t cls, uint32_t limit_only, uint32_t node_mode,
+                                                        uint32_t xcd)
 {
     // Per lane: start-of-substep position (12) and inverse mass (4) of its
     // node 2m in the four node rows, then that node's position (12).
@@ -558,7 +566,9 @@ __global__ __launch_bounds__(BLOCK) void k_elastic_rows(NodeView n, const vs_bod
         __syncthreads();
         // Node x0 is final in this lane's slot: lane m-1 wrote it back after
         // cell x0 - 1, or, at the row's start, this lane's own cell x0 did.
-        if (on) {
+        // Node x0 + 1 is this lane's low corners.
+        const bool fold = node_mode != 0u && (sc.flags & VS_SCENE_FOLD) != 0u;
+        if (on && !fold) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const float lx = lds[(16 + 3 * k + 0) * BLOCK + tid];
@@ -573,6 +583,67 @@ __global__ __launch_bounds__(BLOCK) void k_elastic_rows(NodeView n, const vs_bod
                     n.px[e] = lx;
                     n.py[e] = ly;
                     n.pz[e] = lz;
+                }
+            }
+        } else if (on) {
+            // finalize.wgsl:44-74 and then predict.wgsl:19-36 per node with
+            // w != 0, as k_finalize_predict runs them; a node with w = 0
+            // keeps its prev and velocity (its prev is rewritten unchanged).
+            // Inside a frame the velocity is not stored: the next finalize
+            // overwrites it before anything reads it.
+            const Tank tk = tank_of(sc, bd);
+            const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                V3 lo = mk(lds[(16 + 3 * k + 0) * BLOCK + tid], lds[(16 + 3 * k + 1) * BLOCK + tid],
+                           lds[(16 + 3 * k + 2) * BLOCK + tid]);
+                V3 qlo = mk(lds[(3 * k + 0) * BLOCK + tid], lds[(3 * k + 1) * BLOCK + tid],
+                            lds[(3 * k + 2) * BLOCK + tid]);
+                const float wlo = lds[(12 + k) * BLOCK + tid];
+                V3 vlo = mk(0.f, 0.f, 0.f), vhi = mk(0.f, 0.f, 0.f);
+                V3 &hi = p[2 * k], &qhi = q[2 * k];
+                const bool mlo = wlo != 0.0f, mhi = pair && w[2 * k] != 0.0f;
+                if (mlo) {
+                    finalize_node(lo, qlo, vlo, tk);
+                    if (node_mode == 1u) predict_node(lo, qlo, vlo, gx, gy, gz, dt);
+                }
+                if (mhi) {
+                    finalize_node(hi, qhi, vhi, tk);
+                    if (node_mode == 1u) predict_node(hi, qhi, vhi, gx, gy, gz, dt);
+                }
+                const uint32_t e = off[k] / 4u;
+                if (pair) {
+                    st2(n.px, off[k], lo.x, hi.x);
+                    st2(n.py, off[k], lo.y, hi.y);
+                    st2(n.pz, off[k], lo.z, hi.z);
+                    if (node_mode == 1u) {
+                        st2(n.qx, off[k], qlo.x, qhi.x);
+                        st2(n.qy, off[k], qlo.y, qhi.y);
+                        st2(n.qz, off[k], qlo.z, qhi.z);
+                    }
+                    if (node_mode == 2u && mlo && mhi) {
+                        st2(n.vx, off[k], vlo.x, vhi.x);
+                        st2(n.vy, off[k], vlo.y, vhi.y);
+                        st2(n.vz, off[k], vlo.z, vhi.z);
+                    } else if (node_mode == 2u && mhi) {
+                        n.vx[e + 1] = vhi.x;
+                        n.vy[e + 1] = vhi.y;
+                        n.vz[e + 1] = vhi.z;
+                    }
+                } else {
+                    n.px[e] = lo.x;
+                    n.py[e] = lo.y;
+                    n.pz[e] = lo.z;
+                    if (node_mode == 1u) {
+                        n.qx[e] = qlo.x;
+                        n.qy[e] = qlo.y;
+                        n.qz[e] = qlo.z;
+                    }
+                }
+                if (node_mode == 2u && mlo && !(pair && mhi)) {
+                    n.vx[e] = vlo.x;
+                    n.vy[e] = vlo.y;
+                    n.vz[e] = vlo.z;
                 }
             }
         }
@@ -1002,9 +1073,10 @@ hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bo
 }
 
 hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
-                               const vs_scene_params *scenes, TileView t, uint32_t cls, uint32_t limit_only)
+                               const vs_scene_params *scenes, TileView t, uint32_t cls, uint32_t limit_only,
+                               uint32_t node_mode)
 {
-    VSIM_DISPATCH_BLOCK(k_elastic_rows, sh, s, n, bodies, scenes, t, cls, limit_only, sh.xcd);
+    VSIM_DISPATCH_BLOCK(k_elastic_rows, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
 }
 
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,

@@ -33,7 +33,7 @@ using vsim::SurfaceView;
 using vsim::TileView;
 
 constexpr uint32_t BUCKET_SLOTS = 32;  // contacts.wgsl:75, bodyContact.js:35
-constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, P_ROW = 12, NPASS = 16;
+constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, P_ROW = 12, P_REST = 16, NPASS = 17;
 constexpr int NODE_PLANES = 13;        // p xyz, prev xyz, vel xyz, w, rest xyz
 constexpr int SURF_PLANES = 9;
 constexpr int DRIVE_PLANES = 4;
@@ -170,6 +170,7 @@ struct vs_world {
 
     uint32_t splat_mode = VS_SPLAT_COV;  // outputs of the splat pass being enqueued
     uint32_t block_node = 256, block_cells = 256, block_surf = 256, block_drive = 64, block_splat = 256;
+    uint32_t block_rest = 256;  // the node pass of P_REST: 64 when its runs are short (folded scenes)
     uint32_t tile_nodes = 256, tile_cells = 256, tile_surf = 256, tile_drive = 64, tile_splat = 256;
 };
 
@@ -242,6 +243,7 @@ SplatView splat_view(vs_world *w)
 uint32_t pass_block(const vs_world *w, int p)
 {
     if (p == P_NODE) return w->block_node;
+    if (p == P_REST) return w->block_rest;
     if ((p >= P_CELL && p < P_CELL + 8) || p >= P_ROW) return w->block_cells;
     if (p == P_SURF) return w->block_surf;
     if (p == P_DRIVE) return w->block_drive;
@@ -250,7 +252,7 @@ uint32_t pass_block(const vs_world *w, int p)
 
 uint32_t pass_tile(const vs_world *w, int p)
 {
-    if (p == P_NODE) return w->tile_nodes;
+    if (p == P_NODE || p == P_REST) return w->tile_nodes;
     if (p >= P_CELL && p < P_CELL + 8) return w->tile_cells;
     if (p == P_SURF) return w->tile_surf;
     if (p == P_DRIVE) return w->tile_drive;
@@ -267,6 +269,48 @@ bool row_body(const vs_world *w, const BodyRec &b)
 // The tile table of an elastic launch: colours 0..7, then the four row
 // classes (colours 2c and 2c+1) as 8..11.
 int elastic_table(int color) { return color < 8 ? P_CELL + color : P_ROW + (color - 8); }
+
+// A row body of a VS_SCENE_FOLD scene has the nodes its last colour pair
+// writes finalized and predicted inside that pass (unless the node passes
+// are kept separate).
+bool fold_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
+{
+    return !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD) && row_body(w, b);
+}
+
+// The nodes of a fold body its last colour pair does not write, as runs of
+// consecutive node ids: rows y = 1 .. 2*(cdy/2) and z = 1 .. 2*(cdz/2) of the
+// node grid are written (row class 3: cells with odd y and odd z), every
+// other node is left to the node pass.
+template <typename F>
+bool fold_leftovers(const BodyRec &b, F &&emit)
+{
+    const uint64_t nx = b.cell_dims[0] + 1u, ny = b.cell_dims[1] + 1u, nz = b.cell_dims[2] + 1u;
+    const uint64_t ylast = 2u * (b.cell_dims[1] / 2u), zlast = 2u * (b.cell_dims[2] / 2u);
+    const bool none = ylast == 0 || zlast == 0;
+    uint64_t run = 0, len = 0;  // pending run [run, run + len)
+    auto add = [&](uint64_t first, uint64_t count) {
+        if (len && run + len == first) {
+            len += count;
+            return true;
+        }
+        const bool ok = len == 0 || emit(b.node_begin + run, len);
+        run = first;
+        len = count;
+        return ok;
+    };
+    bool ok = true;
+    for (uint64_t z = 0; z < nz && ok; z++) {
+        if (none || z == 0 || z > zlast) {
+            ok &= add(z * nx * ny, nx * ny);
+            continue;
+        }
+        ok &= add(z * nx * ny, nx);                        // y = 0
+        if (ylast + 1 < ny) ok &= add((z * ny + ylast + 1) * nx, (ny - ylast - 1) * nx);  // y > ylast
+    }
+    if (ok && len) ok = emit(b.node_begin + run, len);
+    return ok;
+}
 
 // Rebuilds every tile table from the admitted scenes, in scene-id order.
 int rebuild_tables(vs_world *w)
@@ -296,6 +340,15 @@ int rebuild_tables(vs_world *w)
                 if (p == P_NODE) {
                     for (size_t b = 0; b < r.bodies.size(); b++)
                         ok &= emit(r.bodies[b].node_begin, r.bodies[b].node_count, (uint32_t)(r.body_begin + b));
+                } else if (p == P_REST) {
+                    for (size_t b = 0; b < r.bodies.size() && ok; b++) {
+                        const BodyRec &br = r.bodies[b];
+                        const uint32_t id = (uint32_t)(r.body_begin + b);
+                        if (fold_body(w, r, br))
+                            ok &= fold_leftovers(br, [&](uint64_t first, uint64_t count) { return emit(first, count, id); });
+                        else
+                            ok &= emit(br.node_begin, br.node_count, id);
+                    }
                 } else if (p < P_CELL + 8) {
                     const int c = p - P_CELL;
                     for (size_t b = 0; b < r.bodies.size(); b++)
@@ -337,6 +390,13 @@ int rebuild_tables(vs_world *w)
             w->h_scene_counts.p[(size_t)p * S + s] = (uint32_t)(n - start);
         }
         w->tile_count[p] = (uint32_t)n;
+        if (p == P_REST) {
+            // Leftover runs of folded bodies are a few node rows each: a
+            // 64-lane workgroup per run then wastes less than 256.
+            uint64_t items = 0;
+            for (uint64_t i = 0; i < n; i++) items += tiles[i].count;
+            w->block_rest = n && items < 128 * n ? 64u : 256u;
+        }
         w->h_tiles[p].p[0] = (uint32_t)n;
         w->h_tiles[p].p[1] = w->h_tiles[p].p[2] = w->h_tiles[p].p[3] = 0;
         HIPW(w, hipMemcpyAsync(w->d_tiles[p], w->h_tiles[p].p, 16 + n * sizeof(vs_tile), hipMemcpyHostToDevice,
@@ -417,7 +477,7 @@ struct Profile {
 // Enqueues one launch of a pass over the whole table (scene < 0) or one
 // scene's slice: `color` picks the elastic colour, `limit_only` the
 // stretch-only sweeps. A pass with no tiles enqueues nothing.
-int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_only)
+int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_only, uint32_t node_mode = 0)
 {
     using namespace vsim;
     auto shape = [&](int p) { return Shape{ grid_for(w, p, scene), pass_block(w, p), scene < 0 ? w->cfg.xcd_chunked : 0u }; };
@@ -438,19 +498,20 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
         if (!sh.grid) return VS_OK;
         if (color >= 8)
             HIPW(w, launch_elastic_rows(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, elastic_table(color), scene),
-                                        (uint32_t)(color - 8), limit_only));
+                                        (uint32_t)(color - 8), limit_only, node_mode));
         else
             HIPW(w, launch_elastic(st, sh, n, w->d_bodies, w->d_scenes, w->d_cells, tile_view(w, P_CELL + color, scene),
                                    limit_only));
         return VS_OK;
     case VS_P_FINALIZE:
-        sh = shape(P_NODE);
-        if (sh.grid) HIPW(w, launch_finalize(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_NODE, scene)));
+        // Every node, or, after folded colour pairs, the ones they leave.
+        sh = shape(P_REST);
+        if (sh.grid) HIPW(w, launch_finalize(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_REST, scene)));
         return VS_OK;
     case VS_P_FINALIZE_PREDICT:
-        sh = shape(P_NODE);
+        sh = shape(P_REST);
         if (sh.grid)
-            HIPW(w, launch_finalize_predict(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_NODE, scene)));
+            HIPW(w, launch_finalize_predict(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_REST, scene)));
         return VS_OK;
     case VS_P_SPLAT:
         sh = shape(P_SPLAT);
@@ -494,24 +555,29 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
     const bool merge = !w->cfg.separate_node_passes;
     hipStream_t st = w->stream;
     // With a profile, events bracket every launch of a non-empty pass.
-    auto pass = [&](int kind, int color, uint32_t limit_only) -> int {
+    auto pass = [&](int kind, int color, uint32_t limit_only, uint32_t node_mode = 0) -> int {
         const int p = kind == VS_P_ELASTIC ? elastic_table(color)
                       : kind == VS_P_DRIVE ? P_DRIVE
                       : kind == VS_P_SPLAT ? P_SPLAT
-                      : (kind == VS_P_PREDICT || kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_NODE
-                                                                                                         : P_SURF;
+                      : kind == VS_P_PREDICT ? P_NODE
+                      : (kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_REST
+                                                                                 : P_SURF;
         const bool timed = prof && grid_for(w, p, scene) != 0;
         if (timed) {
             prof->marks.emplace_back(kind, prof->used);
             HIPW(w, hipEventRecord(prof->next(), st));
         }
-        if (int rc = enqueue_pass(w, kind, color, scene, limit_only)) return rc;
+        if (int rc = enqueue_pass(w, kind, color, scene, limit_only, node_mode)) return rc;
         if (timed) HIPW(w, hipEventRecord(prof->next(), st));
         return VS_OK;
     };
     // Nothing runs between one substep's finalize and the next one's
-    // predict, so inside a frame the two share one node pass.
+    // predict, so inside a frame the two share one node pass. In a scene
+    // with nothing between its last colour and its finalize either, the last
+    // colour-pair pass runs both for the nodes it writes, and the node pass
+    // takes the rest.
     for (uint32_t s = 0; s < substeps; s++) {
+        const bool last = s + 1 == substeps;
         if (s == 0 || !merge)
             if (int rc = pass(VS_P_PREDICT, 0, 0)) return rc;
         if (int rc = pass(VS_P_DRIVE, 0, 0)) return rc;
@@ -519,12 +585,13 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
         // goes with the two colour passes of the other bodies.
         for (int sweep = 0; sweep < (stretch ? 4 : 1); sweep++)
             for (int c = 0; c < 4; c++)
-                for (int e : { 8 + c, 2 * c, 2 * c + 1 })
-                    if (int rc = pass(VS_P_ELASTIC, e, sweep > 0 ? 1u : 0u)) return rc;
+                for (int e : { 8 + c, 2 * c, 2 * c + 1 }) {
+                    const uint32_t fold = merge && sweep == 0 && e == 11 ? (last ? 2u : 1u) : 0u;
+                    if (int rc = pass(VS_P_ELASTIC, e, sweep > 0 ? 1u : 0u, fold)) return rc;
+                }
         if (contact)
             for (int kind : { VS_P_BIN, VS_P_SORT, VS_P_COLLIDE, VS_P_SETTLE })
                 if (int rc = pass(kind, 0, 0)) return rc;
-        const bool last = s + 1 == substeps;
         if (int rc = pass(merge && !last ? VS_P_FINALIZE_PREDICT : VS_P_FINALIZE, 0, 0)) return rc;
     }
     if (splats)
@@ -669,6 +736,10 @@ static int world_init(vs_world *w)
     W->splats.init(c.max_splats);
     // A table needs a tile per `tile` items plus one partial tile per body.
     W->tile_cap[P_NODE] = c.max_nodes / W->tile_nodes + c.max_bodies + 1;
+    // Leftover runs of a fold body: at most one per z-slice of its node grid
+    // (a slice's last rows join the next slice's first), plus one; a slice
+    // holds at least 4 nodes.
+    W->tile_cap[P_REST] = c.max_nodes / W->tile_nodes + c.max_nodes / 4 + 2 * c.max_bodies + 1;
     W->tile_cap[P_SURF] = c.max_surface / W->tile_surf + c.max_scenes + 1;
     W->tile_cap[P_DRIVE] = c.max_driven / W->tile_drive + c.max_scenes + 1;
     W->tile_cap[P_SPLAT] = c.max_splats / W->tile_splat + c.max_bodies + 1;
@@ -907,6 +978,7 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
     P.surf_base = (uint32_t)r.surf_begin;
     P.surf_count = (uint32_t)nsurf;
     P.bucket_slots = BUCKET_SLOTS;
+    P.flags = !r.contact && !r.stretch ? VS_SCENE_FOLD : 0u;
 
     const size_t N = w->cfg.max_nodes;
     uint64_t node_at = r.node_begin, cell_at = r.cell_begin, splat_at = r.splat_begin, surf_at = r.surf_begin;
@@ -1143,7 +1215,7 @@ int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph
     std::vector<uint32_t> key = { substeps, mode, (uint32_t)w->any_stretch, (uint32_t)w->any_contact,
                                   w->cfg.separate_node_passes, w->pair_cap,
                                   w->block_cells,
-                                  w->cfg.xcd_chunked, w->cfg.separate_colours };
+                                  w->cfg.xcd_chunked, w->cfg.separate_colours, w->block_rest };
     for (int p = 0; p < NPASS; p++) key.push_back(grid_for(w, p, -1));
     if (!w->exec || key != w->graph_key) {
         if (w->exec) HIPW(w, hipGraphExecDestroy(w->exec));
@@ -1370,8 +1442,9 @@ int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, ui
     const int p = kind == VS_P_ELASTIC ? P_CELL
                   : kind == VS_P_DRIVE ? P_DRIVE
                   : kind == VS_P_SPLAT ? P_SPLAT
-                  : (kind == VS_P_PREDICT || kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_NODE
-                                                                                                     : P_SURF;
+                  : kind == VS_P_PREDICT ? P_NODE
+                  : (kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_REST
+                                                                             : P_SURF;
     const int kinds = kind == VS_P_ELASTIC ? 12 : 1;  // elastic: colours 0..7 and row classes 8..11
     auto table = [&](int c) { return kind == VS_P_ELASTIC ? elastic_table(c) : p; };
     uint32_t per_round = 0;
