@@ -235,8 +235,8 @@ static void project_tet(v3 *p, const v3 *prev, const real *w, const int *tet, m3
 }
 
 /* The same projection in the evaluation order the GPU kernel uses. The
- * mathematics is project.wgsl's, rewritten so that it needs no square root
- * and no 3x3 scaling:
+ * mathematics is project.wgsl's, rewritten so that it needs no square root,
+ * one division, and no 3x3 scaling:
  *
  * - Every Kuhn tetrahedron is an axis path, so F = Ds * Dm^-1 = s * E, with
  *   E the path's edge differences (axis by axis) and s = 1/h; the cofactor
@@ -246,25 +246,30 @@ static void project_tet(v3 *p, const v3 *prev, const real *w, const int *tet, m3
  *   gradients into and out of it: d_i = c * e_i with c = s / |E| and e_i
  *   the edge differences of E, h_i = s^3 * f_i with f_i those of K
  *   (d0 = -(d1 + d2 + d3) holds by construction).
- * - With u = c * lambda_D and v = s^3 * lambda_H the 2x2 block of
- *   xpbdCouple.wgsl:58-71 becomes
- *       [c^2 Ge st + aD,  c^2 Gef st ] [u]   [-(s^2 + r c^2 Re)      ]
- *       [s^6 Gef st,      s^6 Gf st + aH] [v] = [-s^3 (C_H + r s^3 Rf)]
- *   (G.. = sum w e.e, e.f, f.f; R. = sum e.x, f.x; st = 1 + r), whose
- *   determinant is the original one, and the corner update is
- *   w_i (e_i u + f_i v), the correction formed before it meets the position
- *   (one rounding at the position's scale). Only c^2 = s^2 / |E|^2 enters,
- *   and c * C_D = s^2.
- *   A tetrahedron with |F| <= 1e-9 has no deviatoric gradient: c^2 = 0,
- *   and then the first row and the coupling are taken as 0.
+ * - With u = c * lambda_D and v = s^3 * lambda_H, and the first row of the
+ *   2x2 block of xpbdCouple.wgsl:58-71 multiplied by 1/c^2 = |E|^2 h^2, the
+ *   block becomes
+ *       [Ge st + aD h^2 |E|^2,  Gef st        ] [u]   [-(|E|^2 + r Re)      ]
+ *       [s^6 Gef st,            s^6 Gf st + aH] [v] = [-s^3 (C_H + r s^3 Rf)]
+ *   (G.. = sum w e.e, e.f, f.f; R. = sum e.x, f.x; st = 1 + r), using
+ *   c * C_D = s^2. Its determinant is the original one times |E|^2 h^2, so
+ *   the singular-block test compares with 1e-20 h^2 |E|^2. The corner
+ *   update is w_i (e_i u + f_i v), the correction formed before it meets the
+ *   position (one rounding at the position's scale).
+ * - A tetrahedron with |F| <= 1e-9 (|E|^2 <= 1e-18 h^2) has no deviatoric
+ *   gradient: u = 0, and the block is the original diagonal one,
+ *   [aD, 0; 0, s^6 Gf st + aH], tested against 1e-20 unscaled.
  * Products feeding a sum are fused (one rounding), and the block divides
  * once. */
-static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real s, real gamma, real aD, real aH,
+static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real h, real gamma, real aD, real aH,
                               real relax)
 {
     const int *tv = KUHN_TETS[t];
     real wt[4] = { w[tv[0]], w[tv[1]], w[tv[2]], w[tv[3]] };
     if (((wt[0] + wt[1]) + wt[2]) + wt[3] == (real)0) return;
+    /* Per-body constants (the kernel forms them once per tile). */
+    real s = (real)1 / h, s2 = s * s, s3 = s2 * s, s6 = s3 * s3, h2 = h * h;
+    real aDh2 = aD * h2, devMin = (real)(float)1e-18 * h2, detMin = (real)(float)1e-20 * h2;
     v3 E[3];
     for (int j = 0; j < 3; j++) E[AXIS[t][j]] = sub(p[PATH[t][j + 1]], p[PATH[t][j]]);
     real e2 = E[0].x * E[0].x;
@@ -276,10 +281,8 @@ static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real s, 
         e2 = FMA(E[c].z, E[c].z, e2);
     }
     v3 K[3] = { crossf(E[1], E[2]), crossf(E[2], E[0]), crossf(E[0], E[1]) };
-    real s2 = s * s, s3 = s2 * s, s6 = s3 * s3;
     real cHyd = FMA(dotf(E[0], K[0]), s3, -gamma);
-    int dev = s2 * e2 > (real)(float)1e-18;
-    real c2 = dev ? s2 / e2 : (real)0;
+    int dev = e2 > devMin;
     v3 e[4], f[4], x[4];
     for (int i = 0; i < 4; i++) {
         int j = POS[t][i];
@@ -304,13 +307,13 @@ static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real s, 
         rf = dotacc(f[i], x[i], rf);
     }
     real st = (real)1 + relax;
-    real A = FMA(c2 * ge, st, aD);
-    real B = (c2 * gef) * st;
+    real A = dev ? FMA(ge, st, aDh2 * e2) : aD;
+    real B = dev ? gef * st : (real)0;
     real C = dev ? (s6 * gef) * st : (real)0;
     real D = FMA(s6 * gf, st, aH);
     real det = FMA(A, D, -(B * C));
-    if (det < (real)(float)1e-20) return;
-    real P = dev ? -FMA(relax, c2 * re, s2) : (real)0;
+    if (det < (dev ? detMin * e2 : (real)(float)1e-20)) return;
+    real P = dev ? -FMA(relax, re, e2) : (real)0;
     real Q = -(s3 * FMA(relax, s3 * rf, cHyd));
     real inv = (real)1 / det;
     real u = FMA(D, P, -(B * Q)) * inv;
@@ -344,7 +347,7 @@ void o_project_kuhn_tet(const REAL *p12, const REAL *prev12, const REAL *w4, int
         project_tet(p, q, w, KUHN_TETS[t], mscale(KUHN_INV[t], s), (real)gamma, (real)alpha_dev, (real)alpha_hyd,
                     (real)relax);
     else
-        project_tet_fused(p, q, w, t, s, (real)gamma, (real)alpha_dev, (real)alpha_hyd, (real)relax);
+        project_tet_fused(p, q, w, t, (real)h, (real)gamma, (real)alpha_dev, (real)alpha_hyd, (real)relax);
     for (int i = 0; i < 4; i++) {
         int c = KUHN_TETS[t][i];
         out12[i * 3] = p[c].x;
@@ -410,7 +413,7 @@ static void project_cell(o_scene *sc, const o_body *cp, uint32_t lin, real dt, i
                 m3 B = mscale(KUHN_INV[t], restScale);
                 project_tet(p, prev, w, KUHN_TETS[t], B, gamma, aD, aH, relax);
             } else {
-                project_tet_fused(p, prev, w, t, restScale, gamma, aD, aH, relax);
+                project_tet_fused(p, prev, w, t, h, gamma, aD, aH, relax);
             }
         }
     }

@@ -76,17 +76,22 @@ __device__ __forceinline__ V3 crossf(V3 a, V3 b)
 // One tetrahedron of project.wgsl:75-153 (the stable Neo-Hookean pair solved
 // as the 2x2 block of xpbdCouple.wgsl:58-71), in the evaluation order the
 // oracle's project_tet_fused defines. With E the path's edge differences
-// (F = s E, s = 1/h), K = cof(E) (cof F = s^2 K), c = s/|E| and the scaled
-// multipliers u = c lambda_D, v = s^3 lambda_H, the block is
-//     [c^2 Ge st + aD,  c^2 Gef st    ] [u]   [-(s^2 + r c^2 Re)     ]
-//     [s^6 Gef st,      s^6 Gf st + aH] [v] = [-s^3 (C_H + r s^3 Rf)]
+// (F = s E, s = 1/h), K = cof(E) (cof F = s^2 K), c = s/|E|, the scaled
+// multipliers u = c lambda_D, v = s^3 lambda_H, and the block's first row
+// multiplied by 1/c^2 = |E|^2 h^2, the block is
+//     [Ge st + aD h^2 |E|^2,  Gef st        ] [u]   [-(|E|^2 + r Re)      ]
+//     [s^6 Gef st,            s^6 Gf st + aH] [v] = [-s^3 (C_H + r s^3 Rf)]
 // over the corner differences e of E and f of K (G.. weighted Gram sums,
 // R. rates, st = 1 + r), and corner i moves by w_i (e_i u + f_i v). Same
-// mathematics and determinant as the WGSL form; no square root, and no 3x3
-// scaling of F or of the gradients.
+// mathematics as the WGSL form, with no square root, one division and no
+// 3x3 scaling of F or of the gradients. |F| <= 1e-9 leaves the deviatoric
+// row out (u = 0), as the WGSL's zero gradient does.
+struct TetConsts {
+    float s3, s6, gamma, aD, aH, aDh2, relax, devMin, detMin;
+};
+
 template <int T>
-__device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const float (&w)[8], float s2, float s3,
-                                            float s6, float gamma, float aD, float aH, float relax)
+__device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const float (&w)[8], const TetConsts &k)
 {
     constexpr Tet tet = TETS[T];
     const float wt[4] = { w[tet.v[0]], w[tet.v[1]], w[tet.v[2]], w[tet.v[3]] };
@@ -105,9 +110,8 @@ __device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const 
         e2 = fmaf(E[c].z, E[c].z, e2);
     }
     const V3 K[3] = { crossf(E[1], E[2]), crossf(E[2], E[0]), crossf(E[0], E[1]) };
-    const float cHyd = fmaf(dotf(E[0], K[0]), s3, -gamma);
-    const bool dev = s2 * e2 > 1e-18f;
-    const float c2 = dev ? s2 / e2 : 0.0f;
+    const float cHyd = fmaf(dotf(E[0], K[0]), k.s3, -k.gamma);
+    const bool dev = e2 > k.devMin;
     V3 e[4], f[4], x[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -133,15 +137,15 @@ __device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const 
         re = dotacc(e[i], x[i], re);
         rf = dotacc(f[i], x[i], rf);
     }
-    const float st = 1.0f + relax;
-    const float A = fmaf(c2 * ge, st, aD);
-    const float B = (c2 * gef) * st;
-    const float C = dev ? (s6 * gef) * st : 0.0f;
-    const float D = fmaf(s6 * gf, st, aH);
+    const float st = 1.0f + k.relax;
+    const float A = dev ? fmaf(ge, st, k.aDh2 * e2) : k.aD;
+    const float B = dev ? gef * st : 0.0f;
+    const float C = dev ? (k.s6 * gef) * st : 0.0f;
+    const float D = fmaf(k.s6 * gf, st, k.aH);
     const float det = fmaf(A, D, -(B * C));
-    if (det < 1e-20f) return;
-    const float P = dev ? -fmaf(relax, c2 * re, s2) : 0.0f;
-    const float Q = -(s3 * fmaf(relax, s3 * rf, cHyd));
+    if (det < (dev ? k.detMin * e2 : 1e-20f)) return;
+    const float P = dev ? -fmaf(k.relax, re, e2) : 0.0f;
+    const float Q = -(k.s3 * fmaf(k.relax, k.s3 * rf, cHyd));
     const float inv = 1.0f / det;
     const float u = fmaf(D, P, -(B * Q)) * inv;
     const float v = fmaf(A, Q, -(C * P)) * inv;
@@ -344,15 +348,20 @@ __global__ __launch_bounds__(BLOCK) void k_finalize_predict(NodeView n, const vs
 // The constants a cell projection reads, per body and dt (project.wgsl:35-60,
 // xpbdCouple.wgsl:58-71).
 struct CellConsts {
-    float rs2, rs3, rs6, gamma, aD, aH, relax, h, maxStretch;  // rsN = (1/h)^N, rounded step by step
+    TetConsts tet;
+    float h, maxStretch;
 };
 
+// Per body and dt, in the oracle's order: s = 1/h and its powers rounded
+// step by step, alphas over dt^2, the thresholds scaled by h^2.
 __device__ __forceinline__ CellConsts cell_consts(const vs_body &bd, float dt)
 {
     const float invDt2 = 1.0f / (dt * dt);
-    const float s = 1.0f / bd.h, s2 = s * s, s3 = s2 * s;
-    return CellConsts{ s2, s3, s3 * s3, bd.gamma, bd.alpha_dev * invDt2, bd.alpha_hyd * invDt2,
-                       bd.relaxation / dt, bd.h, bd.max_stretch };
+    const float h = bd.h, s = 1.0f / h, s2 = s * s, s3 = s2 * s, h2 = h * h;
+    const float aD = bd.alpha_dev * invDt2;
+    return CellConsts{ TetConsts{ s3, s3 * s3, bd.gamma, aD, bd.alpha_hyd * invDt2, aD * h2, bd.relaxation / dt,
+                                  1e-18f * h2, 1e-20f * h2 },
+                       h, bd.max_stretch };
 }
 
 // One cell: the six Kuhn tetrahedra in order, then the tensile limit
@@ -362,12 +371,12 @@ __device__ __forceinline__ void solve_cell(V3 (&p)[8], const V3 (&q)[8], const f
                                            uint32_t limit_only)
 {
     if (!limit_only) {
-        project_tet<0>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
-        project_tet<1>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
-        project_tet<2>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
-        project_tet<3>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
-        project_tet<4>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
-        project_tet<5>(p, q, w, k.rs2, k.rs3, k.rs6, k.gamma, k.aD, k.aH, k.relax);
+        project_tet<0>(p, q, w, k.tet);
+        project_tet<1>(p, q, w, k.tet);
+        project_tet<2>(p, q, w, k.tet);
+        project_tet<3>(p, q, w, k.tet);
+        project_tet<4>(p, q, w, k.tet);
+        project_tet<5>(p, q, w, k.tet);
     }
     if (k.maxStretch > 1.0f) {
         for (int sweep = 0; sweep < 2; sweep++) {
@@ -461,8 +470,9 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
 // `node_mode` (class 3, the last colours, in a scene flagged VS_SCENE_FOLD):
 // after the colours, each node the lane writes is finalized (1, 2) and then
 // predicted for the next substep (1), as the node passes would do next.
+// Four waves per SIMD: at most 128 VGPRs.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_elastic_rows(NodeView n, const vs_body *__restrict__ bodies,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_elastic_rows(NodeView n, const vs_body *__restrict__ bodies,
                                                         const vs_scene_params *__restrict__ scenes, TileView tv,
                                                         uint32_t cls, uint32_t limit_only, uint32_t node_mode,
                                                         uint32_t xcd)
