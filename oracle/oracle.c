@@ -266,7 +266,10 @@ static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real h, 
 {
     const int *tv = KUHN_TETS[t];
     real wt[4] = { w[tv[0]], w[tv[1]], w[tv[2]], w[tv[3]] };
-    if (((wt[0] + wt[1]) + wt[2]) + wt[3] == (real)0) return;
+    /* Weights are inverse masses, never negative: the sum is 0 exactly when
+     * the WGSL's is. Grouped by path position (ends, middle) for Gef below. */
+    real wend = wt[0] + wt[3], wmid = wt[1] + wt[2];
+    if (wend + wmid == (real)0) return;
     /* Per-body constants (the kernel forms them once per tile). */
     real s = (real)1 / h, s2 = s * s, s3 = s2 * s, s6 = s3 * s3, h2 = h * h;
     real aDh2 = aD * h2, devMin = (real)(float)1e-18 * h2, detMin = (real)(float)1e-20 * h2;
@@ -281,7 +284,8 @@ static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real h, 
         e2 = FMA(E[c].z, E[c].z, e2);
     }
     v3 K[3] = { crossf(E[1], E[2]), crossf(E[2], E[0]), crossf(E[0], E[1]) };
-    real cHyd = FMA(dotf(E[0], K[0]), s3, -gamma);
+    real detE = dotf(E[0], K[0]);
+    real cHyd = FMA(detE, s3, -gamma);
     int dev = e2 > devMin;
     v3 e[4], f[4], x[4];
     for (int i = 0; i < 4; i++) {
@@ -298,14 +302,18 @@ static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real h, 
         }
         x[i] = sub(p[tv[i]], q[tv[i]]);
     }
-    real ge = 0, gef = 0, gf = 0, re = 0, rf = 0;
+    real ge = 0, gf = 0, re = 0, rf = 0;
     for (int i = 0; i < 4; i++) {
         ge = FMA(wt[i], dotf(e[i], e[i]), ge);
-        gef = FMA(wt[i], dotf(e[i], f[i]), gef);
         gf = FMA(wt[i], dotf(f[i], f[i]), gf);
         re = dotacc(e[i], x[i], re);
         rf = dotacc(f[i], x[i], rf);
     }
+    /* e_i . f_i is det E at the path's ends and 2 det E inside it (E_a . K_a =
+     * det E, E_a . K_b = 0 for a != b), so Gef = det E * sum of the weights
+     * by path position (1, 2, 2, 1). Every Kuhn tetrahedron lists its path's
+     * ends, corners 0 and 7, first and last. */
+    real gef = detE * FMA((real)2, wmid, wend);
     real st = (real)1 + relax;
     real A = dev ? FMA(ge, st, aDh2 * e2) : aD;
     real B = dev ? gef * st : (real)0;

@@ -95,7 +95,11 @@ __device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const 
 {
     constexpr Tet tet = TETS[T];
     const float wt[4] = { w[tet.v[0]], w[tet.v[1]], w[tet.v[2]], w[tet.v[3]] };
-    if (((wt[0] + wt[1]) + wt[2]) + wt[3] == 0.0f) return;
+    // Inverse masses are never negative: this sum is 0 exactly when the
+    // WGSL's is. Grouped by path position (ends, middle) for Gef below.
+    static_assert(tet.v[0] == 0 && tet.v[3] == 7, "a Kuhn tetrahedron lists its path's ends first and last");
+    const float wend = wt[0] + wt[3], wmid = wt[1] + wt[2];
+    if (wend + wmid == 0.0f) return;
 
     V3 E[3];
 #pragma unroll
@@ -110,7 +114,8 @@ __device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const 
         e2 = fmaf(E[c].z, E[c].z, e2);
     }
     const V3 K[3] = { crossf(E[1], E[2]), crossf(E[2], E[0]), crossf(E[0], E[1]) };
-    const float cHyd = fmaf(dotf(E[0], K[0]), k.s3, -k.gamma);
+    const float detE = dotf(E[0], K[0]);
+    const float cHyd = fmaf(detE, k.s3, -k.gamma);
     const bool dev = e2 > k.devMin;
     V3 e[4], f[4], x[4];
 #pragma unroll
@@ -128,15 +133,18 @@ __device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const 
         }
         x[i] = sub(p[tet.v[i]], q[tet.v[i]]);
     }
-    float ge = 0.f, gef = 0.f, gf = 0.f, re = 0.f, rf = 0.f;
+    float ge = 0.f, gf = 0.f, re = 0.f, rf = 0.f;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         ge = fmaf(wt[i], dotf(e[i], e[i]), ge);
-        gef = fmaf(wt[i], dotf(e[i], f[i]), gef);
         gf = fmaf(wt[i], dotf(f[i], f[i]), gf);
         re = dotacc(e[i], x[i], re);
         rf = dotacc(f[i], x[i], rf);
     }
+    // e_i . f_i is det E at the path's ends and 2 det E inside it (E_a . K_a
+    // = det E, E_a . K_b = 0 for a != b): Gef = det E * (w0 + 2 w1 + 2 w2 + w3)
+    // by path position.
+    const float gef = detE * fmaf(2.0f, wmid, wend);
     const float st = 1.0f + k.relax;
     const float A = dev ? fmaf(ge, st, k.aDh2 * e2) : k.aD;
     const float B = dev ? gef * st : 0.0f;
