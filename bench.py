@@ -4,14 +4,14 @@ Default workload: 256 independent scenes (jelly blocks, lattice side ~ U{8..48},
 10k-500k Gaussians each, 10 substeps per 1/60 s frame), stepped batched -- one
 launch per pass across every scene, the frame replayed from a hipGraph -- and,
 beside it, the plain per-scene launch path on the same scenes. A step is one
-frame of every scene: 10 substeps (predict, 8 colour passes, finalize) and the
-splat deformation. Inputs are synthetic and resident on the device before the
-timed region.
+frame of every scene: 10 substeps (predict, the 8 colours -- in pairs, by rows
+of cells, one launch per pair -- finalize) and the splat deformation. Inputs
+are synthetic and resident on the device before the timed region.
 
 Prints ONE JSON line. `value` is the batched whole-job node-substeps/s;
 `plain`, `speedup`, frame latency p50/p99, the per-pass profile, the roofline
-of the dominant kernel (the elastic colour pass) and the CPU baseline (the f32
-oracle on a bounded sample, one core) ride along.
+of the dominant kernel (the elastic colour-pair pass) and the CPU baseline
+(the f32 oracle on a bounded sample, one core) ride along.
 """
 import argparse
 import json
@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+VALU_ISSUE_PER_S = 256 * 2.4e9   # wave instructions per second: one per CU per cycle
 # Algorithmic bytes (SoA f32) per item of each pass, per launch:
 ELASTIC_BYTES_PER_CELL = 8 * (12 + 12 + 4) + 8 * 12 + 4   # gather p, prev, w; scatter p; cell id
 PREDICT_BYTES_PER_NODE = 64
@@ -34,16 +35,43 @@ FINALIZE_BYTES_PER_NODE = 52
 SPLAT_BYTES = 76
 
 
-def elastic_traffic_bytes():
-    """HBM bytes per batched colour-pass launch from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (profiles/batched256_hbm_traffic.md, scaled
-    by the copy calibration), or None when it is absent."""
-    path = os.path.join(ROOT, "profiles", "batched256_hbm_traffic.md")
+def _profile_rows(name, kernel):
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
-        return None
-    rows = [line.split("|") for line in open(path) if line.startswith("| k_elastic")]
-    totals = [float(r[6]) for r in rows if len(r) > 6]
-    return sum(totals) / len(totals) * 1e6 if totals else None
+        return []
+    return [line.split("|") for line in open(path) if line.startswith(f"| {kernel}<")]
+
+
+def elastic_traffic_bytes(kernel):
+    """HBM bytes per batched launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/batched256_hbm_traffic.md, scaled
+    by the copy calibration), averaged over the frame's launches, or None
+    when it is absent."""
+    rows = [(float(r[3]), float(r[6])) for r in _profile_rows("batched256_hbm_traffic.md", kernel) if len(r) > 6]
+    n = sum(d for d, _ in rows)
+    return sum(d * v for d, v in rows) / n * 1e6 if n else None
+
+
+def elastic_valu_instructions(kernel):
+    """VALU wave instructions per batched launch of `kernel` (rocprofv3
+    SQ_INSTS_VALU, profiles/batched256_valu.md), or None."""
+    rows = [(float(r[3]), float(r[4])) for r in _profile_rows("batched256_valu.md", kernel) if len(r) > 6]
+    n = sum(d for d, _ in rows)
+    return sum(d * v for d, v in rows) / n * 1e6 if n else None
+
+
+def pair_launch_bytes(scs):
+    """Algorithmic bytes of one colour-pair launch, averaged over the row
+    classes: 40 B (p, prev, w read; p written) per node of the rows it covers
+    (class (py, pz): cell rows y = py + 2a, z = pz + 2b, two node rows each)."""
+    total = 0
+    for sc in scs:
+        for b in sc.bodies:
+            cx, cy, cz = b.lattice.cell_dims
+            for cls in range(4):
+                py, pz = cls & 1, cls >> 1
+                total += (cx + 1) * 2 * ((cy + 1 - py) // 2) * 2 * ((cz + 1 - pz) // 2)
+    return 40.0 * total / 4
 
 
 def log(msg):
@@ -304,9 +332,17 @@ def main():
     el_ms, el_n = world.time_pass("elastic", substeps)
     el_avg_s = el_ms / el_n / 1e3
     # One sweep is 8 colour launches, or 4 colour-pair launches (rows of a
-    # dense lattice read no cell list: 320 B a cell and colour).
+    # dense lattice read no cell list: 320 B a cell and colour). The bytes
+    # are the per-colour model's for the cells a launch solves, whatever the
+    # launch actually moves (a pair moves each node once for two colours).
     per_sweep = el_n // substeps
-    el_bytes = cells * (ELASTIC_BYTES_PER_CELL if per_sweep == 8 else ELASTIC_BYTES_PER_CELL - 4) / per_sweep
+    pairs = per_sweep == 4
+    el_kernel = "k_elastic_rows" if pairs else "k_elastic"
+    colour_model_bytes = cells * (ELASTIC_BYTES_PER_CELL - 4 if pairs else ELASTIC_BYTES_PER_CELL) / per_sweep
+    # A colour-pair launch reads p, prev, w (28 B) and writes p (12 B) of the
+    # nodes its rows cover, once for both colours: those are its algorithmic
+    # bytes, averaged over the four row classes.
+    el_bytes = pair_launch_bytes(scs) if pairs else colour_model_bytes
     achieved = el_bytes / el_avg_s / 1e9
     log(f"[bench] colour pass: {el_n} launches, {el_avg_s * 1e6:.2f} us avg, {achieved:.0f} GB/s algorithmic")
     # ---- the splat pass alone, covariance output and with the engine format
@@ -371,10 +407,14 @@ def main():
         "nodes": nodes, "cells": cells, "splats": splats, "substeps_per_frame": substeps,
         "profile_ms": {k: {"total": v[0], "launches": v[1]} for k, v in prof.items() if v[1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": elastic_traffic_bytes(),
-                     "kernel": "k_elastic (colour pass)", "bytes_per_launch": el_bytes,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": elastic_traffic_bytes(el_kernel),
+                     "kernel": el_kernel + (" (colour pair)" if pairs else " (colour pass)"),
+                     "bytes_per_launch": el_bytes,
+                     "colour_model_bytes_per_launch": colour_model_bytes,
                      "avg_launch_us": el_avg_s * 1e6,
-                     "avg_launch_us_with_events": prof["elastic"][0] / prof["elastic"][1] * 1e3},
+                     "avg_launch_us_with_events": prof["elastic"][0] / prof["elastic"][1] * 1e3,
+                     "valu_issue_frac": (lambda v: v / (el_avg_s * VALU_ISSUE_PER_S) if v else None)(
+                         elastic_valu_instructions(el_kernel))},
         "splat_pass_us": splat_us,
         "serving": serving,
         "ingest": ingest_out,

@@ -974,9 +974,11 @@ __device__ __forceinline__ void ellipsoid_axes(float d[3], float o[3], float q[4
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_splats(const float *__restrict__ px, const float *__restrict__ py,
                                                   const float *__restrict__ pz, const vs_body *__restrict__ bodies,
-                                                  SplatView sp, TileView tv)
+                                                  SplatView sp, TileView tv, uint32_t xcd)
 {
-    const Range r = tile_range(*tv.count, 0);
+    // With `xcd`, each XCD takes a contiguous run of tiles: a body's splats
+    // (stored in cell order) then gather its corner nodes through one L2.
+    const Range r = tile_range(*tv.count, xcd);
     for (uint32_t t = r.first; t < r.end; t += r.stride) {
         const vs_tile tile = tv.tiles[t];
         const vs_body bd = bodies[tile.body];
@@ -1130,7 +1132,7 @@ hipError_t launch_drive(hipStream_t s, Shape sh, NodeView n, const vs_scene_para
 hipError_t launch_splats(hipStream_t s, Shape sh, const float *px, const float *py, const float *pz,
                          const vs_body *bodies, SplatView sp, TileView t)
 {
-    VSIM_DISPATCH_BLOCK(k_splats, sh, s, px, py, pz, bodies, sp, t);
+    VSIM_DISPATCH_BLOCK(k_splats, sh, s, px, py, pz, bodies, sp, t, sh.xcd);
 }
 
 }  // namespace vsim

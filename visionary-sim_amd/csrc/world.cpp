@@ -514,7 +514,11 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
             HIPW(w, launch_finalize_predict(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_REST, scene)));
         return VS_OK;
     case VS_P_SPLAT:
+        // Contiguous tile runs per XCD (the grid rounded up to a multiple of
+        // 8; surplus workgroups find no tile).
         sh = shape(P_SPLAT);
+        sh.grid = (sh.grid + 7u) / 8u * 8u;
+        sh.xcd = 1;
         if (sh.grid)
             HIPW(w, launch_splats(st, sh, n.px, n.py, n.pz, w->d_bodies, splat_view(w), tile_view(w, P_SPLAT, scene)));
         return VS_OK;

@@ -3,11 +3,15 @@
     python -m vsim.profsum trace <dir>/..._kernel_trace.csv
     python -m vsim.profsum pmc <fetch_counter_collection.csv> <write_counter_collection.csv> \
         <copy_fetch.csv> <copy_write.csv>
+    python -m vsim.profsum valu <counter_collection.csv with SQ_INSTS_VALU>
 
 `trace` prints per (kernel, launch grid) call counts and average durations, so
 the batched launches (large grids) separate from the per-scene ones. `pmc`
 prints FETCH_SIZE / WRITE_SIZE per dispatch, scaled to bytes by what the same
-counters read for a device-to-device copy of known size.
+counters read for a device-to-device copy of known size. `valu` prints
+SQ_INSTS_VALU (wave instructions) per dispatch against the VALU issue peak
+of the dispatch's duration: one wave instruction per CU per cycle, 256 CUs
+at 2.4 GHz.
 """
 import collections
 import csv
@@ -67,8 +71,33 @@ def pmc(fetch, write, cfetch, cwrite, copy_bytes=1 << 30):
     return "\n".join(out)
 
 
+VALU_ISSUE_PER_S = 256 * 2.4e9  # wave instructions per second, MI355X
+
+
+def valu(path, batched_min=1 << 19):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != "SQ_INSTS_VALU":
+            continue
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+        d[(short(r["Kernel_Name"]), int(r.get("Grid_Size") or 0))].append((float(r["Counter_Value"]), dur))
+    out = ["| kernel | grid (threads) | dispatches | VALU wave instructions (M) | avg us | VALU issue / peak |",
+           "|---|---:|---:|---:|---:|---:|"]
+    for key in sorted(d, key=lambda k: -sum(v for v, _ in d[k])):
+        if key[1] < batched_min or "rocclr" in key[0]:
+            continue
+        v = d[key]
+        inst = sum(x for x, _ in v) / len(v)
+        dur = sum(y for _, y in v) / len(v)
+        out.append(f"| {key[0]} | {key[1]} | {len(v)} | {inst / 1e6:.1f} | {dur * 1e6:.1f} | "
+                   f"{inst / (dur * VALU_ISSUE_PER_S):.2f} |")
+    return "\n".join(out)
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
         print(trace(sys.argv[2]))
+    elif sys.argv[1] == "valu":
+        print(valu(sys.argv[2]))
     else:
         print(pmc(*sys.argv[2:6]))
