@@ -94,9 +94,11 @@ def make_world(scs, **kw):
     cells = sum(s.n_cells for s in scs)
     splats = sum(s.n_splats for s in scs)
     driven = sum(len(s.drive.nodes) for s in scs if s.drive is not None)
+    surf = sum(s.n_surface_active for s in scs)
+    buckets = sum(int(np.prod(s.grid()[2])) for s in scs if s.contact_active)
     return native.World(max_scenes=len(scs) + 1, max_bodies=sum(len(s.bodies) for s in scs) + 1,
-                        max_nodes=nodes + 1024, max_cells=cells + 1024, max_surface=1024, max_buckets=1024,
-                        max_driven=driven + 64, max_splats=splats + 1024, **kw)
+                        max_nodes=nodes + 1024, max_cells=cells + 1024, max_surface=surf + 1024,
+                        max_buckets=buckets + 1024, max_driven=driven + 64, max_splats=splats + 1024, **kw)
 
 
 def set_targets(world, ids, scs, frame):
@@ -230,7 +232,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="batched256", choices=["batched256", "c1", "c2"])
+    ap.add_argument("--workload", default="batched256", choices=["batched256", "c1", "c2", "c3"])
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -283,9 +285,13 @@ def main():
     elif args.workload == "c1":
         scs = [scenes.c1_scene()]
         config = {"workload": "16^3 block, 50k Gaussians, 10 substeps per frame", "scenes": 1}
-    else:
+    elif args.workload == "c2":
         scs = [scenes.c2_scene()]
         config = {"workload": "64^3 block, 2M Gaussians, 27-node sinusoid handle, 10 substeps per frame", "scenes": 1}
+    else:
+        scs = [scenes.c3_scene()]
+        config = {"workload": "512 bodies of 8^3 cells (h 0.01 m) in a 1 m tank, node-node and node-wall contact, "
+                              f"E log-uniform 1e4-1e6, {scs[0].substeps} substeps per frame", "scenes": 1}
     nodes = sum(s.n_nodes for s in scs)
     cells = sum(s.n_cells for s in scs)
     splats = sum(s.n_splats for s in scs)
@@ -407,14 +413,16 @@ def main():
         "nodes": nodes, "cells": cells, "splats": splats, "substeps_per_frame": substeps,
         "profile_ms": {k: {"total": v[0], "launches": v[1]} for k, v in prof.items() if v[1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": elastic_traffic_bytes(el_kernel),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     # counter figures are profiled on the default workload only
+                     "traffic": elastic_traffic_bytes(el_kernel) if args.workload == "batched256" else None,
                      "kernel": el_kernel + (" (colour pair)" if pairs else " (colour pass)"),
                      "bytes_per_launch": el_bytes,
                      "colour_model_bytes_per_launch": colour_model_bytes,
                      "avg_launch_us": el_avg_s * 1e6,
                      "avg_launch_us_with_events": prof["elastic"][0] / prof["elastic"][1] * 1e3,
                      "valu_issue_frac": (lambda v: v / (el_avg_s * VALU_ISSUE_PER_S) if v else None)(
-                         elastic_valu_instructions(el_kernel))},
+                         elastic_valu_instructions(el_kernel) if args.workload == "batched256" else None)},
         "splat_pass_us": splat_us,
         "serving": serving,
         "ingest": ingest_out,

@@ -35,6 +35,10 @@ struct SurfaceView {
     uint32_t *key;   // bucket of the record this substep
     uint32_t *rank;  // arrival slot returned by the bucket counter
     float *dx, *dy, *dz, *hits;
+    // Optional: the record's node position and radius, written by bin for
+    // collide (positions do not change between the two). Null: collide reads
+    // them through the node id, as the reference does.
+    float4 *pay = nullptr;
 };
 
 // contactGrid (bodyContact.js:125-127): counters, then slots per bucket.

@@ -701,9 +701,11 @@ __global__ __launch_bounds__(BLOCK) void k_bin(NodeView n, const vs_scene_params
             const uint32_t s = tile.begin + i;
             const uint32_t node = sv.node[s];
             uint32_t x, y, z;
-            grid_coord(sc, ld3(n.px, n.py, n.pz, node), x, y, z);
+            const V3 p = ld3(n.px, n.py, n.pz, node);
+            grid_coord(sc, p, x, y, z);
             const uint32_t b = x + dx * (y + dy * z);
             sv.key[s] = b;
+            if (sv.pay) sv.pay[s] = make_float4(p.x, p.y, p.z, sv.radius[s]);
             const uint32_t slot = atomicAdd(&g.count[gb + b], 1u);
             sv.rank[s] = slot;
             if (slot < slots) g.slots[(size_t)(gb + b) * slots + slot] = s;
@@ -751,6 +753,63 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restric
     }
 }
 
+// The record a lane resolves against its mates (collide.wgsl:16-125).
+struct Collider {
+    uint32_t s, node, body;
+    float radius, wi, frictionI;
+    V3 pi, movedI;
+    V3 sum;
+    float hits;
+    int32_t logged;
+};
+
+// One mate record `other` of `c`, with its node position `pj` and radius
+// `rj`: the reach test and, on contact, the correction (collide.wgsl:52-118).
+__device__ __forceinline__ void collide_mate(Collider &c, uint32_t other, uint32_t mateBody, V3 pj, float rj,
+                                             NodeView n, const vs_body *__restrict__ bodies, SurfaceView sv,
+                                             PairLog log)
+{
+    if (mateBody == c.body) return;
+    const V3 d = sub(c.pi, pj);
+    const float dist2 = dot(d, d);
+    const float reach = c.radius + rj;
+    if (dist2 >= reach * reach) return;
+    if (log.cap) {
+        if ((uint32_t)c.logged < log.cap) log.mates[(size_t)c.s * log.cap + c.logged] = other;
+        c.logged++;
+    }
+    const uint32_t mate = sv.node[other];
+    const float wj = n.w[mate];
+    const float total = c.wi + wj;
+    if (total <= 0.0f) return;
+    const float share = c.wi / total;
+    const float dist = sqrtf(dist2);
+    V3 normal = mk(0.f, c.node < mate ? 1.0f : -1.0f, 0.f);
+    if (dist > 1e-9f) normal = mk(d.x / dist, d.y / dist, d.z / dist);
+    const float depth = reach - dist;
+    V3 corr = mul(normal, depth * share);
+    const V3 sliding = sub(c.movedI, sub(pj, ld3(n.qx, n.qy, n.qz, mate)));
+    const V3 tangent = sub(sliding, mul(normal, dot(sliding, normal)));
+    const float slid = sqrtf(dot(tangent, tangent));
+    if (slid > 1e-9f) {
+        const float fj = bodies[mateBody].friction;
+        const float mu = c.frictionI < fj ? c.frictionI : fj;
+        const float lim = (mu * depth) / slid;
+        corr = sub(corr, mul(tangent, share * (lim < 1.0f ? lim : 1.0f)));
+    }
+    c.sum = add(c.sum, corr);
+    c.hits += 1.0f;
+}
+
+// collide.wgsl:16-125: per record, the 3x3x3 neighbour buckets in dz, dy, dx
+// order and their slots ascending; Jacobi (writes only its own record).
+//
+// With the bin pass's payload (sv.pay), a mate's position and radius come
+// with one load, the 27 bucket counts are loaded together first, and the
+// first slot of each bucket of a z-plane is loaded for all nine buckets
+// before any is resolved: a lane waits on a few rounds of independent loads
+// instead of a chain of dependent ones per bucket. The order in which mates
+// are resolved, and so the sum, is the same.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_collide(NodeView n, const vs_body *__restrict__ bodies,
                                                    const vs_scene_params *__restrict__ scenes, SurfaceView sv,
@@ -763,75 +822,112 @@ __global__ __launch_bounds__(BLOCK) void k_collide(NodeView n, const vs_body *__
         const uint32_t slots = sc.bucket_slots, gb = sc.grid_base;
         const int dx = (int)sc.grid_dims[0], dy = (int)sc.grid_dims[1], dz = (int)sc.grid_dims[2];
         for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
-            const uint32_t s = tile.begin + i;
-            const uint32_t node = sv.node[s], body = sv.body[s];
-            const float radius = sv.radius[s];
-            const float wi = n.w[node];
-            const V3 pi = ld3(n.px, n.py, n.pz, node);
-            const V3 movedI = sub(pi, ld3(n.qx, n.qy, n.qz, node));
-            const float frictionI = bodies[body].friction;
-            const uint32_t key = sv.key[s];
+            Collider c;
+            c.s = tile.begin + i;
+            c.node = sv.node[c.s];
+            c.body = sv.body[c.s];
+            c.radius = sv.radius[c.s];
+            c.wi = n.w[c.node];
+            c.pi = ld3(n.px, n.py, n.pz, c.node);
+            c.movedI = sub(c.pi, ld3(n.qx, n.qy, n.qz, c.node));
+            c.frictionI = bodies[c.body].friction;
+            c.sum = mk(0.f, 0.f, 0.f);
+            c.hits = 0.f;
+            c.logged = 0;
+            const uint32_t key = sv.key[c.s];
             const int cx = (int)(key % (uint32_t)dx), cy = (int)((key / (uint32_t)dx) % (uint32_t)dy),
                       cz = (int)(key / (uint32_t)(dx * dy));
-            V3 sum = mk(0.f, 0.f, 0.f);
-            float hits = 0.f;
-            int32_t logged = 0;
-            for (int oz = -1; oz <= 1; oz++) {
-                const int z = cz + oz;
-                if (z < 0 || z >= dz) continue;
-                for (int oy = -1; oy <= 1; oy++) {
-                    const int y = cy + oy;
-                    if (y < 0 || y >= dy) continue;
-                    for (int ox = -1; ox <= 1; ox++) {
-                        const int x = cx + ox;
-                        if (x < 0 || x >= dx) continue;
+            if (sv.pay) {
+                // Bucket m = 9 (oz+1) + 3 (oy+1) + (ox+1); out of the grid holds 0.
+                uint32_t held[27];
+#pragma unroll
+                for (int m = 0; m < 27; m++) {
+                    const int x = cx + m % 3 - 1, y = cy + (m / 3) % 3 - 1, z = cz + m / 9 - 1;
+                    const bool in = x >= 0 && x < dx && y >= 0 && y < dy && z >= 0 && z < dz;
+                    const uint32_t cnt = in ? g.count[gb + (uint32_t)(x + dx * (y + dy * z))] : 0u;
+                    held[m] = cnt < slots ? cnt : slots;
+                }
+#pragma unroll
+                for (int plane = 0; plane < 3; plane++) {
+                    uint32_t first[9], mb[9];
+                    float4 pl[9];
+#pragma unroll
+                    for (int j = 0; j < 9; j++) {
+                        const int m = plane * 9 + j;
+                        first[j] = 0u;
+                        if (held[m]) {
+                            const int x = cx + m % 3 - 1, y = cy + (m / 3) % 3 - 1, z = cz + m / 9 - 1;
+                            first[j] = g.slots[(size_t)(gb + (uint32_t)(x + dx * (y + dy * z))) * slots];
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < 9; j++) {
+                        const int m = plane * 9 + j;
+                        mb[j] = 0u;
+                        pl[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (held[m]) {
+                            mb[j] = sv.body[first[j]];
+                            pl[j] = sv.pay[first[j]];
+                        }
+                    }
+                    // Buckets to visit in order: the first slot within reach
+                    // of another body, or more than one slot held. Contacts
+                    // are rare next to reach tests, so the visits reload what
+                    // they need and run through one copy of the contact code.
+                    uint32_t visit = 0u;
+#pragma unroll
+                    for (int j = 0; j < 9; j++) {
+                        const int m = plane * 9 + j;
+                        const V3 d = sub(c.pi, mk(pl[j].x, pl[j].y, pl[j].z));
+                        const float reach = c.radius + pl[j].w;
+                        const bool near = held[m] && mb[j] != c.body && dot(d, d) < reach * reach;
+                        visit |= (near || held[m] > 1u ? 1u : 0u) << j;
+                    }
+                    while (visit) {
+                        const int j = __builtin_ctz(visit);
+                        visit &= visit - 1u;
+                        const int m = plane * 9 + j;
+                        const int x = cx + m % 3 - 1, y = cy + (m / 3) % 3 - 1, z = cz + m / 9 - 1;
                         const uint32_t bk = gb + (uint32_t)(x + dx * (y + dy * z));
                         const uint32_t cnt = g.count[bk];
-                        const uint32_t held = cnt < slots ? cnt : slots;
+                        const uint32_t hk = cnt < slots ? cnt : slots;
                         const uint32_t *base = g.slots + (size_t)bk * slots;
-                        for (uint32_t k = 0; k < held; k++) {
+                        for (uint32_t k = 0; k < hk; k++) {
                             const uint32_t other = base[k];
-                            const uint32_t mateBody = sv.body[other];
-                            if (mateBody == body) continue;
-                            const uint32_t mate = sv.node[other];
-                            const V3 pj = ld3(n.px, n.py, n.pz, mate);
-                            const V3 d = sub(pi, pj);
-                            const float dist2 = dot(d, d);
-                            const float reach = radius + sv.radius[other];
-                            if (dist2 >= reach * reach) continue;
-                            if (log.cap) {
-                                if ((uint32_t)logged < log.cap) log.mates[(size_t)s * log.cap + logged] = other;
-                                logged++;
+                            const float4 q = sv.pay[other];
+                            collide_mate(c, other, sv.body[other], mk(q.x, q.y, q.z), q.w, n, bodies, sv, log);
+                        }
+                    }
+                }
+            } else {
+                for (int oz = -1; oz <= 1; oz++) {
+                    const int z = cz + oz;
+                    if (z < 0 || z >= dz) continue;
+                    for (int oy = -1; oy <= 1; oy++) {
+                        const int y = cy + oy;
+                        if (y < 0 || y >= dy) continue;
+                        for (int ox = -1; ox <= 1; ox++) {
+                            const int x = cx + ox;
+                            if (x < 0 || x >= dx) continue;
+                            const uint32_t bk = gb + (uint32_t)(x + dx * (y + dy * z));
+                            const uint32_t cnt = g.count[bk];
+                            const uint32_t held = cnt < slots ? cnt : slots;
+                            const uint32_t *base = g.slots + (size_t)bk * slots;
+                            for (uint32_t k = 0; k < held; k++) {
+                                const uint32_t other = base[k];
+                                const uint32_t mate = sv.node[other];
+                                collide_mate(c, other, sv.body[other], ld3(n.px, n.py, n.pz, mate), sv.radius[other], n,
+                                             bodies, sv, log);
                             }
-                            const float wj = n.w[mate];
-                            const float total = wi + wj;
-                            if (total <= 0.0f) continue;
-                            const float share = wi / total;
-                            const float dist = sqrtf(dist2);
-                            V3 normal = mk(0.f, node < mate ? 1.0f : -1.0f, 0.f);
-                            if (dist > 1e-9f) normal = mk(d.x / dist, d.y / dist, d.z / dist);
-                            const float depth = reach - dist;
-                            V3 corr = mul(normal, depth * share);
-                            const V3 sliding = sub(movedI, sub(pj, ld3(n.qx, n.qy, n.qz, mate)));
-                            const V3 tangent = sub(sliding, mul(normal, dot(sliding, normal)));
-                            const float slid = sqrtf(dot(tangent, tangent));
-                            if (slid > 1e-9f) {
-                                const float fj = bodies[mateBody].friction;
-                                const float mu = frictionI < fj ? frictionI : fj;
-                                const float lim = (mu * depth) / slid;
-                                corr = sub(corr, mul(tangent, share * (lim < 1.0f ? lim : 1.0f)));
-                            }
-                            sum = add(sum, corr);
-                            hits += 1.0f;
                         }
                     }
                 }
             }
-            if (log.cap) log.count[s] = logged;
-            sv.dx[s] = sum.x;
-            sv.dy[s] = sum.y;
-            sv.dz[s] = sum.z;
-            sv.hits[s] = hits;
+            if (log.cap) log.count[c.s] = c.logged;
+            sv.dx[c.s] = c.sum.x;
+            sv.dy[c.s] = c.sum.y;
+            sv.dz[c.s] = c.sum.z;
+            sv.hits[c.s] = c.hits;
         }
     }
 }

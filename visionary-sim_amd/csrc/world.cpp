@@ -139,6 +139,7 @@ struct vs_world {
     vs_body *d_bodies = nullptr;
     vs_scene_params *d_scenes = nullptr;
     uint32_t *d_surf = nullptr;
+    float4 *d_surf_pay = nullptr;  // per record: node position and radius at bin time
     uint32_t *d_grid = nullptr;
     uint32_t *d_drive = nullptr;
     float *d_splat_in = nullptr, *d_splat_out = nullptr;
@@ -217,7 +218,8 @@ SurfaceView surf_view(vs_world *w)
                         reinterpret_cast<float *>(s + 5 * S),
                         reinterpret_cast<float *>(s + 6 * S),
                         reinterpret_cast<float *>(s + 7 * S),
-                        reinterpret_cast<float *>(s + 8 * S) };
+                        reinterpret_cast<float *>(s + 8 * S),
+                        w->d_surf_pay };
 }
 
 GridView grid_view(vs_world *w) { return GridView{ w->d_grid, w->d_grid + w->cfg.max_buckets }; }
@@ -722,6 +724,7 @@ static int world_init(vs_world *w)
     ALLOC(W->d_bodies, sizeof(vs_body) * c.max_bodies);
     ALLOC(W->d_scenes, sizeof(vs_scene_params) * c.max_scenes);
     ALLOC(W->d_surf, sizeof(uint32_t) * SURF_PLANES * c.max_surface);
+    ALLOC(W->d_surf_pay, sizeof(float4) * c.max_surface);
     ALLOC(W->d_grid, sizeof(uint32_t) * (1 + BUCKET_SLOTS) * c.max_buckets);
     ALLOC(W->d_drive, sizeof(uint32_t) * DRIVE_PLANES * c.max_driven);
     ALLOC(W->d_splat_in, sizeof(float) * 10 * c.max_splats);
@@ -797,7 +800,7 @@ void vs_world_destroy(vs_world *w)
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : { w->upload_done, w->tables_done, w->params_done, w->tune[0], w->tune[1] })
         if (ev) (void)hipEventDestroy(ev);
-    void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf,
+    void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf, w->d_surf_pay,
                      w->d_grid,  w->d_drive, w->d_splat_in, w->d_splat_out, w->d_pairlog, w->d_scene_counts };
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1410,7 +1413,7 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out)
     }
     const vs_world_config &c = w->cfg;
     out->bytes_reserved = 4ull * NODE_PLANES * c.max_nodes + 4ull * c.max_cells + sizeof(vs_body) * c.max_bodies +
-                          sizeof(vs_scene_params) * c.max_scenes + 4ull * SURF_PLANES * c.max_surface +
+                          sizeof(vs_scene_params) * c.max_scenes + (4ull * SURF_PLANES + 16) * c.max_surface +
                           4ull * (1 + BUCKET_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
                           4ull * (10 + SPLAT_OUT_PLANES) * c.max_splats;
     out->graph_captures = w->captures;
