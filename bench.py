@@ -8,10 +8,14 @@ frame of every scene: 10 substeps (predict, the 8 colours -- in pairs, by rows
 of cells, one launch per pair -- finalize) and the splat deformation. Inputs
 are synthetic and resident on the device before the timed region.
 
+The autotuner (vs_world_autotune) picks the launch shapes and schedule first
+(--no-autotune keeps the defaults).
+
 Prints ONE JSON line. `value` is the batched whole-job node-substeps/s;
 `plain`, `speedup`, frame latency p50/p99, the per-pass profile, the roofline
-of the dominant kernel (the elastic colour-pair pass) and the CPU baseline
-(the f32 oracle on a bounded sample, one core) ride along.
+of the dominant kernel (the elastic colour-pair pass), each pass's achieved
+GB/s against HBM peak, and the CPU baseline (the f32 oracle on a bounded
+sample, one core) ride along.
 """
 import argparse
 import json
@@ -72,6 +76,53 @@ def pair_launch_bytes(scs):
                 py, pz = cls & 1, cls >> 1
                 total += (cx + 1) * 2 * ((cy + 1 - py) // 2) * 2 * ((cz + 1 - pz) // 2)
     return 40.0 * total / 4
+
+
+def leftover_nodes(scs, pairs=True, merged=True):
+    """Nodes the node pass moves inside a frame when the last colour pair
+    finalizes and predicts its own: every node of a scene with contact or
+    tensile sweeps, the rows class 3 leaves (y = 0, z = 0 and the last ones on
+    odd sides) of the dense bodies of other scenes."""
+    total = 0
+    for sc in scs:
+        fold = not sc.contact_active and all(b.material.max_stretch <= 1.0 for b in sc.bodies)
+        for b in sc.bodies:
+            cx, cy, cz = b.lattice.cell_dims
+            dense = len(b.lattice.cells) == cx * cy * cz
+            n = b.lattice.node_count
+            if fold and dense and pairs and merged:
+                n -= (cx + 1) * 2 * (cy // 2) * 2 * (cz // 2)
+            total += n
+    return total
+
+
+def pass_rates(prof, scs, substeps, splats, pair_bytes, pairs, merged):
+    """Achieved algorithmic GB/s of each pass of one instrumented batched
+    frame (HIP events around each launch, so small launches read low), against
+    HBM peak. Contact figures are lower bounds: the bytes per examined slot
+    are left out."""
+    nodes = sum(s.n_nodes for s in scs)
+    records = sum(s.n_surface_active for s in scs)
+    per_launch = {
+        "predict": nodes * PREDICT_BYTES_PER_NODE,
+        "finalize": leftover_nodes(scs, pairs, merged) * FINALIZE_BYTES_PER_NODE,
+        "finalize+predict": leftover_nodes(scs, pairs, merged) * 52,
+        "elastic": pair_bytes,
+        "splat": splats * SPLAT_BYTES,
+        "bin": records * 40,
+        "sort": records * 4,
+        "collide": records * 44,
+        "settle": records * 44,
+    }
+    out = {}
+    for k, (ms, n) in prof.items():
+        if not n or k not in per_launch:
+            continue
+        us = ms / n * 1e3
+        gbs = per_launch[k] / (us * 1e-6) / 1e9
+        out[k] = {"launches": n, "avg_us": us, "bytes_per_launch": per_launch[k], "gb_s": gbs,
+                  "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+    return out
 
 
 def log(msg):
@@ -237,7 +288,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of the autotuner's pick")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--separate-node-passes", action="store_true",
                     help="run each substep's finalize and the next one's predict as two node passes (same "
@@ -307,7 +359,8 @@ def main():
     sync(world)
     log("[bench] scenes resident")
     tune = None
-    if args.autotune:
+    # The autotuner may pick either schedule; an explicit schedule flag keeps it.
+    if not (args.no_autotune or args.separate_colours or args.separate_node_passes):
         tune = json.loads(world.autotune(substeps))
         for c in tune["candidates"]:
             log(f"[bench] autotune {c}")
@@ -412,6 +465,7 @@ def main():
         "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
         "nodes": nodes, "cells": cells, "splats": splats, "substeps_per_frame": substeps,
         "profile_ms": {k: {"total": v[0], "launches": v[1]} for k, v in prof.items() if v[1]},
+        "passes": pass_rates(prof, scs, substeps, splats, el_bytes, pairs, not args.separate_node_passes),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      # counter figures are profiled on the default workload only
