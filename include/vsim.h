@@ -88,6 +88,10 @@ typedef struct vs_scene_params {
  * no tensile-limit sweeps between them): the last colour-pair pass may
  * finalize, and predict the next substep, for the nodes it writes. */
 #define VS_SCENE_FOLD 1u
+/* A scene with no kinematic drive (nothing between its predict and its first
+ * colour): at a frame's first substep the first colour-pair pass may predict
+ * the nodes it writes. */
+#define VS_SCENE_FOLD_PREDICT 2u
 
 /* One unit of work for a launch: `count` consecutive items from `begin` in the
  * pass's item array (nodes, cells, surface records, driven records, splats),

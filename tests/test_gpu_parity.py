@@ -270,7 +270,9 @@ def test_merged_node_passes_equal_separate_and_oracle(case):
         out.append(w.read_nodes(sid))
         prof = w.profile(sc.substeps)
         assert prof["finalize+predict"][1] == (0 if separate else sc.substeps - 1)
-        assert prof["predict"][1] == (sc.substeps if separate else 1)
+        # merged: one frame-start predict, or none when the first colour pair
+        # predicts every node (odd-sided dense lattices without drive)
+        assert prof["predict"][1] == sc.substeps if separate else prof["predict"][1] <= 1
         w.close()
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     if sc.drive is None:

@@ -478,6 +478,9 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
 // `node_mode` (class 3, the last colours, in a scene flagged VS_SCENE_FOLD):
 // after the colours, each node the lane writes is finalized (1, 2) and then
 // predicted for the next substep (1), as the node passes would do next.
+// `node_mode` 3 (class 0, the first colours, at a frame's first substep, in
+// a scene flagged VS_SCENE_FOLD_PREDICT): each node the lane holds is
+// predicted before the colours, as the predict pass would do first.
 // Four waves per SIMD: at most 128 VGPRs.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_elastic_rows(NodeView n, const vs_body *__restrict__ bodies,
@@ -541,6 +544,25 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                 w[2 * k] = w[2 * k + 1] = 0.f;
             }
         }
+        const bool pre = node_mode == 3u && (sc.flags & VS_SCENE_FOLD_PREDICT) != 0u;
+        if (on && pre) {
+            // predict.wgsl:19-36 per node with w != 0 (k_predict's
+            // arithmetic); the velocity it leaves is dead inside the frame.
+            const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                V3 v0, v1 = mk(0.f, 0.f, 0.f);
+                if (pair) {
+                    const F2 a = ld2(n.vx, off[k]), b = ld2(n.vy, off[k]), c = ld2(n.vz, off[k]);
+                    v0 = mk(a.x, b.x, c.x);
+                    v1 = mk(a.y, b.y, c.y);
+                } else {
+                    v0 = ld3(n.vx, n.vy, n.vz, off[k] / 4u);
+                }
+                if (w[2 * k] != 0.0f) predict_node(p[2 * k], q[2 * k], v0, gx, gy, gz, dt);
+                if (pair && w[2 * k + 1] != 0.0f) predict_node(p[2 * k + 1], q[2 * k + 1], v1, gx, gy, gz, dt);
+            }
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             lds[(3 * k + 0) * BLOCK + tid] = q[2 * k].x;
@@ -585,7 +607,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         // Node x0 is final in this lane's slot: lane m-1 wrote it back after
         // cell x0 - 1, or, at the row's start, this lane's own cell x0 did.
         // Node x0 + 1 is this lane's low corners.
-        const bool fold = node_mode != 0u && (sc.flags & VS_SCENE_FOLD) != 0u;
+        const bool fold = (node_mode == 1u || node_mode == 2u) && (sc.flags & VS_SCENE_FOLD) != 0u;
         if (on && !fold) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -601,6 +623,23 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                     n.px[e] = lx;
                     n.py[e] = ly;
                     n.pz[e] = lz;
+                }
+                if (pre) {
+                    // The start-of-substep positions the predict set: node x0's
+                    // in this lane's slot, node x0 + 1's in the low corners.
+                    const float ux = lds[(3 * k + 0) * BLOCK + tid];
+                    const float uy = lds[(3 * k + 1) * BLOCK + tid];
+                    const float uz = lds[(3 * k + 2) * BLOCK + tid];
+                    if (pair) {
+                        st2(n.qx, off[k], ux, q[2 * k].x);
+                        st2(n.qy, off[k], uy, q[2 * k].y);
+                        st2(n.qz, off[k], uz, q[2 * k].z);
+                    } else {
+                        const uint32_t e = off[k] / 4u;
+                        n.qx[e] = ux;
+                        n.qy[e] = uy;
+                        n.qz[e] = uz;
+                    }
                 }
             }
         } else if (on) {

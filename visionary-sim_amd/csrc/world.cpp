@@ -33,7 +33,8 @@ using vsim::SurfaceView;
 using vsim::TileView;
 
 constexpr uint32_t BUCKET_SLOTS = 32;  // contacts.wgsl:75, bodyContact.js:35
-constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, P_ROW = 12, P_REST = 16, NPASS = 17;
+constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, P_ROW = 12, P_REST = 16, P_REST0 = 17,
+              NPASS = 18;
 constexpr int NODE_PLANES = 13;        // p xyz, prev xyz, vel xyz, w, rest xyz
 constexpr int SURF_PLANES = 9;
 constexpr int DRIVE_PLANES = 4;
@@ -171,7 +172,8 @@ struct vs_world {
 
     uint32_t splat_mode = VS_SPLAT_COV;  // outputs of the splat pass being enqueued
     uint32_t block_node = 256, block_cells = 256, block_surf = 256, block_drive = 64, block_splat = 256;
-    uint32_t block_rest = 256;  // the node pass of P_REST: 64 when its runs are short (folded scenes)
+    uint32_t block_rest = 256;   // the node pass of P_REST: 64 when its runs are short (folded scenes)
+    uint32_t block_rest0 = 256;  // the frame's first predict over P_REST0, likewise
     uint32_t tile_nodes = 256, tile_cells = 256, tile_surf = 256, tile_drive = 64, tile_splat = 256;
 };
 
@@ -246,6 +248,7 @@ uint32_t pass_block(const vs_world *w, int p)
 {
     if (p == P_NODE) return w->block_node;
     if (p == P_REST) return w->block_rest;
+    if (p == P_REST0) return w->block_rest0;
     if ((p >= P_CELL && p < P_CELL + 8) || p >= P_ROW) return w->block_cells;
     if (p == P_SURF) return w->block_surf;
     if (p == P_DRIVE) return w->block_drive;
@@ -254,7 +257,7 @@ uint32_t pass_block(const vs_world *w, int p)
 
 uint32_t pass_tile(const vs_world *w, int p)
 {
-    if (p == P_NODE || p == P_REST) return w->tile_nodes;
+    if (p == P_NODE || p == P_REST || p == P_REST0) return w->tile_nodes;
     if (p >= P_CELL && p < P_CELL + 8) return w->tile_cells;
     if (p == P_SURF) return w->tile_surf;
     if (p == P_DRIVE) return w->tile_drive;
@@ -280,18 +283,29 @@ bool fold_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
     return !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD) && row_body(w, b);
 }
 
-// The nodes of a fold body its last colour pair does not write, as runs of
-// consecutive node ids: rows y = 1 .. 2*(cdy/2) and z = 1 .. 2*(cdz/2) of the
-// node grid are written (row class 3: cells with odd y and odd z), every
-// other node is left to the node pass.
+// A row body of a VS_SCENE_FOLD_PREDICT scene has the nodes its first colour
+// pair writes predicted inside that pass at a frame's first substep.
+bool fold_predict_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
+{
+    return !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD_PREDICT) && row_body(w, b);
+}
+
+// The nodes of a body outside node rows y = ylo..yhi and z = zlo..zhi (an
+// empty range when lo > hi), as runs of consecutive node ids. Row class c
+// writes the node rows y = (c & 1) .. (c & 1) + 2 ny_c - 1 (and likewise z):
+// class 3 (the last pair) y = 1 .. 2 floor(cdy/2), class 0 (the first)
+// y = 0 .. 2 ceil(cdy/2) - 1. The rest is left to a node pass.
 template <typename F>
-bool fold_leftovers(const BodyRec &b, F &&emit)
+bool leftover_runs(const BodyRec &b, uint32_t cls, F &&emit)
 {
     const uint64_t nx = b.cell_dims[0] + 1u, ny = b.cell_dims[1] + 1u, nz = b.cell_dims[2] + 1u;
-    const uint64_t ylast = 2u * (b.cell_dims[1] / 2u), zlast = 2u * (b.cell_dims[2] / 2u);
-    const bool none = ylast == 0 || zlast == 0;
+    const uint64_t py = cls & 1u, pz = cls >> 1;
+    const uint64_t nyr = (b.cell_dims[1] + 1u - py) / 2u, nzr = (b.cell_dims[2] + 1u - pz) / 2u;
+    const bool none = nyr == 0 || nzr == 0;
+    const uint64_t ylo = py, yhi = py + 2 * nyr - 1, zlo = pz, zhi = pz + 2 * nzr - 1;
     uint64_t run = 0, len = 0;  // pending run [run, run + len)
     auto add = [&](uint64_t first, uint64_t count) {
+        if (count == 0) return true;
         if (len && run + len == first) {
             len += count;
             return true;
@@ -303,12 +317,12 @@ bool fold_leftovers(const BodyRec &b, F &&emit)
     };
     bool ok = true;
     for (uint64_t z = 0; z < nz && ok; z++) {
-        if (none || z == 0 || z > zlast) {
+        if (none || z < zlo || z > zhi) {
             ok &= add(z * nx * ny, nx * ny);
             continue;
         }
-        ok &= add(z * nx * ny, nx);                        // y = 0
-        if (ylast + 1 < ny) ok &= add((z * ny + ylast + 1) * nx, (ny - ylast - 1) * nx);  // y > ylast
+        ok &= add(z * nx * ny, ylo * nx);                              // y < ylo
+        if (yhi + 1 < ny) ok &= add((z * ny + yhi + 1) * nx, (ny - yhi - 1) * nx);  // y > yhi
     }
     if (ok && len) ok = emit(b.node_begin + run, len);
     return ok;
@@ -342,12 +356,15 @@ int rebuild_tables(vs_world *w)
                 if (p == P_NODE) {
                     for (size_t b = 0; b < r.bodies.size(); b++)
                         ok &= emit(r.bodies[b].node_begin, r.bodies[b].node_count, (uint32_t)(r.body_begin + b));
-                } else if (p == P_REST) {
+                } else if (p == P_REST || p == P_REST0) {
+                    // P_REST: the nodes the last pair does not finalize;
+                    // P_REST0: those the first pair does not predict.
                     for (size_t b = 0; b < r.bodies.size() && ok; b++) {
                         const BodyRec &br = r.bodies[b];
                         const uint32_t id = (uint32_t)(r.body_begin + b);
-                        if (fold_body(w, r, br))
-                            ok &= fold_leftovers(br, [&](uint64_t first, uint64_t count) { return emit(first, count, id); });
+                        auto to = [&](uint64_t first, uint64_t count) { return emit(first, count, id); };
+                        if (p == P_REST ? fold_body(w, r, br) : fold_predict_body(w, r, br))
+                            ok &= leftover_runs(br, p == P_REST ? 3u : 0u, to);
                         else
                             ok &= emit(br.node_begin, br.node_count, id);
                     }
@@ -392,12 +409,12 @@ int rebuild_tables(vs_world *w)
             w->h_scene_counts.p[(size_t)p * S + s] = (uint32_t)(n - start);
         }
         w->tile_count[p] = (uint32_t)n;
-        if (p == P_REST) {
+        if (p == P_REST || p == P_REST0) {
             // Leftover runs of folded bodies are a few node rows each: a
             // 64-lane workgroup per run then wastes less than 256.
             uint64_t items = 0;
             for (uint64_t i = 0; i < n; i++) items += tiles[i].count;
-            w->block_rest = n && items < 128 * n ? 64u : 256u;
+            (p == P_REST ? w->block_rest : w->block_rest0) = n && items < 128 * n ? 64u : 256u;
         }
         w->h_tiles[p].p[0] = (uint32_t)n;
         w->h_tiles[p].p[1] = w->h_tiles[p].p[2] = w->h_tiles[p].p[3] = 0;
@@ -487,10 +504,13 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
     hipStream_t st = w->stream;
     Shape sh{};
     switch (kind) {
-    case VS_P_PREDICT:
-        sh = shape(P_NODE);
-        if (sh.grid) HIPW(w, launch_predict(st, sh, n, w->d_scenes, tile_view(w, P_NODE, scene)));
+    case VS_P_PREDICT: {
+        // color 1: only the nodes the first colour pair does not predict.
+        const int p = color ? P_REST0 : P_NODE;
+        sh = shape(p);
+        if (sh.grid) HIPW(w, launch_predict(st, sh, n, w->d_scenes, tile_view(w, p, scene)));
         return VS_OK;
+    }
     case VS_P_DRIVE:
         sh = shape(P_DRIVE);
         if (sh.grid) HIPW(w, launch_drive(st, sh, n, w->d_scenes, drive_view(w), tile_view(w, P_DRIVE, scene), -1));
@@ -565,7 +585,7 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
         const int p = kind == VS_P_ELASTIC ? elastic_table(color)
                       : kind == VS_P_DRIVE ? P_DRIVE
                       : kind == VS_P_SPLAT ? P_SPLAT
-                      : kind == VS_P_PREDICT ? P_NODE
+                      : kind == VS_P_PREDICT ? (color ? P_REST0 : P_NODE)
                       : (kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_REST
                                                                                  : P_SURF;
         const bool timed = prof && grid_for(w, p, scene) != 0;
@@ -581,18 +601,21 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
     // predict, so inside a frame the two share one node pass. In a scene
     // with nothing between its last colour and its finalize either, the last
     // colour-pair pass runs both for the nodes it writes, and the node pass
-    // takes the rest.
+    // takes the rest. At the frame's first substep, in a scene with no drive
+    // between predict and the first colour, the first colour-pair pass
+    // predicts the nodes it holds and the predict pass takes the rest.
     for (uint32_t s = 0; s < substeps; s++) {
         const bool last = s + 1 == substeps;
         if (s == 0 || !merge)
-            if (int rc = pass(VS_P_PREDICT, 0, 0)) return rc;
+            if (int rc = pass(VS_P_PREDICT, merge ? 1 : 0, 0)) return rc;
         if (int rc = pass(VS_P_DRIVE, 0, 0)) return rc;
         // Colours in order; row class c (colours 2c, 2c+1 of the row bodies)
         // goes with the two colour passes of the other bodies.
         for (int sweep = 0; sweep < (stretch ? 4 : 1); sweep++)
             for (int c = 0; c < 4; c++)
                 for (int e : { 8 + c, 2 * c, 2 * c + 1 }) {
-                    const uint32_t fold = merge && sweep == 0 && e == 11 ? (last ? 2u : 1u) : 0u;
+                    uint32_t fold = merge && sweep == 0 && e == 11 ? (last ? 2u : 1u) : 0u;
+                    if (merge && sweep == 0 && s == 0 && e == 8) fold = 3u;
                     if (int rc = pass(VS_P_ELASTIC, e, sweep > 0 ? 1u : 0u, fold)) return rc;
                 }
         if (contact)
@@ -747,6 +770,7 @@ static int world_init(vs_world *w)
     // (a slice's last rows join the next slice's first), plus one; a slice
     // holds at least 4 nodes.
     W->tile_cap[P_REST] = c.max_nodes / W->tile_nodes + c.max_nodes / 4 + 2 * c.max_bodies + 1;
+    W->tile_cap[P_REST0] = W->tile_cap[P_REST];
     W->tile_cap[P_SURF] = c.max_surface / W->tile_surf + c.max_scenes + 1;
     W->tile_cap[P_DRIVE] = c.max_driven / W->tile_drive + c.max_scenes + 1;
     W->tile_cap[P_SPLAT] = c.max_splats / W->tile_splat + c.max_bodies + 1;
@@ -985,7 +1009,7 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
     P.surf_base = (uint32_t)r.surf_begin;
     P.surf_count = (uint32_t)nsurf;
     P.bucket_slots = BUCKET_SLOTS;
-    P.flags = !r.contact && !r.stretch ? VS_SCENE_FOLD : 0u;
+    P.flags = (!r.contact && !r.stretch ? VS_SCENE_FOLD : 0u) | (d->n_driven == 0 ? VS_SCENE_FOLD_PREDICT : 0u);
 
     const size_t N = w->cfg.max_nodes;
     uint64_t node_at = r.node_begin, cell_at = r.cell_begin, splat_at = r.splat_begin, surf_at = r.surf_begin;
@@ -1222,7 +1246,7 @@ int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph
     std::vector<uint32_t> key = { substeps, mode, (uint32_t)w->any_stretch, (uint32_t)w->any_contact,
                                   w->cfg.separate_node_passes, w->pair_cap,
                                   w->block_cells,
-                                  w->cfg.xcd_chunked, w->cfg.separate_colours, w->block_rest };
+                                  w->cfg.xcd_chunked, w->cfg.separate_colours, w->block_rest, w->block_rest0 };
     for (int p = 0; p < NPASS; p++) key.push_back(grid_for(w, p, -1));
     if (!w->exec || key != w->graph_key) {
         if (w->exec) HIPW(w, hipGraphExecDestroy(w->exec));
