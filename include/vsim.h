@@ -96,10 +96,13 @@ typedef struct vs_scene_params {
 /* One unit of work for a launch: `count` consecutive items from `begin` in the
  * pass's item array (nodes, cells, surface records, driven records, splats),
  * all of one body (or, for contact and drive, one scene). A tile buffer is a
- * 16-byte header {count, 0, 0, 0} followed by the tiles. */
+ * 16-byte header {count, 0, 0, 0} followed by the tiles. In a colour-pair
+ * table a tile is `count` rows of cells from row `begin`, or, with
+ * VS_NODE_TILE set in `count`, a run of nodes from node `begin`. */
 typedef struct vs_tile {
     uint32_t begin, count, body, scene;
 } vs_tile;
+#define VS_NODE_TILE 0x80000000u
 
 /* ------------------------------------------------- layer 1: the backend */
 

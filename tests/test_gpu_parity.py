@@ -269,7 +269,10 @@ def test_merged_node_passes_equal_separate_and_oracle(case):
             w.frame(sc.substeps, False, f == 1)
         out.append(w.read_nodes(sid))
         prof = w.profile(sc.substeps)
-        assert prof["finalize+predict"][1] == (0 if separate else sc.substeps - 1)
+        # merged: one node pass a substep after the first, or none when every
+        # body's last colour pair finalizes and predicts all its nodes
+        fp = prof["finalize+predict"][1]
+        assert fp == 0 if separate else fp in (0, sc.substeps - 1)
         # merged: one frame-start predict, or none when the first colour pair
         # predicts every node (odd-sided dense lattices without drive)
         assert prof["predict"][1] == sc.substeps if separate else prof["predict"][1] <= 1

@@ -481,6 +481,10 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
 // `node_mode` 3 (class 0, the first colours, at a frame's first substep, in
 // a scene flagged VS_SCENE_FOLD_PREDICT): each node the lane holds is
 // predicted before the colours, as the predict pass would do first.
+//
+// A tile with VS_NODE_TILE set in its count is a run of nodes instead of
+// rows: the nodes of a folded body the class's rows leave out. It gets the
+// node arithmetic of `node_mode` alone (nothing when the mode is 0).
 // Four waves per SIMD: at most 128 VGPRs.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_elastic_rows(NodeView n, const vs_body *__restrict__ bodies,
@@ -498,6 +502,41 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         const vs_tile tile = tv.tiles[t];
         const vs_body bd = bodies[tile.body];
         const vs_scene_params &sc = scenes[bd.scene];
+        if (tile.count & VS_NODE_TILE) {
+            const bool fin = (node_mode == 1u || node_mode == 2u) && (sc.flags & VS_SCENE_FOLD) != 0u;
+            const bool pre = node_mode == 3u && (sc.flags & VS_SCENE_FOLD_PREDICT) != 0u;
+            if (limit_only || !(fin || pre)) continue;
+            const uint32_t count = tile.count & ~VS_NODE_TILE;
+            const Tank tk = tank_of(sc, bd);
+            const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
+            for (uint32_t i = tid; i < count; i += BLOCK) {
+                const uint32_t k = tile.begin + i;
+                if (n.w[k] == 0.0f) continue;
+                V3 p = ld3(n.px, n.py, n.pz, k), q, v;
+                if (pre) {
+                    // k_predict's arithmetic; the velocity is dead in the frame.
+                    v = ld3(n.vx, n.vy, n.vz, k);
+                    predict_node(p, q, v, gx, gy, gz, dt);
+                } else {
+                    q = ld3(n.qx, n.qy, n.qz, k);
+                    finalize_node(p, q, v, tk);
+                    if (node_mode == 1u) predict_node(p, q, v, gx, gy, gz, dt);
+                }
+                if (pre || node_mode == 1u) {
+                    n.qx[k] = q.x;
+                    n.qy[k] = q.y;
+                    n.qz[k] = q.z;
+                } else {
+                    n.vx[k] = v.x;
+                    n.vy[k] = v.y;
+                    n.vz[k] = v.z;
+                }
+                n.px[k] = p.x;
+                n.py[k] = p.y;
+                n.pz[k] = p.z;
+            }
+            continue;
+        }
         if (limit_only && !(bd.max_stretch > 1.0f)) continue;
         const CellConsts kc = cell_consts(bd, sc.dt);
         const uint32_t cdx = bd.cell_dims[0];

@@ -357,16 +357,14 @@ int rebuild_tables(vs_world *w)
                     for (size_t b = 0; b < r.bodies.size(); b++)
                         ok &= emit(r.bodies[b].node_begin, r.bodies[b].node_count, (uint32_t)(r.body_begin + b));
                 } else if (p == P_REST || p == P_REST0) {
-                    // P_REST: the nodes the last pair does not finalize;
-                    // P_REST0: those the first pair does not predict.
+                    // P_REST: the nodes of bodies whose last pair does not
+                    // finalize them; P_REST0: of those whose first pair does
+                    // not predict them (a folded body's leftover nodes go to
+                    // node tiles of that pair's launch).
                     for (size_t b = 0; b < r.bodies.size() && ok; b++) {
                         const BodyRec &br = r.bodies[b];
-                        const uint32_t id = (uint32_t)(r.body_begin + b);
-                        auto to = [&](uint64_t first, uint64_t count) { return emit(first, count, id); };
-                        if (p == P_REST ? fold_body(w, r, br) : fold_predict_body(w, r, br))
-                            ok &= leftover_runs(br, p == P_REST ? 3u : 0u, to);
-                        else
-                            ok &= emit(br.node_begin, br.node_count, id);
+                        if (p == P_REST ? fold_body(w, r, br) : fold_predict_body(w, r, br)) continue;
+                        ok &= emit(br.node_begin, br.node_count, (uint32_t)(r.body_begin + b));
                     }
                 } else if (p < P_CELL + 8) {
                     const int c = p - P_CELL;
@@ -384,14 +382,29 @@ int rebuild_tables(vs_world *w)
                         const uint64_t nz = (br.cell_dims[2] + 1u - (cls >> 1)) / 2u;
                         const uint64_t per = w->block_cells / (br.cell_dims[0] / 2u + 1u);
                         const uint64_t rows = ny * nz;
+                        const uint32_t id = (uint32_t)(r.body_begin + b);
                         for (uint64_t o = 0; o < rows; o += per) {
                             if (n >= w->tile_cap[p]) {
                                 ok = false;
                                 break;
                             }
-                            tiles[n++] = vs_tile{ (uint32_t)o, (uint32_t)std::min<uint64_t>(per, rows - o),
-                                                  (uint32_t)(r.body_begin + b), s };
+                            tiles[n++] = vs_tile{ (uint32_t)o, (uint32_t)std::min<uint64_t>(per, rows - o), id, s };
                         }
+                        // The nodes the first pair predicts (class 0) or the
+                        // last pair finalizes (class 3) that its rows leave
+                        // out, as node tiles of the same launch.
+                        const bool nodes = cls == 3 ? fold_body(w, r, br) : cls == 0 && fold_predict_body(w, r, br);
+                        if (ok && nodes)
+                            ok &= leftover_runs(br, cls, [&](uint64_t first, uint64_t count) {
+                                for (uint64_t o = 0; o < count; o += w->block_cells) {
+                                    if (n >= w->tile_cap[p]) return false;
+                                    tiles[n++] = vs_tile{ (uint32_t)(first + o),
+                                                          (uint32_t)std::min<uint64_t>(w->block_cells, count - o) |
+                                                              VS_NODE_TILE,
+                                                          id, s };
+                                }
+                                return true;
+                            });
                     }
                 } else if (p == P_SURF) {
                     if (r.contact) ok &= emit(r.surf_begin, r.surf_count, (uint32_t)r.body_begin);
@@ -778,7 +791,11 @@ static int world_init(vs_world *w)
     // colour table for all cells (tiles are 16 bytes, cheap to over-provision).
     for (int k = 0; k < 8; k++) W->tile_cap[P_CELL + k] = c.max_cells / 64 + c.max_bodies + 64;
     // A full row tile holds at least block/2 >= 32 cells of its class.
-    for (int k = 0; k < 4; k++) W->tile_cap[P_ROW + k] = c.max_cells / 32 + c.max_bodies + 64;
+    // Plus, for classes 0 and 3, node tiles of the rows' leftover nodes (at
+    // most one run per z-slice plus two per body, split at block_cells >= 64
+    // nodes; a slice holds at least 4 nodes).
+    for (int k = 0; k < 4; k++)
+        W->tile_cap[P_ROW + k] = c.max_cells / 32 + c.max_nodes / 64 + c.max_nodes / 4 + 3 * c.max_bodies + 64;
     for (int p = 0; p < NPASS; p++) {
         const size_t bytes = 16 + W->tile_cap[p] * sizeof(vs_tile);
         ALLOC(W->d_tiles[p], bytes);
