@@ -744,6 +744,19 @@ void o_substeps(o_scene *sc, int count)
  * (covariance.wgsl:30-42) for every splat. */
 void o_ellipsoid_axes(const REAL cov6[6], REAL rot[4], REAL scale3[3]);
 
+/* a + l * d, per component, one rounding each. */
+static inline v3 lerpf(v3 a, v3 d, real l) { return mk(FMA(l, d.x, a.x), FMA(l, d.y, a.y), FMA(l, d.z, a.z)); }
+
+/* deformationAt (sample.wgsl:40-80) and deformedCovariance
+ * (covariance.wgsl:18-42) per splat, in the evaluation order the GPU uses.
+ * The trilinear sum over the eight corners and its gradient are taken apart
+ * axis by axis (the same mathematics: the weight of corner (fx, fy, fz) is
+ * f_x f_y f_z with f = local or 1 - local, and its gradient the same product
+ * with one factor replaced by +-1/h): along x each corner pair gives its
+ * difference D and its interpolant P = p0 + lx D; along y the P give E and
+ * Q, the D give Dy; along z, Q gives the position and, with E and Dy, the
+ * three gradient columns. The rotation R(q) (xyzw), A = F R diag(s) and
+ * A A^T are sums of products, each fused. */
 void o_deform_splats(const o_scene *sc, const o_splats *sp)
 {
     for (int i = 0; i < sp->n; i++) {
@@ -763,43 +776,46 @@ void o_deform_splats(const o_scene *sc, const o_splats *sp)
             local[a] = grid - cell;
             base[a] = (uint32_t)cell;
         }
-        v3 pos = mk(0, 0, 0), col0 = mk(0, 0, 0), col1 = mk(0, 0, 0), col2 = mk(0, 0, 0);
-        for (int c = 0; c < 8; c++) {
-            int far[3] = { c & 1, (c >> 1) & 1, (c >> 2) & 1 };
-            real f[3], dir[3];
-            for (int a = 0; a < 3; a++) {
-                f[a] = far[a] ? local[a] : (real)1 - local[a];
-                dir[a] = far[a] ? (real)1 : (real)-1;
-            }
-            uint32_t ix = base[0] + (uint32_t)far[0], iy = base[1] + (uint32_t)far[1], iz = base[2] + (uint32_t)far[2];
-            uint32_t n = bd->node_offset + ix + bd->node_dims[0] * (iy + bd->node_dims[1] * iz);
-            v3 p = load3(sc->px, sc->py, sc->pz, n);
-            pos = add(pos, scale(p, (f[0] * f[1]) * f[2]));
-            v3 g = scale(mk((dir[0] * f[1]) * f[2], (dir[1] * f[0]) * f[2], (dir[2] * f[0]) * f[1]), inv);
-            col0 = add(col0, scale(p, g.x));
-            col1 = add(col1, scale(p, g.y));
-            col2 = add(col2, scale(p, g.z));
+        uint32_t nx = bd->node_dims[0], nxy = bd->node_dims[0] * bd->node_dims[1];
+        uint32_t n0 = bd->node_offset + base[0] + nx * base[1] + nxy * base[2];
+        v3 P[4], D[4]; /* j = fy + 2 fz */
+        for (int j = 0; j < 4; j++) {
+            uint32_t n = n0 + ((j & 1) ? nx : 0u) + ((j & 2) ? nxy : 0u);
+            v3 p0 = load3(sc->px, sc->py, sc->pz, n), p1 = load3(sc->px, sc->py, sc->pz, n + 1);
+            D[j] = sub(p1, p0);
+            P[j] = lerpf(p0, D[j], local[0]);
         }
+        v3 E[2], Q[2], Dy[2];
+        for (int fz = 0; fz < 2; fz++) {
+            E[fz] = sub(P[1 + 2 * fz], P[2 * fz]);
+            Q[fz] = lerpf(P[2 * fz], E[fz], local[1]);
+            Dy[fz] = lerpf(D[2 * fz], sub(D[1 + 2 * fz], D[2 * fz]), local[1]);
+        }
+        v3 Z = sub(Q[1], Q[0]);
+        v3 pos = lerpf(Q[0], Z, local[2]);
+        v3 col0 = scale(lerpf(Dy[0], sub(Dy[1], Dy[0]), local[2]), inv);
+        v3 col1 = scale(lerpf(E[0], sub(E[1], E[0]), local[2]), inv);
+        v3 col2 = scale(Z, inv);
         sp->ox[i] = pos.x;
         sp->oy[i] = pos.y;
         sp->oz[i] = pos.z;
 
         real x = sp->quat[i * 4], y = sp->quat[i * 4 + 1], z = sp->quat[i * 4 + 2], w = sp->quat[i * 4 + 3];
-        v3 r0 = mk((real)1 - (real)2 * (y * y + z * z), (real)2 * (x * y + z * w), (real)2 * (x * z - y * w));
-        v3 r1 = mk((real)2 * (x * y - z * w), (real)1 - (real)2 * (x * x + z * z), (real)2 * (y * z + x * w));
-        v3 r2 = mk((real)2 * (x * z + y * w), (real)2 * (y * z - x * w), (real)1 - (real)2 * (x * x + y * y));
+        v3 r0 = mk(FMA((real)-2, FMA(y, y, z * z), (real)1), (real)2 * FMA(x, y, z * w), (real)2 * FMA(x, z, -(y * w)));
+        v3 r1 = mk((real)2 * FMA(x, y, -(z * w)), FMA((real)-2, FMA(x, x, z * z), (real)1), (real)2 * FMA(y, z, x * w));
+        v3 r2 = mk((real)2 * FMA(x, z, y * w), (real)2 * FMA(y, z, -(x * w)), FMA((real)-2, FMA(x, x, y * y), (real)1));
         /* gradient * basis[k] = col0*b.x + col1*b.y + col2*b.z, then * scale_k */
-        v3 a0 = scale(add(add(scale(col0, r0.x), scale(col1, r0.y)), scale(col2, r0.z)), sp->sx[i]);
-        v3 a1 = scale(add(add(scale(col0, r1.x), scale(col1, r1.y)), scale(col2, r1.z)), sp->sy[i]);
-        v3 a2 = scale(add(add(scale(col0, r2.x), scale(col1, r2.y)), scale(col2, r2.z)), sp->sz[i]);
+        v3 a0 = scale(lerpf(lerpf(scale(col0, r0.x), col1, r0.y), col2, r0.z), sp->sx[i]);
+        v3 a1 = scale(lerpf(lerpf(scale(col0, r1.x), col1, r1.y), col2, r1.z), sp->sy[i]);
+        v3 a2 = scale(lerpf(lerpf(scale(col0, r2.x), col1, r2.y), col2, r2.z), sp->sz[i]);
         /* a * transpose(a): (r, c) = a0[r]a0[c] + a1[r]a1[c] + a2[r]a2[c] */
         size_t n = (size_t)sp->n;
-        sp->cov[0 * n + i] = (a0.x * a0.x + a1.x * a1.x) + a2.x * a2.x;
-        sp->cov[1 * n + i] = (a0.x * a0.y + a1.x * a1.y) + a2.x * a2.y;
-        sp->cov[2 * n + i] = (a0.x * a0.z + a1.x * a1.z) + a2.x * a2.z;
-        sp->cov[3 * n + i] = (a0.y * a0.y + a1.y * a1.y) + a2.y * a2.y;
-        sp->cov[4 * n + i] = (a0.y * a0.z + a1.y * a1.z) + a2.y * a2.z;
-        sp->cov[5 * n + i] = (a0.z * a0.z + a1.z * a1.z) + a2.z * a2.z;
+        sp->cov[0 * n + i] = FMA(a2.x, a2.x, FMA(a1.x, a1.x, a0.x * a0.x));
+        sp->cov[1 * n + i] = FMA(a2.x, a2.y, FMA(a1.x, a1.y, a0.x * a0.y));
+        sp->cov[2 * n + i] = FMA(a2.x, a2.z, FMA(a1.x, a1.z, a0.x * a0.z));
+        sp->cov[3 * n + i] = FMA(a2.y, a2.y, FMA(a1.y, a1.y, a0.y * a0.y));
+        sp->cov[4 * n + i] = FMA(a2.y, a2.z, FMA(a1.y, a1.z, a0.y * a0.z));
+        sp->cov[5 * n + i] = FMA(a2.z, a2.z, FMA(a1.z, a1.z, a0.z * a0.z));
         if (sp->rot) {
             REAL c6[6], q[4], s3[3];
             for (int k = 0; k < 6; k++) c6[k] = sp->cov[(size_t)k * n + i];

@@ -27,6 +27,8 @@ __device__ __forceinline__ V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, 
 __device__ __forceinline__ V3 mul(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 __device__ __forceinline__ V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
 __device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+// a + l * d, per component, one rounding each.
+__device__ __forceinline__ V3 lerpf(V3 a, V3 d, float l) { return mk(fmaf(l, d.x, a.x), fmaf(l, d.y, a.y), fmaf(l, d.z, a.z)); }
 
 // Kuhn tetrahedra (kuhn.js:28-35). Each is a path of three axis steps from
 // corner 0 to corner 7 (corner c = (c&1, (c>>1)&1, (c>>2)&1)), so
@@ -1162,7 +1164,11 @@ __global__ __launch_bounds__(BLOCK) void k_splats(const float *__restrict__ px, 
         const uint32_t ndx = bd.node_dims[0], ndxy = bd.node_dims[0] * bd.node_dims[1];
         for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
             const uint32_t k = tile.begin + i;
+            // Every input of the splat up front, so that its loads are in
+            // flight together (the outputs may alias them for the compiler).
             const float rest[3] = { sp.mx[k], sp.my[k], sp.mz[k] };
+            const float4 qv = sp.quat[k];
+            const float sx = sp.sx[k], sy = sp.sy[k], sz = sp.sz[k];
             float local[3];
             uint32_t base[3];
 #pragma unroll
@@ -1175,40 +1181,43 @@ __global__ __launch_bounds__(BLOCK) void k_splats(const float *__restrict__ px, 
                 base[a] = (uint32_t)cell;
             }
             const uint32_t n0 = bd.node_offset + base[0] + ndx * base[1] + ndxy * base[2];
-            V3 pos = mk(0.f, 0.f, 0.f), c0 = pos, c1 = pos, c2 = pos;
+            // The trilinear sum and its gradient axis by axis (the oracle's
+            // o_deform_splats order).
+            V3 Pj[4], Dj[4];  // j = fy + 2 fz
 #pragma unroll
-            for (int c = 0; c < 8; c++) {
-                const int fx = c & 1, fy = (c >> 1) & 1, fz = (c >> 2) & 1;
-                const float f0 = fx ? local[0] : 1.0f - local[0];
-                const float f1 = fy ? local[1] : 1.0f - local[1];
-                const float f2 = fz ? local[2] : 1.0f - local[2];
-                const uint32_t node = n0 + (uint32_t)fx + (fy ? ndx : 0u) + (fz ? ndxy : 0u);
-                const V3 p = ld3(px, py, pz, node);
-                pos = add(pos, mul(p, (f0 * f1) * f2));
-                // dir * f * f with dir = +-1: the sign is exact.
-                const float g0 = (fx ? f1 * f2 : -(f1 * f2)) * inv;
-                const float g1 = (fy ? f0 * f2 : -(f0 * f2)) * inv;
-                const float g2 = (fz ? f0 * f1 : -(f0 * f1)) * inv;
-                c0 = add(c0, mul(p, g0));
-                c1 = add(c1, mul(p, g1));
-                c2 = add(c2, mul(p, g2));
+            for (int j = 0; j < 4; j++) {
+                const uint32_t e = n0 + (j & 1 ? ndx : 0u) + (j & 2 ? ndxy : 0u);
+                const V3 p0 = ld3(px, py, pz, e), p1 = ld3(px, py, pz, e + 1u);
+                Dj[j] = sub(p1, p0);
+                Pj[j] = lerpf(p0, Dj[j], local[0]);
             }
+            V3 E[2], Q[2], Dy[2];
+#pragma unroll
+            for (int fz = 0; fz < 2; fz++) {
+                E[fz] = sub(Pj[1 + 2 * fz], Pj[2 * fz]);
+                Q[fz] = lerpf(Pj[2 * fz], E[fz], local[1]);
+                Dy[fz] = lerpf(Dj[2 * fz], sub(Dj[1 + 2 * fz], Dj[2 * fz]), local[1]);
+            }
+            const V3 Z = sub(Q[1], Q[0]);
+            const V3 pos = lerpf(Q[0], Z, local[2]);
+            const V3 c0 = mul(lerpf(Dy[0], sub(Dy[1], Dy[0]), local[2]), inv);
+            const V3 c1 = mul(lerpf(E[0], sub(E[1], E[0]), local[2]), inv);
+            const V3 c2 = mul(Z, inv);
             sp.ox[k] = pos.x;
             sp.oy[k] = pos.y;
             sp.oz[k] = pos.z;
-            const float4 qv = sp.quat[k];
             const float x = qv.x, y = qv.y, z = qv.z, w = qv.w;
-            const V3 r0 = mk(1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y + z * w), 2.0f * (x * z - y * w));
-            const V3 r1 = mk(2.0f * (x * y - z * w), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z + x * w));
-            const V3 r2 = mk(2.0f * (x * z + y * w), 2.0f * (y * z - x * w), 1.0f - 2.0f * (x * x + y * y));
-            const V3 a0 = mul(add(add(mul(c0, r0.x), mul(c1, r0.y)), mul(c2, r0.z)), sp.sx[k]);
-            const V3 a1 = mul(add(add(mul(c0, r1.x), mul(c1, r1.y)), mul(c2, r1.z)), sp.sy[k]);
-            const V3 a2 = mul(add(add(mul(c0, r2.x), mul(c1, r2.y)), mul(c2, r2.z)), sp.sz[k]);
+            const V3 r0 = mk(fmaf(-2.0f, fmaf(y, y, z * z), 1.0f), 2.0f * fmaf(x, y, z * w), 2.0f * fmaf(x, z, -(y * w)));
+            const V3 r1 = mk(2.0f * fmaf(x, y, -(z * w)), fmaf(-2.0f, fmaf(x, x, z * z), 1.0f), 2.0f * fmaf(y, z, x * w));
+            const V3 r2 = mk(2.0f * fmaf(x, z, y * w), 2.0f * fmaf(y, z, -(x * w)), fmaf(-2.0f, fmaf(x, x, y * y), 1.0f));
+            const V3 a0 = mul(lerpf(lerpf(mul(c0, r0.x), c1, r0.y), c2, r0.z), sx);
+            const V3 a1 = mul(lerpf(lerpf(mul(c0, r1.x), c1, r1.y), c2, r1.z), sy);
+            const V3 a2 = mul(lerpf(lerpf(mul(c0, r2.x), c1, r2.y), c2, r2.z), sz);
             const size_t P = sp.plane;
-            float d[3] = { (a0.x * a0.x + a1.x * a1.x) + a2.x * a2.x, (a0.y * a0.y + a1.y * a1.y) + a2.y * a2.y,
-                           (a0.z * a0.z + a1.z * a1.z) + a2.z * a2.z };
-            float o[3] = { (a0.x * a0.y + a1.x * a1.y) + a2.x * a2.y, (a0.y * a0.z + a1.y * a1.z) + a2.y * a2.z,
-                           (a0.x * a0.z + a1.x * a1.z) + a2.x * a2.z };
+            float d[3] = { fmaf(a2.x, a2.x, fmaf(a1.x, a1.x, a0.x * a0.x)), fmaf(a2.y, a2.y, fmaf(a1.y, a1.y, a0.y * a0.y)),
+                           fmaf(a2.z, a2.z, fmaf(a1.z, a1.z, a0.z * a0.z)) };
+            float o[3] = { fmaf(a2.x, a2.y, fmaf(a1.x, a1.y, a0.x * a0.y)), fmaf(a2.y, a2.z, fmaf(a1.y, a1.z, a0.y * a0.z)),
+                           fmaf(a2.x, a2.z, fmaf(a1.x, a1.z, a0.x * a0.z)) };
             if (sp.mode & VS_SPLAT_COV) {
                 sp.cov[0 * P + k] = d[0];
                 sp.cov[1 * P + k] = o[0];
