@@ -46,22 +46,30 @@ def _profile_rows(name, kernel):
     return [line.split("|") for line in open(path) if line.startswith(f"| {kernel}<")]
 
 
+def _most_dispatched(name, kernel, column):
+    """`column` of the row of `kernel` with the most dispatches in a committed
+    profile summary: the plain colour-pair launch of the frame (the first and
+    last pairs, which also run node passes, are fewer), as the back-to-back
+    timing measures it. None when the summary is absent."""
+    rows = [r for r in _profile_rows(name, kernel) if len(r) > column]
+    if not rows:
+        return None
+    return float(max(rows, key=lambda r: int(r[3]))[column])
+
+
 def elastic_traffic_bytes(kernel):
     """HBM bytes per batched launch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (profiles/batched256_hbm_traffic.md, scaled
-    by the copy calibration), averaged over the frame's launches, or None
-    when it is absent."""
-    rows = [(float(r[3]), float(r[6])) for r in _profile_rows("batched256_hbm_traffic.md", kernel) if len(r) > 6]
-    n = sum(d for d, _ in rows)
-    return sum(d * v for d, v in rows) / n * 1e6 if n else None
+    by the copy calibration)."""
+    v = _most_dispatched("batched256_hbm_traffic.md", kernel, 6)
+    return v * 1e6 if v is not None else None
 
 
 def elastic_valu_instructions(kernel):
     """VALU wave instructions per batched launch of `kernel` (rocprofv3
-    SQ_INSTS_VALU, profiles/batched256_valu.md), or None."""
-    rows = [(float(r[3]), float(r[4])) for r in _profile_rows("batched256_valu.md", kernel) if len(r) > 6]
-    n = sum(d for d, _ in rows)
-    return sum(d * v for d, v in rows) / n * 1e6 if n else None
+    SQ_INSTS_VALU, profiles/batched256_valu.md)."""
+    v = _most_dispatched("batched256_valu.md", kernel, 4)
+    return v * 1e6 if v is not None else None
 
 
 def pair_launch_bytes(scs):
