@@ -703,11 +703,13 @@ void o_finalize(o_scene *sc)
         sc->px[n] = p.x;
         sc->py[n] = p.y;
         sc->pz[n] = p.z;
-        real damp = damp_factor(bd->damping, sc->dt);
+        /* (p - prev) / dt * exp(-damping dt) as (p - prev) * (exp(-damping dt) / dt),
+         * the quotient rounded once: one product a component on the GPU. */
+        real rate = damp_factor(bd->damping, sc->dt) / dt;
         v3 v = sub(p, prev);
-        sc->vx[n] = (v.x / dt) * damp;
-        sc->vy[n] = (v.y / dt) * damp;
-        sc->vz[n] = (v.z / dt) * damp;
+        sc->vx[n] = v.x * rate;
+        sc->vy[n] = v.y * rate;
+        sc->vz[n] = v.z * rate;
     }
 }
 

@@ -276,18 +276,21 @@ __device__ __forceinline__ V3 plane(V3 p, V3 prev, float depth, float friction)
 }
 
 // The uniforms finalize reads for one body of one scene.
+// `rate` = exp(-damping dt) / dt, the f32 quotient of the two factors, formed
+// once per tile: the damped velocity is then one product a component.
 struct Tank {
-    float lx, ly, lz, hx, hz, fr, damp, dt;
+    float lx, ly, lz, hx, hz, fr, rate;
 };
 
 __device__ __forceinline__ Tank tank_of(const vs_scene_params &sc, const vs_body &b)
 {
-    return Tank{ sc.tank_min[0], sc.tank_min[1], sc.tank_min[2], sc.tank_max[0], sc.tank_max[2], b.friction, b.damp,
-                 sc.dt };
+    return Tank{ sc.tank_min[0], sc.tank_min[1], sc.tank_min[2], sc.tank_max[0], sc.tank_max[2], b.friction,
+                 b.damp / sc.dt };
 }
 
 // finalize.wgsl:44-74 for one node with w != 0: the five tank planes (no
-// ceiling), then the damped velocity.
+// ceiling), then the damped velocity (p - prev) / dt * exp(-damping dt), as
+// (p - prev) * rate (the oracle's o_finalize order).
 __device__ __forceinline__ void finalize_node(V3 &p, V3 prev, V3 &v, const Tank &k)
 {
     p = plane<1, 1>(p, prev, k.ly - p.y, k.fr);
@@ -295,7 +298,7 @@ __device__ __forceinline__ void finalize_node(V3 &p, V3 prev, V3 &v, const Tank 
     p = plane<0, -1>(p, prev, p.x - k.hx, k.fr);
     p = plane<2, 1>(p, prev, k.lz - p.z, k.fr);
     p = plane<2, -1>(p, prev, p.z - k.hz, k.fr);
-    v = mk(((p.x - prev.x) / k.dt) * k.damp, ((p.y - prev.y) / k.dt) * k.damp, ((p.z - prev.z) / k.dt) * k.damp);
+    v = mk((p.x - prev.x) * k.rate, (p.y - prev.y) * k.rate, (p.z - prev.z) * k.rate);
 }
 
 template <int BLOCK>
