@@ -98,7 +98,8 @@ typedef struct vs_scene_params {
  * all of one body (or, for contact and drive, one scene). A tile buffer is a
  * 16-byte header {count, 0, 0, 0} followed by the tiles. In a colour-pair
  * table a tile is `count` rows of cells from row `begin`, or, with
- * VS_NODE_TILE set in `count`, a run of nodes from node `begin`. */
+ * VS_NODE_TILE set in `count`, entries `begin` .. `begin + count - 1` of the
+ * body's list of the nodes those rows leave out (in node-id order). */
 typedef struct vs_tile {
     uint32_t begin, count, body, scene;
 } vs_tile;

@@ -95,6 +95,50 @@ struct Shape {
     uint32_t xcd;      // 1: contiguous tile runs per XCD
 };
 
+// The nodes of a dense body that row class `cls` leaves out, numbered in
+// node-id order. The class writes node rows y = ylo..yhi and z = zlo..zhi,
+// ylo = cls & 1, yhi = ylo + 2 nyr - 1 with nyr = (cdy + 1 - ylo) / 2 its
+// rows along y (likewise z); the rest is: the planes z < zlo (`lead` nodes),
+// then in each plane zlo..zhi the rows y < ylo (`rlo` nodes) and y > yhi,
+// then the planes z > zhi. With no rows (nyr or nzr 0) it is the whole body.
+// The node tiles of a colour-pair launch cover index ranges of this list, so
+// a tile is a full workgroup of leftover nodes however short their runs are.
+struct Leftover {
+    uint32_t nxy, lead, rlo, per, mid, total, zlo, skip, tail_at;
+
+    __host__ __device__ Leftover(const uint32_t cell_dims[3], uint32_t cls)
+    {
+        const uint32_t nx = cell_dims[0] + 1u, ny = cell_dims[1] + 1u, nz = cell_dims[2] + 1u;
+        const uint32_t py = cls & 1u, pz = cls >> 1;
+        const uint32_t nyr = (cell_dims[1] + 1u - py) / 2u, nzr = (cell_dims[2] + 1u - pz) / 2u;
+        nxy = nx * ny;
+        if (nyr == 0u || nzr == 0u) {
+            lead = total = nxy * nz;
+            rlo = per = mid = zlo = skip = tail_at = 0u;
+            return;
+        }
+        const uint32_t yhi = py + 2u * nyr - 1u, zhi = pz + 2u * nzr - 1u;
+        zlo = pz;
+        lead = pz * nxy;
+        rlo = py * nx;
+        per = rlo + (ny - 1u - yhi) * nx;
+        mid = (zhi - pz + 1u) * per;
+        skip = 2u * nyr * nx;  // a mid plane's rows y = ylo..yhi
+        tail_at = (zhi + 1u) * nxy;
+        total = lead + mid + (nz - 1u - zhi) * nxy;
+    }
+
+    // The body-relative node id of leftover node i < total.
+    __host__ __device__ uint32_t node(uint32_t i) const
+    {
+        if (i < lead) return i;
+        const uint32_t j = i - lead;
+        if (j >= mid) return tail_at + (j - mid);
+        const uint32_t plane = j / per, r = j - plane * per;
+        return (zlo + plane) * nxy + r + (r >= rlo ? skip : 0u);
+    }
+};
+
 // Host launchers; each enqueues one kernel on `stream`. They return the HIP
 // launch status.
 hipError_t launch_predict(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, TileView t);
@@ -109,7 +153,8 @@ hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bo
 // `count` rows of the class from row `begin` (row = a + ny*b for y = (cls&1)
 // + 2a, z = (cls>>1) + 2b, ny = the class's rows along y). `node_mode` 1
 // finalizes and predicts, 2 finalizes, the nodes written, in scenes flagged
-// VS_SCENE_FOLD.
+// VS_SCENE_FOLD; a VS_NODE_TILE tile is `count` entries of the body's
+// Leftover list from entry `begin`.
 hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
                                const vs_scene_params *scenes, TileView t, uint32_t cls, uint32_t limit_only,
                                uint32_t node_mode);

@@ -487,9 +487,10 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
 // a scene flagged VS_SCENE_FOLD_PREDICT): each node the lane holds is
 // predicted before the colours, as the predict pass would do first.
 //
-// A tile with VS_NODE_TILE set in its count is a run of nodes instead of
-// rows: the nodes of a folded body the class's rows leave out. It gets the
-// node arithmetic of `node_mode` alone (nothing when the mode is 0).
+// A tile with VS_NODE_TILE set in its count holds nodes instead of rows:
+// entries begin .. begin + count - 1 of the body's Leftover list (the nodes
+// of a folded body the class's rows leave out). It gets the node arithmetic
+// of `node_mode` alone (nothing when the mode is 0).
 // Four waves per SIMD: at most 128 VGPRs.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_elastic_rows(NodeView n, const vs_body *__restrict__ bodies,
@@ -512,10 +513,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             const bool pre = node_mode == 3u && (sc.flags & VS_SCENE_FOLD_PREDICT) != 0u;
             if (limit_only || !(fin || pre)) continue;
             const uint32_t count = tile.count & ~VS_NODE_TILE;
+            const Leftover rest(bd.cell_dims, cls);
             const Tank tk = tank_of(sc, bd);
             const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
             for (uint32_t i = tid; i < count; i += BLOCK) {
-                const uint32_t k = tile.begin + i;
+                const uint32_t k = bd.node_offset + rest.node(tile.begin + i);
                 if (n.w[k] == 0.0f) continue;
                 V3 p = ld3(n.px, n.py, n.pz, k), q, v;
                 if (pre) {

@@ -290,42 +290,12 @@ bool fold_predict_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
     return !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD_PREDICT) && row_body(w, b);
 }
 
-// The nodes of a body outside node rows y = ylo..yhi and z = zlo..zhi (an
-// empty range when lo > hi), as runs of consecutive node ids. Row class c
-// writes the node rows y = (c & 1) .. (c & 1) + 2 ny_c - 1 (and likewise z):
-// class 3 (the last pair) y = 1 .. 2 floor(cdy/2), class 0 (the first)
-// y = 0 .. 2 ceil(cdy/2) - 1. The rest is left to a node pass.
-template <typename F>
-bool leftover_runs(const BodyRec &b, uint32_t cls, F &&emit)
+// The nodes of a row body that class `cls` leaves to a node pass: class 3
+// (the last pair) writes node rows y = 1 .. 2 floor(cdy/2), class 0 (the
+// first) y = 0 .. 2 ceil(cdy/2) - 1, likewise along z (vsim::Leftover).
+uint32_t leftover_count(const BodyRec &b, uint32_t cls)
 {
-    const uint64_t nx = b.cell_dims[0] + 1u, ny = b.cell_dims[1] + 1u, nz = b.cell_dims[2] + 1u;
-    const uint64_t py = cls & 1u, pz = cls >> 1;
-    const uint64_t nyr = (b.cell_dims[1] + 1u - py) / 2u, nzr = (b.cell_dims[2] + 1u - pz) / 2u;
-    const bool none = nyr == 0 || nzr == 0;
-    const uint64_t ylo = py, yhi = py + 2 * nyr - 1, zlo = pz, zhi = pz + 2 * nzr - 1;
-    uint64_t run = 0, len = 0;  // pending run [run, run + len)
-    auto add = [&](uint64_t first, uint64_t count) {
-        if (count == 0) return true;
-        if (len && run + len == first) {
-            len += count;
-            return true;
-        }
-        const bool ok = len == 0 || emit(b.node_begin + run, len);
-        run = first;
-        len = count;
-        return ok;
-    };
-    bool ok = true;
-    for (uint64_t z = 0; z < nz && ok; z++) {
-        if (none || z < zlo || z > zhi) {
-            ok &= add(z * nx * ny, nx * ny);
-            continue;
-        }
-        ok &= add(z * nx * ny, ylo * nx);                              // y < ylo
-        if (yhi + 1 < ny) ok &= add((z * ny + yhi + 1) * nx, (ny - yhi - 1) * nx);  // y > yhi
-    }
-    if (ok && len) ok = emit(b.node_begin + run, len);
-    return ok;
+    return vsim::Leftover(b.cell_dims, cls).total;
 }
 
 // Rebuilds every tile table from the admitted scenes, in scene-id order.
@@ -392,19 +362,17 @@ int rebuild_tables(vs_world *w)
                         }
                         // The nodes the first pair predicts (class 0) or the
                         // last pair finalizes (class 3) that its rows leave
-                        // out, as node tiles of the same launch.
+                        // out, as node tiles of the same launch: full
+                        // workgroups of entries of the body's leftover list.
                         const bool nodes = cls == 3 ? fold_body(w, r, br) : cls == 0 && fold_predict_body(w, r, br);
-                        if (ok && nodes)
-                            ok &= leftover_runs(br, cls, [&](uint64_t first, uint64_t count) {
-                                for (uint64_t o = 0; o < count; o += w->block_cells) {
-                                    if (n >= w->tile_cap[p]) return false;
-                                    tiles[n++] = vs_tile{ (uint32_t)(first + o),
-                                                          (uint32_t)std::min<uint64_t>(w->block_cells, count - o) |
-                                                              VS_NODE_TILE,
-                                                          id, s };
-                                }
-                                return true;
-                            });
+                        const uint32_t left = ok && nodes ? leftover_count(br, cls) : 0u;
+                        for (uint32_t o = 0; o < left; o += w->block_cells) {
+                            if (n >= w->tile_cap[p]) {
+                                ok = false;
+                                break;
+                            }
+                            tiles[n++] = vs_tile{ o, std::min(w->block_cells, left - o) | VS_NODE_TILE, id, s };
+                        }
                     }
                 } else if (p == P_SURF) {
                     if (r.contact) ok &= emit(r.surf_begin, r.surf_count, (uint32_t)r.body_begin);
@@ -423,8 +391,8 @@ int rebuild_tables(vs_world *w)
         }
         w->tile_count[p] = (uint32_t)n;
         if (p == P_REST || p == P_REST0) {
-            // Leftover runs of folded bodies are a few node rows each: a
-            // 64-lane workgroup per run then wastes less than 256.
+            // Tables of small bodies (tiles of under 128 nodes on average)
+            // waste less with 64-lane workgroups than with 256.
             uint64_t items = 0;
             for (uint64_t i = 0; i < n; i++) items += tiles[i].count;
             (p == P_REST ? w->block_rest : w->block_rest0) = n && items < 128 * n ? 64u : 256u;
@@ -791,11 +759,10 @@ static int world_init(vs_world *w)
     // colour table for all cells (tiles are 16 bytes, cheap to over-provision).
     for (int k = 0; k < 8; k++) W->tile_cap[P_CELL + k] = c.max_cells / 64 + c.max_bodies + 64;
     // A full row tile holds at least block/2 >= 32 cells of its class.
-    // Plus, for classes 0 and 3, node tiles of the rows' leftover nodes (at
-    // most one run per z-slice plus two per body, split at block_cells >= 64
-    // nodes; a slice holds at least 4 nodes).
+    // Plus, for classes 0 and 3, node tiles of the rows' leftover nodes (a
+    // body's leftover list in pieces of block_cells >= 64 entries).
     for (int k = 0; k < 4; k++)
-        W->tile_cap[P_ROW + k] = c.max_cells / 32 + c.max_nodes / 64 + c.max_nodes / 4 + 3 * c.max_bodies + 64;
+        W->tile_cap[P_ROW + k] = c.max_cells / 32 + c.max_nodes / 64 + 2 * c.max_bodies + 64;
     for (int p = 0; p < NPASS; p++) {
         const size_t bytes = 16 + W->tile_cap[p] * sizeof(vs_tile);
         ALLOC(W->d_tiles[p], bytes);
