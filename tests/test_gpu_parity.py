@@ -194,6 +194,46 @@ def test_batched_equals_alone_and_graph_equals_eager():
         assert np.array_equal(alone.read_nodes(a)[0], got)
 
 
+def test_autotune_leaves_state_and_results_unchanged():
+    """vs_world_autotune times every launch-shape candidate from the same
+    snapshot of the node planes and puts the snapshot back: the scenes are
+    where they were, and frames run with the shape it picks equal frames run
+    with the default shape (a dense block, a sparse lattice, a contact scene
+    with a stretch limit, a driven block)."""
+    def mixed():
+        mix = [scenes.jelly_block(s, splats=200, seed=s, name=f"t{s}") for s in (6, 11)]
+        mix[1].drive = scenes.handle_drive(mix[1], seed=3)
+        mix.append(_sparse_scene(5))
+        c = scenes.c3_scene(bodies=8, seed=9)
+        for b in c.bodies:
+            b.material.max_stretch = 1.05
+        mix.append(c)
+        return mix
+
+    out = []
+    for tune in (False, True):
+        mix = mixed()
+        w = world_for(*mix)
+        ids = [w.add(s) for s in mix]
+        w.frame(4, splats=True, graph=False)
+        before = [w.read_nodes(i) for i in ids]
+        if tune:
+            report = w.autotune(4)
+            print(report[:200])
+            after = [w.read_nodes(i) for i in ids]
+            for (p, v), (p2, v2) in zip(before, after):
+                assert np.array_equal(p, p2) and np.array_equal(v, v2)
+        for f in range(3):
+            for sid, sc in zip(ids, mix):
+                if sc.drive is not None:
+                    w.set_target(sid, *sc.drive.frame_target(f, 4 * sc.dt))
+            w.frame(4, splats=True, graph=(f > 0))
+        out.append([w.read_nodes(i)[0] for i in ids] + [w.read_splats(i)[1] for i in ids])
+        w.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def serve(requests, max_nodes, realtime=False):
     nodes = max_nodes
     cells = max(s.n_cells for s, _ in requests) * 4

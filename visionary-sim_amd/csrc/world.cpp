@@ -747,10 +747,9 @@ static int world_init(vs_world *w)
     W->splats.init(c.max_splats);
     // A table needs a tile per `tile` items plus one partial tile per body.
     W->tile_cap[P_NODE] = c.max_nodes / W->tile_nodes + c.max_bodies + 1;
-    // Leftover runs of a fold body: at most one per z-slice of its node grid
-    // (a slice's last rows join the next slice's first), plus one; a slice
-    // holds at least 4 nodes.
-    W->tile_cap[P_REST] = c.max_nodes / W->tile_nodes + c.max_nodes / 4 + 2 * c.max_bodies + 1;
+    // P_REST and P_REST0 list whole bodies (those whose colour pairs do not
+    // take their node passes), like P_NODE.
+    W->tile_cap[P_REST] = c.max_nodes / W->tile_nodes + c.max_bodies + 1;
     W->tile_cap[P_REST0] = W->tile_cap[P_REST];
     W->tile_cap[P_SURF] = c.max_surface / W->tile_surf + c.max_scenes + 1;
     W->tile_cap[P_DRIVE] = c.max_driven / W->tile_drive + c.max_scenes + 1;
@@ -1498,8 +1497,10 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
     if (!w || substeps == 0) return VS_E_INVALID;
     HIPW(w, hipSetDevice(w->cfg.device));
     if (int rc = prepare(w)) return rc;
-    // The search steps the live scenes; snapshot the node planes and put them
-    // back afterwards so tuning leaves the simulation where it was.
+    // The search steps the live scenes; snapshot the node planes, start every
+    // candidate from them (so each is timed on the same state, not on where
+    // the previous candidates left the scenes), and put them back afterwards
+    // so tuning leaves the simulation where it was.
     const size_t bytes = sizeof(float) * 10 * w->cfg.max_nodes;
     void *snap = nullptr;
     HIPW(w, hipMalloc(&snap, bytes));
@@ -1532,6 +1533,7 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
     auto time = [&](Cand &c) -> int {
         apply(c);
         if (int rc = prepare(w)) return rc;
+        HIPW(w, hipMemcpyAsync(w->d_nodes, snap, bytes, hipMemcpyDeviceToDevice, w->stream));
         if (int rc = enqueue_frame(w, substeps, 0, -1)) return rc;  // warm-up
         hipEvent_t a = w->tune[0], b = w->tune[1];
         HIPW(w, hipEventRecord(a, w->stream));
