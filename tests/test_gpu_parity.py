@@ -87,6 +87,36 @@ def test_contact_keys_and_pairs_bit_exact():
     assert counts.sum() > 0
 
 
+@pytest.mark.parametrize("count,side,bucket", [(4, 6, 0.05), (6, 8, 0.06), (8, 8, 0.08)])
+def test_overfull_buckets_bit_exact(count, side, bucket):
+    """Buckets several voxels wide hold up to 86-199 records: each overfull
+    bucket keeps its 32 lowest record ids, ascending (sort.wgsl:27-37; on the
+    GPU one wave scans the scene's records for it). Keys, accepted pairs and
+    positions against the oracle over three frames."""
+    from oracle.oracle import OracleScene
+    sc = scenes.contact_stack(count, side=side)
+    sc.bucket_size = bucket
+    w = world_for(sc)
+    w.log_pairs(128)
+    sid = w.add(sc)
+    o = OracleScene(sc, "f32", pair_cap=128)
+    overfull = 0
+    for f in range(3):
+        w.frame(sc.substeps, False, f > 0)
+        o.frame(sc.substeps)
+        keys, counts, pairs = w.read_contacts(sid)
+        assert np.array_equal(keys, o.keys)
+        assert np.array_equal(counts, o.pair_count)
+        for s in np.nonzero(counts)[0]:
+            n = min(counts[s], 128)
+            assert np.array_equal(pairs[s, :n], o.pairs[s, :n])
+        overfull += int((np.bincount(keys) > 32).sum())
+        pos, vel = w.read_nodes(sid)
+        assert rel_err(pos, o.p) <= RTOL and rel_err(vel, o.v) <= RTOL
+    print(f"overfull buckets at frame ends: {overfull}; exact {np.mean(pos == o.p):.6f}")
+    assert overfull > 0
+
+
 def test_contact_many_bodies_one_substep():
     from oracle.oracle import OracleScene
     sc = scenes.c3_scene(bodies=64)

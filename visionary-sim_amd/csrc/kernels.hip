@@ -798,7 +798,41 @@ __global__ __launch_bounds__(BLOCK) void k_bin(NodeView n, const vs_scene_params
     }
 }
 
-// sort.wgsl:14-53, run by the record that arrived first in each bucket.
+// An overfull bucket `b` keeps the lowest `slots` record ids of the scene's
+// records in it, ascending (sort.wgsl:27-37): the same scan from the scene's
+// first record upwards, taken by a whole wave, 4 x 64 keys a round. Lane l
+// reads record o0 + 64u + l; a ballot marks the matches and each match's
+// place is the count of matches before it in record order, so the slots and
+// their order are those of a one-record-at-a-time scan. Called by every lane
+// of the wave with the same `b` (the loop bounds are wave-uniform).
+__device__ __forceinline__ void overflow_scan(const vs_scene_params &sc, const uint32_t *__restrict__ key,
+                                              uint32_t *base, uint32_t b, uint32_t slots)
+{
+    constexpr uint32_t U = 4;
+    const uint32_t lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t end = sc.surf_base + sc.surf_count;
+    uint32_t kept = 0;
+    for (uint32_t o0 = sc.surf_base; o0 < end && kept < slots; o0 += 64u * U) {
+        uint32_t k[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t o = o0 + 64u * u + lane;
+            k[u] = o < end ? key[o] : ~0u;  // bucket keys are grid indices, never ~0
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const bool m = k[u] == b;
+            const uint64_t mask = __ballot(m);
+            const uint32_t pos = kept + (uint32_t)__popcll(mask & below);
+            if (m && pos < slots) base[pos] = o0 + 64u * u + lane;
+            kept += (uint32_t)__popcll(mask);
+        }
+    }
+}
+
+// sort.wgsl:14-53, run by the record that arrived first in each bucket; an
+// overfull bucket's scan is run by that record's whole wave.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restrict__ scenes, SurfaceView sv, GridView g,
                                                 TileView tv)
@@ -808,31 +842,32 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restric
         const vs_tile tile = tv.tiles[t];
         const vs_scene_params &sc = scenes[tile.scene];
         const uint32_t slots = sc.bucket_slots, gb = sc.grid_base;
-        for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
-            const uint32_t s = tile.begin + i;
-            if (sv.rank[s] != 0u) continue;
-            const uint32_t b = sv.key[s];
-            const uint32_t held = g.count[gb + b];
-            uint32_t *base = g.slots + (size_t)(gb + b) * slots;
-            if (held > slots) {
-                // Overflow: keep the lowest `slots` record ids of the bucket.
-                uint32_t kept = 0;
-                const uint32_t end = sc.surf_base + sc.surf_count;
-                for (uint32_t o = sc.surf_base; o < end && kept < slots; o++) {
-                    if (sv.key[o] == b) base[kept++] = o;
+        // Every lane runs every round (the rounds are uniform over the
+        // workgroup), so the ballots below see the whole wave.
+        for (uint32_t i0 = 0; i0 < tile.count; i0 += BLOCK) {
+            const uint32_t i = i0 + threadIdx.x;
+            uint32_t b = 0u;
+            bool over = false;
+            if (i < tile.count && sv.rank[tile.begin + i] == 0u) {
+                b = sv.key[tile.begin + i];
+                const uint32_t held = g.count[gb + b];
+                uint32_t *base = g.slots + (size_t)(gb + b) * slots;
+                over = held > slots;
+                for (uint32_t a = 1; a < held && !over; a++) {
+                    const uint32_t value = base[a];
+                    uint32_t at = a;
+                    while (at > 0u) {
+                        const uint32_t previous = base[at - 1u];
+                        if (previous <= value) break;
+                        base[at] = previous;
+                        at--;
+                    }
+                    base[at] = value;
                 }
-                continue;
             }
-            for (uint32_t a = 1; a < held; a++) {
-                const uint32_t value = base[a];
-                uint32_t at = a;
-                while (at > 0u) {
-                    const uint32_t previous = base[at - 1u];
-                    if (previous <= value) break;
-                    base[at] = previous;
-                    at--;
-                }
-                base[at] = value;
+            for (uint64_t pending = __ballot(over); pending; pending &= pending - 1ull) {
+                const uint32_t bb = (uint32_t)__shfl((int)b, (int)__builtin_ctzll(pending));
+                overflow_scan(sc, sv.key, g.slots + (size_t)(gb + bb) * slots, bb, slots);
             }
         }
     }
