@@ -798,41 +798,52 @@ __global__ __launch_bounds__(BLOCK) void k_bin(NodeView n, const vs_scene_params
     }
 }
 
-// An overfull bucket `b` keeps the lowest `slots` record ids of the scene's
-// records in it, ascending (sort.wgsl:27-37): the same scan from the scene's
-// first record upwards, taken by a whole wave, 4 x 64 keys a round. Lane l
-// reads record o0 + 64u + l; a ballot marks the matches and each match's
-// place is the count of matches before it in record order, so the slots and
-// their order are those of a one-record-at-a-time scan. Called by every lane
-// of the wave with the same `b` (the loop bounds are wave-uniform).
+// Overfull buckets keep the lowest `slots` record ids of the scene's records
+// in them, ascending (sort.wgsl:27-37): the reference's scan from the scene's
+// first record upwards, taken by a whole wave for every overfull bucket its
+// lanes hold at once (lane j's bucket b, for each bit j of `pending`), 16 x 64
+// keys a round (16 loads in flight a lane). Lane l reads record o0 + 64u + l;
+// per bucket a ballot marks the matches and each match's place is the count
+// of matches before it in record order, so the slots and their order are
+// those of a one-record-at-a-time scan. A bucket leaves the scan once it holds
+// `slots`. `grid` is the scene's slot array; every lane of the wave calls this
+// with the same `pending` (the loop bounds are wave-uniform).
 __device__ __forceinline__ void overflow_scan(const vs_scene_params &sc, const uint32_t *__restrict__ key,
-                                              uint32_t *base, uint32_t b, uint32_t slots)
+                                              uint32_t *grid, uint32_t b, uint64_t pending, uint32_t slots)
 {
-    constexpr uint32_t U = 4;
+    constexpr uint32_t U = 16;
     const uint32_t lane = __lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t end = sc.surf_base + sc.surf_count;
-    uint32_t kept = 0;
-    for (uint32_t o0 = sc.surf_base; o0 < end && kept < slots; o0 += 64u * U) {
+    uint32_t kept = 0;  // lane j: bucket b's matches so far
+    for (uint32_t o0 = sc.surf_base; o0 < end && pending; o0 += 64u * U) {
         uint32_t k[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t o = o0 + 64u * u + lane;
             k[u] = o < end ? key[o] : ~0u;  // bucket keys are grid indices, never ~0
         }
+        for (uint64_t left = pending; left; left &= left - 1ull) {
+            const int j = __builtin_ctzll(left);
+            const uint32_t bb = (uint32_t)__shfl((int)b, j);
+            uint32_t kj = (uint32_t)__shfl((int)kept, j);
+            uint32_t *base = grid + (size_t)bb * slots;
 #pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-            const bool m = k[u] == b;
-            const uint64_t mask = __ballot(m);
-            const uint32_t pos = kept + (uint32_t)__popcll(mask & below);
-            if (m && pos < slots) base[pos] = o0 + 64u * u + lane;
-            kept += (uint32_t)__popcll(mask);
+            for (uint32_t u = 0; u < U; u++) {
+                const bool m = k[u] == bb;
+                const uint64_t mask = __ballot(m);
+                const uint32_t pos = kj + (uint32_t)__popcll(mask & below);
+                if (m && pos < slots) base[pos] = o0 + 64u * u + lane;
+                kj += (uint32_t)__popcll(mask);
+            }
+            if (lane == (uint32_t)j) kept = kj;
+            if (kj >= slots) pending &= ~(1ull << j);
         }
     }
 }
 
-// sort.wgsl:14-53, run by the record that arrived first in each bucket; an
-// overfull bucket's scan is run by that record's whole wave.
+// sort.wgsl:14-53, run by the record that arrived first in each bucket; the
+// overfull buckets' scan is run by those records' wave.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restrict__ scenes, SurfaceView sv, GridView g,
                                                 TileView tv)
@@ -865,10 +876,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restric
                     base[at] = value;
                 }
             }
-            for (uint64_t pending = __ballot(over); pending; pending &= pending - 1ull) {
-                const uint32_t bb = (uint32_t)__shfl((int)b, (int)__builtin_ctzll(pending));
-                overflow_scan(sc, sv.key, g.slots + (size_t)(gb + bb) * slots, bb, slots);
-            }
+            const uint64_t pending = __ballot(over);
+            if (pending) overflow_scan(sc, sv.key, g.slots + (size_t)gb * slots, b, pending, slots);
         }
     }
 }
