@@ -35,6 +35,9 @@ using vsim::TileView;
 constexpr uint32_t BUCKET_SLOTS = 32;  // contacts.wgsl:75, bodyContact.js:35
 constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, P_ROW = 12, P_REST = 16, P_REST0 = 17,
               NPASS = 18;
+
+// The four colour-pair (row class) tables.
+constexpr bool is_row_pass(int p) { return p >= P_ROW && p < P_ROW + 4; }
 constexpr int NODE_PLANES = 13;        // p xyz, prev xyz, vel xyz, w, rest xyz
 constexpr int SURF_PLANES = 9;
 constexpr int DRIVE_PLANES = 4;
@@ -153,6 +156,11 @@ struct vs_world {
     uint64_t tile_cap[NPASS] = {};
     uint32_t *d_scene_counts = nullptr;  // [NPASS][max_scenes]
     HostPinned<uint32_t> h_tiles[NPASS];
+    // The colour-pair tables once more, in the order the batched launches
+    // take them (rows of the most lanes first, node tiles last); the tables
+    // above keep each scene's tiles together for the per-scene launches.
+    uint32_t *d_rows_by_work[4] = {};
+    HostPinned<uint32_t> h_rows_by_work[4];
     HostPinned<uint32_t> h_scene_counts;
     std::vector<uint32_t> tile_count;                 // [NPASS]
     std::vector<std::vector<uint32_t>> scene_first;   // [NPASS][max_scenes]
@@ -401,6 +409,22 @@ int rebuild_tables(vs_world *w)
         w->h_tiles[p].p[1] = w->h_tiles[p].p[2] = w->h_tiles[p].p[3] = 0;
         HIPW(w, hipMemcpyAsync(w->d_tiles[p], w->h_tiles[p].p, 16 + n * sizeof(vs_tile), hipMemcpyHostToDevice,
                                w->stream));
+        if (is_row_pass(p)) {
+            // Tiles of one launch touch disjoint nodes, so their order is
+            // free: the batched launch dispatches the rows of the most lanes
+            // first and the short tiles (a body's last rows, node tiles)
+            // last, where they fill the launch's tail.
+            uint32_t *h = w->h_rows_by_work[p - P_ROW].p;
+            vs_tile *sorted = reinterpret_cast<vs_tile *>(h + 4);
+            std::memcpy(h, w->h_tiles[p].p, 16 + n * sizeof(vs_tile));
+            auto lanes = [&](const vs_tile &t) -> uint64_t {
+                if (t.count & VS_NODE_TILE) return 0;
+                return (uint64_t)t.count * (w->h_bodies[t.body].cell_dims[0] / 2u + 1u);
+            };
+            std::stable_sort(sorted, sorted + n, [&](const vs_tile &a, const vs_tile &b) { return lanes(a) > lanes(b); });
+            HIPW(w, hipMemcpyAsync(w->d_rows_by_work[p - P_ROW], h, 16 + n * sizeof(vs_tile), hipMemcpyHostToDevice,
+                                   w->stream));
+        }
     }
     HIPW(w, hipMemcpyAsync(w->d_scene_counts, w->h_scene_counts.p, sizeof(uint32_t) * NPASS * S,
                            hipMemcpyHostToDevice, w->stream));
@@ -438,6 +462,10 @@ int prepare(vs_world *w)
 // A tile view over the whole table (scene < 0) or one scene's slice of it.
 TileView tile_view(vs_world *w, int p, int scene)
 {
+    if (scene < 0 && is_row_pass(p)) {
+        uint32_t *d = w->d_rows_by_work[p - P_ROW];
+        return TileView{ d, reinterpret_cast<const vs_tile *>(d + 4) };
+    }
     const vs_tile *tiles = reinterpret_cast<const vs_tile *>(w->d_tiles[p] + 4);
     if (scene < 0) return TileView{ w->d_tiles[p], tiles };
     return TileView{ w->d_scene_counts + (size_t)p * w->cfg.max_scenes + scene, tiles + w->scene_first[p][scene] };
@@ -768,6 +796,13 @@ static int world_init(vs_world *w)
         HIPW(W, hipHostMalloc(reinterpret_cast<void **>(&W->h_tiles[p].p), bytes, hipHostMallocDefault));
         std::memset(W->h_tiles[p].p, 0, 16);
         HIPW(W, hipMemset(W->d_tiles[p], 0, 16));
+        if (is_row_pass(p)) {
+            ALLOC(W->d_rows_by_work[p - P_ROW], bytes);
+            HIPW(W, hipHostMalloc(reinterpret_cast<void **>(&W->h_rows_by_work[p - P_ROW].p), bytes,
+                                  hipHostMallocDefault));
+            std::memset(W->h_rows_by_work[p - P_ROW].p, 0, 16);
+            HIPW(W, hipMemset(W->d_rows_by_work[p - P_ROW], 0, 16));
+        }
     }
     HIPW(W, hipHostMalloc(reinterpret_cast<void **>(&W->h_scene_counts.p), sizeof(uint32_t) * NPASS * c.max_scenes,
                           hipHostMallocDefault));
@@ -813,6 +848,8 @@ void vs_world_destroy(vs_world *w)
         if (p) (void)hipFree(p);
     for (int p = 0; p < NPASS; p++)
         if (w->d_tiles[p]) (void)hipFree(w->d_tiles[p]);
+    for (uint32_t *d : w->d_rows_by_work)
+        if (d) (void)hipFree(d);
     if (w->stream) (void)hipStreamDestroy(w->stream);
     if (w->upload) (void)hipStreamDestroy(w->upload);
     delete w;
