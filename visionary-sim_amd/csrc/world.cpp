@@ -462,7 +462,9 @@ int prepare(vs_world *w)
 // A tile view over the whole table (scene < 0) or one scene's slice of it.
 TileView tile_view(vs_world *w, int p, int scene)
 {
-    if (scene < 0 && is_row_pass(p)) {
+    // XCD-chunked launches give each XCD one contiguous run of the table, so
+    // they keep the scene order (a region stays on one XCD from pass to pass).
+    if (scene < 0 && is_row_pass(p) && !w->cfg.xcd_chunked) {
         uint32_t *d = w->d_rows_by_work[p - P_ROW];
         return TileView{ d, reinterpret_cast<const vs_tile *>(d + 4) };
     }
