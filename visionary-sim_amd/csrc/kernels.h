@@ -42,9 +42,17 @@ struct SurfaceView {
 };
 
 // contactGrid (bodyContact.js:125-127): counters, then slots per bucket.
+// Optional (not in the reference's grid): `ovf`, the records that arrive at a
+// full bucket, as (record, grid bucket) in arrival order, `ovf_count` of them
+// this substep (appends past `ovf_cap` are dropped: sort then finds fewer
+// members than the bucket counted and rescans); sort finds an overfull
+// bucket's members there instead of rescanning the scene's records.
 struct GridView {
     uint32_t *count;
     uint32_t *slots;
+    uint2 *ovf = nullptr;
+    uint32_t *ovf_count = nullptr;
+    uint32_t ovf_cap = 0;
 };
 
 // Optional log of the pairs collide accepts (record, mate), `cap` per record.

@@ -794,6 +794,10 @@ __global__ __launch_bounds__(BLOCK) void k_bin(NodeView n, const vs_scene_params
             const uint32_t slot = atomicAdd(&g.count[gb + b], 1u);
             sv.rank[s] = slot;
             if (slot < slots) g.slots[(size_t)(gb + b) * slots + slot] = s;
+            else if (g.ovf) {
+                const uint32_t at = atomicAdd(g.ovf_count, 1u);
+                if (at < g.ovf_cap) g.ovf[at] = make_uint2(s, gb + b);
+            }
         }
     }
 }
@@ -842,8 +846,73 @@ __device__ __forceinline__ void overflow_scan(const vs_scene_params &sc, const u
     }
 }
 
+// The wave-wide minimum of v.
+__device__ __forceinline__ uint32_t wave_min(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d));
+    return v;
+}
+
+// An overfull bucket (grid bucket `gbk`, `held` records) from its members:
+// the `slots` that arrived first (in `base`) and the records bin listed in
+// g.ovf. Each lane keeps the members it reads (at most K), then `slots`
+// rounds of wave minimum write the lowest ids ascending: the slots the
+// scene-wide scan would give. Returns false, having written nothing, when a
+// lane would hold more than K members or the members found are not `held`
+// (the caller then scans). Called by every lane of the wave with the same
+// bucket.
+__device__ __forceinline__ bool overflow_members(GridView g, uint32_t *base, uint32_t gbk, uint32_t held,
+                                                 uint32_t slots)
+{
+    constexpr uint32_t K = 8, U = 8;
+    const uint32_t lane = __lane_id();
+    if (slots > 64u || held > 64u * K) return false;
+    uint32_t mem[K], n = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) mem[k] = ~0u;
+    if (lane < slots) {
+        mem[0] = base[lane];
+        n = 1;
+    }
+    bool spill = false;
+    const uint32_t total = min(*g.ovf_count, g.ovf_cap);
+    for (uint32_t o0 = 0; o0 < total; o0 += 64u * U) {
+        uint2 e[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t o = o0 + 64u * u + lane;
+            e[u] = o < total ? g.ovf[o] : make_uint2(0u, ~0u);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            if (e[u].y != gbk) continue;
+#pragma unroll
+            for (uint32_t k = 0; k < K; k++)
+                if (k == n) mem[k] = e[u].x;
+            spill |= n >= K;
+            n++;
+        }
+    }
+    uint32_t found = n;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) found += (uint32_t)__shfl_xor((int)found, d);
+    if (__ballot(spill) != 0ull || found != held) return false;
+    uint32_t prev = 0u;
+    for (uint32_t r = 0; r < slots; r++) {
+        uint32_t best = ~0u;
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++)
+            if (mem[k] != ~0u && (r == 0u || mem[k] > prev) && mem[k] < best) best = mem[k];
+        prev = wave_min(best);
+        if (lane == 0u) base[r] = prev;
+    }
+    return true;
+}
+
 // sort.wgsl:14-53, run by the record that arrived first in each bucket; the
-// overfull buckets' scan is run by those records' wave.
+// overfull buckets are taken by those records' wave, from their members when
+// bin listed them, else by the scan.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restrict__ scenes, SurfaceView sv, GridView g,
                                                 TileView tv)
@@ -876,7 +945,15 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restric
                     base[at] = value;
                 }
             }
-            const uint64_t pending = __ballot(over);
+            uint64_t pending = __ballot(over);
+            if (pending && g.ovf) {
+                for (uint64_t left = pending; left; left &= left - 1ull) {
+                    const int j = __builtin_ctzll(left);
+                    const uint32_t bb = (uint32_t)__shfl((int)b, j);
+                    if (overflow_members(g, g.slots + (size_t)(gb + bb) * slots, gb + bb, g.count[gb + bb], slots))
+                        pending &= ~(1ull << j);
+                }
+            }
             if (pending) overflow_scan(sc, sv.key, g.slots + (size_t)gb * slots, b, pending, slots);
         }
     }
@@ -1065,6 +1142,9 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_settle(NodeView n, const vs_scene_params *__restrict__ scenes,
                                                   SurfaceView sv, GridView g, TileView tv)
 {
+    // The overflow list is empty again for the next substep's bin (which
+    // runs after this launch).
+    if (g.ovf_count && blockIdx.x == 0 && threadIdx.x == 0) *g.ovf_count = 0u;
     const Range r = tile_range(*tv.count, 0);
     for (uint32_t t = r.first; t < r.end; t += r.stride) {
         const vs_tile tile = tv.tiles[t];

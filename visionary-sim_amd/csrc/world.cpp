@@ -145,6 +145,8 @@ struct vs_world {
     uint32_t *d_surf = nullptr;
     float4 *d_surf_pay = nullptr;  // per record: node position and radius at bin time
     uint32_t *d_grid = nullptr;
+    uint2 *d_ovf = nullptr;            // records that found their bucket full: (record, grid bucket)
+    uint32_t *d_ovf_count = nullptr;   // how many this substep (settle empties it)
     uint32_t *d_drive = nullptr;
     float *d_splat_in = nullptr, *d_splat_out = nullptr;
     int32_t *d_pairlog = nullptr;
@@ -232,7 +234,11 @@ SurfaceView surf_view(vs_world *w)
                         w->d_surf_pay };
 }
 
-GridView grid_view(vs_world *w) { return GridView{ w->d_grid, w->d_grid + w->cfg.max_buckets }; }
+GridView grid_view(vs_world *w)
+{
+    return GridView{ w->d_grid, w->d_grid + w->cfg.max_buckets, w->d_ovf, w->d_ovf_count,
+                     (uint32_t)w->cfg.max_surface };
+}
 
 vsim::DriveView drive_view(vs_world *w)
 {
@@ -760,12 +766,15 @@ static int world_init(vs_world *w)
     ALLOC(W->d_surf, sizeof(uint32_t) * SURF_PLANES * c.max_surface);
     ALLOC(W->d_surf_pay, sizeof(float4) * c.max_surface);
     ALLOC(W->d_grid, sizeof(uint32_t) * (1 + BUCKET_SLOTS) * c.max_buckets);
+    ALLOC(W->d_ovf, sizeof(uint2) * c.max_surface);
+    ALLOC(W->d_ovf_count, sizeof(uint32_t));
     ALLOC(W->d_drive, sizeof(uint32_t) * DRIVE_PLANES * c.max_driven);
     ALLOC(W->d_splat_in, sizeof(float) * 10 * c.max_splats);
     ALLOC(W->d_splat_out, sizeof(float) * SPLAT_OUT_PLANES * c.max_splats);
     ALLOC(W->d_scene_counts, sizeof(uint32_t) * NPASS * c.max_scenes);
     HIPW(W, hipMemset(W->d_nodes, 0, sizeof(float) * NODE_PLANES * c.max_nodes));
     HIPW(W, hipMemset(W->d_grid, 0, sizeof(uint32_t) * c.max_buckets));
+    HIPW(W, hipMemset(W->d_ovf_count, 0, sizeof(uint32_t)));
     HIPW(W, hipMemset(W->d_scenes, 0, sizeof(vs_scene_params) * c.max_scenes));
     W->nodes.init(c.max_nodes);
     W->cells.init(c.max_cells);
@@ -845,7 +854,8 @@ void vs_world_destroy(vs_world *w)
     for (hipEvent_t ev : { w->upload_done, w->tables_done, w->params_done, w->tune[0], w->tune[1] })
         if (ev) (void)hipEventDestroy(ev);
     void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf, w->d_surf_pay,
-                     w->d_grid,  w->d_drive, w->d_splat_in, w->d_splat_out, w->d_pairlog, w->d_scene_counts };
+                     w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_drive, w->d_splat_in, w->d_splat_out,
+                     w->d_pairlog, w->d_scene_counts };
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int p = 0; p < NPASS; p++)
@@ -1459,7 +1469,7 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out)
     }
     const vs_world_config &c = w->cfg;
     out->bytes_reserved = 4ull * NODE_PLANES * c.max_nodes + 4ull * c.max_cells + sizeof(vs_body) * c.max_bodies +
-                          sizeof(vs_scene_params) * c.max_scenes + (4ull * SURF_PLANES + 16) * c.max_surface +
+                          sizeof(vs_scene_params) * c.max_scenes + (4ull * SURF_PLANES + 24) * c.max_surface +
                           4ull * (1 + BUCKET_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
                           4ull * (10 + SPLAT_OUT_PLANES) * c.max_splats;
     out->graph_captures = w->captures;
