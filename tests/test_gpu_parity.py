@@ -87,12 +87,14 @@ def test_contact_keys_and_pairs_bit_exact():
     assert counts.sum() > 0
 
 
-@pytest.mark.parametrize("count,side,bucket", [(4, 6, 0.05), (6, 8, 0.06), (8, 8, 0.08)])
+@pytest.mark.parametrize("count,side,bucket", [(4, 6, 0.05), (6, 8, 0.06), (8, 8, 0.08), (8, 8, 0.25)])
 def test_overfull_buckets_bit_exact(count, side, bucket):
-    """Buckets several voxels wide hold up to 86-199 records: each overfull
-    bucket keeps its 32 lowest record ids, ascending (sort.wgsl:27-37; on the
-    GPU one wave scans the scene's records for it). Keys, accepted pairs and
-    positions against the oracle over three frames."""
+    """Buckets several voxels wide hold up to 86-199 records (0.25 m: up to
+    1271): each overfull bucket keeps its 32 lowest record ids, ascending
+    (sort.wgsl:27-37). On the GPU sort takes them from the bucket's members
+    (arrival slots and bin's overflow list); buckets of more than 512 records
+    go to the wave-wide rescan of the scene's records. Keys, accepted pairs
+    and positions against the oracle over three frames."""
     from oracle.oracle import OracleScene
     sc = scenes.contact_stack(count, side=side)
     sc.bucket_size = bucket
