@@ -982,4 +982,16 @@ void o_cell_residuals(const o_scene *sc, uint32_t body, uint32_t lin, REAL *out1
     }
 }
 
+/* The tensile limit's edge table (STRETCH_EDGES, stretch.js:5-7) as the
+ * cell sweep reads it: corner pairs and unit-cell lengths, 28 entries. */
+void o_stretch_edges(int32_t *a, int32_t *b, double *len)
+{
+    build_tables();
+    for (int e = 0; e < 28; e++) {
+        a[e] = STRETCH_A[e];
+        b[e] = STRETCH_B[e];
+        len[e] = (double)STRETCH_LEN[e];
+    }
+}
+
 int o_real_size(void) { return (int)sizeof(REAL); }

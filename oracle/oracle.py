@@ -104,6 +104,7 @@ def load(precision="f32"):
     lib.o_project_tet.argtypes = [C.c_void_p] * 4 + [C.c_double] * 4 + [C.c_void_p]
     lib.o_project_kuhn_tet.argtypes = [C.c_void_p] * 3 + [C.c_int] + [C.c_double] * 5 + [C.c_int, C.c_void_p]
     lib.o_cell_residuals.argtypes = [sp, C.c_uint32, C.c_uint32, C.c_void_p]
+    lib.o_stretch_edges.argtypes = [C.c_void_p] * 3
     lib.o_real_size.restype = C.c_int
     assert lib.o_real_size() == (4 if precision == "f32" else 8)
     _libs[precision] = lib
@@ -138,6 +139,16 @@ def grip_radius(pos, mass, anchor, begin, end, radius, limit):
     k = load("f64").o_grip_radius(*ptrs, m.ctypes.data, begin, end, anchor, float(radius), limit,
                                   nodes.ctypes.data, offs.ctypes.data, C.byref(found))
     return nodes[:k].copy(), offs[:k].copy(), found.value
+
+
+def stretch_edges(precision="f64"):
+    """The tensile limit's corner pairs and unit-cell lengths as the
+    oracle's cell sweep reads them (STRETCH_EDGES, stretch.js:5-7)."""
+    a = np.zeros(28, np.int32)
+    b = np.zeros(28, np.int32)
+    length = np.zeros(28, np.float64)
+    load(precision).o_stretch_edges(a.ctypes.data, b.ctypes.data, length.ctypes.data)
+    return a, b, length
 
 
 class Grid:

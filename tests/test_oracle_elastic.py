@@ -335,3 +335,98 @@ def test_fused_form_finds_the_same_collapse_barrier(oracle):
         else:
             lost = mid
     assert abs(ok - collapse_scale(s["c"][2])) < 0.5 * 10 ** -2
+
+
+def _stretch_block():
+    sc = scenes.jelly_block(8, pinned=True, name="stretch")
+    sc.bodies[0].material.max_stretch = 1.02
+    sc.drive = scenes.handle_drive(sc, seed=5, law=0, amplitude=0.3, frequency=2.0)
+    return sc
+
+
+def _drag_block():
+    sc = scenes.jelly_block(12, pinned=False, tank_margin=0.3, name="drag")
+    sc.tank_min = (sc.tank_min[0], 0.0, sc.tank_min[2])
+    sc.drive = scenes.handle_drive(sc, seed=3, law=0)
+    return sc
+
+
+WIDE_CASES = {
+    "contact_stack": (lambda: scenes.contact_stack(), 3),
+    "hard_drive": (_drag_block, 3),
+    "stretch": (_stretch_block, 3),
+    "bodies64_contact": (lambda: scenes.c3_scene(bodies=64), 2),
+}
+
+
+@pytest.mark.parametrize("case", sorted(WIDE_CASES))
+def test_fused_frames_track_the_literal_formulas_where_the_branches_fire(oracle, case):
+    # Scenes where contact, the hard drive, the tensile sweeps and fast
+    # collisions exercise what a resting block does not. The GPU's fused f32
+    # order against the literal WGSL formulas in f32, relative to the largest
+    # coordinate: within 1e-5 after one substep and after whole frames; and
+    # against f64, one substep within 1e-5 for both forms, while over whole
+    # frames (where contact between fast bodies lets rounding grow) the fused
+    # form stays as close to f64 as the literal one.
+    make, frames = WIDE_CASES[case]
+    sc = make()
+    fused = oracle.OracleScene(sc, "f32", literal=False)
+    lit = oracle.OracleScene(sc, "f32", literal=True)
+    ref = oracle.OracleScene(sc, "f64", literal=True)
+    start = np.asarray(sc.arrays()["pos"]).T
+    for o in (fused, lit, ref):
+        if sc.drive is not None:
+            o.set_target(*sc.drive.frame_target(0, sc.substeps * sc.dt))
+        o.substep()
+    scale = np.max(np.abs(ref.p))
+    one = (np.max(np.abs(fused.p - lit.p)) / scale, np.max(np.abs(fused.p - ref.p)) / scale,
+           np.max(np.abs(lit.p - ref.p)) / scale)
+    fused = oracle.OracleScene(sc, "f32", literal=False)
+    lit = oracle.OracleScene(sc, "f32", literal=True)
+    ref = oracle.OracleScene(sc, "f64", literal=True)
+    for _ in range(frames):
+        for o in (fused, lit, ref):
+            o.frame()
+    scale = np.max(np.abs(ref.p))
+    d = np.max(np.abs(fused.p - lit.p)) / scale
+    e_f = np.max(np.abs(fused.p - ref.p)) / scale
+    e_l = np.max(np.abs(lit.p - ref.p)) / scale
+    moved_by = np.max(np.abs(ref.p - start)) / scale
+    print(f"{case}: one substep: fused vs literal {one[0]:.3e}, vs f64 {one[1]:.3e} / {one[2]:.3e}; "
+          f"{frames} frames: fused vs literal {d:.3e}, vs f64 {e_f:.3e} / {e_l:.3e}; motion {moved_by:.3e}")
+    assert moved_by > 1e-3
+    assert max(one) <= 1e-5
+    assert d <= 1e-5
+    assert e_f <= 1.5 * e_l + 1e-6
+
+
+@pytest.mark.parametrize("shape", ["collapsed", "flat", "inverted", "needle"])
+def test_both_forms_agree_on_degenerate_tets(oracle, shape):
+    # The branches a resting block never takes: F = 0 (every corner on one
+    # point: C_D below its 1e-9 floor, zero cofactors), a flat tet (det F = 0),
+    # an inverted one (det F < 0) and a needle (three corners on one point).
+    # Literal and fused agree to 1e-9 of the motion in f64, the result is
+    # finite, and F = 0 moves nothing.
+    for t in range(6):
+        s = rest_tet(0.3, t)
+        p = s["p"].copy()
+        if shape == "collapsed":
+            p[:] = p[0]
+        elif shape == "flat":
+            p[:, 2] = p[0, 2]
+        elif shape == "inverted":
+            p[3] = 2 * p[0] - p[3]
+        else:
+            p[1:] = p[1]
+        prev = p - 0.01 * H
+        for relax in (0.0, 0.012 / DT):
+            a = kuhn(oracle, s, p, t, prev, relax, literal=True)
+            b = kuhn(oracle, s, p, t, prev, relax, literal=False)
+            assert np.all(np.isfinite(a)) and np.all(np.isfinite(b))
+            motion = np.max(np.abs(a - p))
+            assert np.max(np.abs(a - b)) <= 1e-9 * max(motion, H), (shape, t, relax)
+            if shape == "collapsed":
+                assert np.array_equal(a, p) and np.array_equal(b, p)
+            a32 = kuhn(oracle, s, p, t, prev, relax, literal=True, precision="f32")
+            b32 = kuhn(oracle, s, p, t, prev, relax, literal=False, precision="f32")
+            assert np.max(np.abs(a32 - b32)) <= 1e-5 * max(np.max(np.abs(a)), H), (shape, t, relax)
