@@ -215,6 +215,24 @@ int vs_dispatch(vs_backend *b, const vs_launch *launches, uint32_t count);
 /* Waits until every enqueued launch has finished. */
 int vs_synchronize(vs_backend *b);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The grip of VS_K_GRIP (withinRadius, grip.js:99-138)
+ * with the question passed by value and its candidate queue and counters in
+ * a static device array the library owns: 256 B + 16 B x
+ * VS_GRIP_NOALLOC_MAX_NODES (4 MiB + 256 B), held on every GPU the library
+ * runs on until the process ends. `pos` (3 planes of `plane` floats) and
+ * `mass` (the grab mass, `plane` floats) are backend buffers, as is `out`,
+ * laid out as VS_B_OUT of VS_K_GRIP for `limit` held nodes; `grid` is 1..1024
+ * workgroups. A body range (params->end - params->begin) above
+ * VS_GRIP_NOALLOC_MAX_NODES is rejected with VS_E_INVALID. Enqueued on the
+ * backend's stream with no host wait; calls no allocating routine. Calls on
+ * one GPU share the array, so calls that overlap on one GPU (two streams,
+ * two backends) race. */
+#define VS_GRIP_NOALLOC_MAX_NODES (1u << 18)
+int vs_grip_noalloc(vs_backend *b, uint64_t pos, uint32_t plane, uint64_t mass, const vs_grab_params *params,
+                    uint32_t limit, uint32_t grid, uint64_t out);
+
 /* ------------------------------------------------------ layer 2: world */
 
 typedef struct vs_world vs_world;

@@ -20,6 +20,8 @@ struct vs_backend {
     std::string error;
     bool destroyed = false;
     uint64_t limit = 0;
+    void *grip_scratch = nullptr;            // this device's static grip array
+    hipError_t grip_scratch_status = hipSuccess;
 };
 
 namespace {
@@ -123,6 +125,9 @@ int vs_backend_create(int device, vs_backend **out)
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) b->limit = prop.totalGlobalMem;
+    // The static array of vs_grip_noalloc, looked up once per device with the
+    // device current; a failure is reported by that call.
+    b->grip_scratch_status = vsim::grip_static_scratch(device, &b->grip_scratch);
     *out = b;
     return VS_OK;
 }
@@ -339,6 +344,46 @@ int vs_dispatch(vs_backend *b, const vs_launch *launches, uint32_t count)
         if (e != hipSuccess) return hip_fail(b, e, KERNEL_NAMES[l.kernel]);
     }
     return VS_OK;
+}
+
+int vs_grip_noalloc(vs_backend *b, uint64_t pos, uint32_t plane, uint64_t mass, const vs_grab_params *params,
+                    uint32_t limit, uint32_t grid, uint64_t out)
+{
+    if (!b || !params) return VS_E_INVALID;
+    if (b->destroyed) return fail(b, VS_E_DESTROYED, "Compute backend is destroyed");
+    const char *name = "sim-grip-radius";
+    if (grid < 1 || grid > 1024) {
+        return fail(b, VS_E_INVALID, std::string(name) + ": invalid dispatch " + std::to_string(grid));
+    }
+    if (plane >= (1u << 29)) return fail(b, VS_E_INVALID, std::string(name) + ": plane exceeds 2^29 floats");
+    if (params->end > params->begin && params->end - params->begin > VS_GRIP_NOALLOC_MAX_NODES) {
+        return fail(b, VS_E_INVALID, std::string(name) + ": body of " + std::to_string(params->end - params->begin) +
+                                         " nodes exceeds the static candidate array of " +
+                                         std::to_string(VS_GRIP_NOALLOC_MAX_NODES) + " nodes");
+    }
+    vs_launch l{};
+    l.kernel = VS_K_GRIP;
+    l.grid = grid;
+    l.plane = plane;
+    l.pair_cap = limit;
+    const std::pair<int, uint64_t> slots[] = { { VS_B_POS, pos }, { VS_B_INVMASS, mass }, { VS_B_OUT, out } };
+    for (const auto &sl : slots) {
+        auto it = b->allocations.find(sl.second);
+        if (it == b->allocations.end() || it->second < slot_bytes(l, sl.first)) {
+            return fail(b, VS_E_INVALID, std::string(name) + ": invalid buffer " + SLOT_NAMES[sl.first]);
+        }
+    }
+    if (b->grip_scratch_status != hipSuccess || !b->grip_scratch) {
+        return hip_fail(b, b->grip_scratch_status != hipSuccess ? b->grip_scratch_status : hipErrorInvalidSymbol,
+                        "sim-grip-radius static array");
+    }
+    (void)hipSetDevice(b->device);
+    const float *px = reinterpret_cast<const float *>(pos);
+    const hipError_t e =
+        vsim::launch_grip_static(b->stream, b->grip_scratch, grid, px, px + plane, px + 2 * (uint64_t)plane,
+                                 reinterpret_cast<const float *>(mass), plane, *params, limit,
+                                 reinterpret_cast<void *>(out));
+    return e == hipSuccess ? VS_OK : hip_fail(b, e, name);
 }
 
 }  // extern "C"

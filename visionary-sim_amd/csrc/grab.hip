@@ -16,6 +16,8 @@
 // one to finish (a device-scope counter after a release fence) reduces or
 // selects, writes the answer and clears the counters it used, so the output
 // buffer is ready for the next launch without a host-side clear.
+#include <mutex>
+
 #include "kernels.h"
 
 namespace vsim {
@@ -152,17 +154,19 @@ struct GripOut {
     uint32_t pad[3];
 };
 
-__global__ __launch_bounds__(GRIP_BLOCK) void k_grip(const float *__restrict__ px, const float *__restrict__ py,
-                                                     const float *__restrict__ pz, const float *__restrict__ mass,
-                                                     uint32_t count, const vs_grab_params *__restrict__ gp,
-                                                     Candidate *cand, uint32_t cap, uint32_t limit, GripOut *out)
+// The grip of one question `g`. `ctr` holds the launch's two counters
+// (candidates appended, workgroups finished), zero at launch; the last
+// workgroup clears them again.
+__device__ __forceinline__ void grip(const float *__restrict__ px, const float *__restrict__ py,
+                                     const float *__restrict__ pz, const float *__restrict__ mass, uint32_t count,
+                                     const vs_grab_params &g, Candidate *cand, uint32_t cap, uint32_t limit,
+                                     uint32_t *ctr, GripOut *out)
 {
     __shared__ double s_key[SELECT_CAP];
     __shared__ uint32_t s_node[SELECT_CAP];
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_at, s_digit, s_before;
     __shared__ bool s_last;
-    const vs_grab_params g = *gp;
     const bool range_ok = g.begin <= g.end && g.end <= count && g.anchor >= g.begin && g.anchor < g.end;
     const bool eligible = range_ok && mass[g.anchor] > 0.0f;
     const double ax = range_ok ? (double)px[g.anchor] : 0.0, ay = range_ok ? (double)py[g.anchor] : 0.0,
@@ -184,7 +188,7 @@ __global__ __launch_bounds__(GRIP_BLOCK) void k_grip(const float *__restrict__ p
             if (m == 0) continue;
             const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&out->count, (uint32_t)__popcll(m));
+            if (lane == leader) base = atomicAdd(&ctr[0], (uint32_t)__popcll(m));
             base = __shfl(base, (int)leader);
             const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
             if (take && slot < cap) cand[slot] = Candidate{ d2, n, 0u };
@@ -193,12 +197,12 @@ __global__ __launch_bounds__(GRIP_BLOCK) void k_grip(const float *__restrict__ p
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
-        s_last = atomicAdd(&out->done, 1u) == gridDim.x - 1;
+        s_last = atomicAdd(&ctr[1], 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    const uint32_t c = __hip_atomic_load(&out->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t c = __hip_atomic_load(&ctr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t *nodes = reinterpret_cast<uint32_t *>(out + 1);
     double *offs = reinterpret_cast<double *>(nodes + limit + (limit & 1u));
     auto finish = [&](uint32_t found, uint32_t kept, uint32_t status) {
@@ -208,6 +212,8 @@ __global__ __launch_bounds__(GRIP_BLOCK) void k_grip(const float *__restrict__ p
             out->status = status;
             out->count = 0u;
             out->done = 0u;
+            ctr[0] = 0u;
+            ctr[1] = 0u;
         }
     };
     if (!eligible) {
@@ -348,7 +354,78 @@ __global__ __launch_bounds__(GRIP_BLOCK) void k_grip(const float *__restrict__ p
     finish(c + 1u, want + 1u, 0u);
 }
 
+// VS_K_GRIP: the question in device memory, the counters in the output header.
+__global__ __launch_bounds__(GRIP_BLOCK) void k_grip(const float *__restrict__ px, const float *__restrict__ py,
+                                                     const float *__restrict__ pz, const float *__restrict__ mass,
+                                                     uint32_t count, const vs_grab_params *__restrict__ gp,
+                                                     Candidate *cand, uint32_t cap, uint32_t limit, GripOut *out)
+{
+    const vs_grab_params g = *gp;
+    grip(px, py, pz, mass, count, g, cand, cap, limit, &out->count, out);
+}
+
+// vs_grip_noalloc: the question by value; counters and candidates in the
+// static array below.
+__global__ __launch_bounds__(GRIP_BLOCK) void k_grip_static(const float *__restrict__ px,
+                                                            const float *__restrict__ py,
+                                                            const float *__restrict__ pz,
+                                                            const float *__restrict__ mass, uint32_t count,
+                                                            vs_grab_params g, uint32_t *ctr, Candidate *cand,
+                                                            uint32_t cap, uint32_t limit, GripOut *out)
+{
+    grip(px, py, pz, mass, count, g, cand, cap, limit, ctr, out);
+}
+
+__global__ void k_clear_counters(uint32_t *ctr)
+{
+    if (threadIdx.x < 2) ctr[threadIdx.x] = 0u;
+}
+
+// The device memory of vs_grip_noalloc: 256 bytes of counters, then
+// VS_GRIP_NOALLOC_MAX_NODES candidates of 16 bytes (4 MiB + 256 B). Every
+// device the library runs on holds its own copy until the process ends. In
+// host code `scratch` names a host-side stand-in, not the device array: host
+// code uses only the address grip_static_scratch() returns for the device.
+constexpr size_t SCRATCH_COUNTERS = 256;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_COUNTERS + sizeof(Candidate) * (size_t)VS_GRIP_NOALLOC_MAX_NODES];
+static_assert(sizeof(Candidate) == 16, "candidate records are 16 bytes");
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_lock;
+void *g_scratch[MAX_DEVICES];
+
 }  // namespace
+
+hipError_t grip_static_scratch(int device, void **ptr)
+{
+    *ptr = nullptr;
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> hold(g_scratch_lock);
+    if (!g_scratch[device]) {
+        // The caller has made `device` current: the symbol resolves to that
+        // device's copy of the array.
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = p;
+    }
+    *ptr = g_scratch[device];
+    return hipSuccess;
+}
+
+hipError_t launch_grip_static(hipStream_t s, void *scratch_ptr, uint32_t grid, const float *px, const float *py,
+                              const float *pz, const float *mass, uint32_t count, const vs_grab_params &params,
+                              uint32_t limit, void *out)
+{
+    uint32_t *ctr = reinterpret_cast<uint32_t *>(scratch_ptr);
+    Candidate *cand = reinterpret_cast<Candidate *>(static_cast<unsigned char *>(scratch_ptr) + SCRATCH_COUNTERS);
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(64), 0, s, ctr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_grip_static, dim3(grid), dim3(GRIP_BLOCK), 0, s, px, py, pz, mass, count, params, ctr, cand,
+                       (uint32_t)VS_GRIP_NOALLOC_MAX_NODES, limit, reinterpret_cast<GripOut *>(out));
+    return hipGetLastError();
+}
 
 hipError_t launch_pick(hipStream_t s, uint32_t grid, const float *px, const float *py, const float *pz,
                        const float *mass, uint32_t count, const vs_grab_params *params, void *out)

@@ -189,6 +189,14 @@ hipError_t launch_pick(hipStream_t s, uint32_t grid, const float *px, const floa
 hipError_t launch_grip(hipStream_t s, uint32_t grid, const float *px, const float *py, const float *pz,
                        const float *mass, uint32_t count, const vs_grab_params *params, void *candidates,
                        uint32_t cap, uint32_t limit, void *out);
+// The grip with its candidates and counters in the library's static device
+// array (vs_grip_noalloc): grip_static_scratch() returns the array's address
+// on `device`, which must be current, looking it up once per device;
+// launch_grip_static() clears the counters with a kernel and runs the grip.
+hipError_t grip_static_scratch(int device, void **ptr);
+hipError_t launch_grip_static(hipStream_t s, void *scratch_ptr, uint32_t grid, const float *px, const float *py,
+                              const float *pz, const float *mass, uint32_t count, const vs_grab_params &params,
+                              uint32_t limit, void *out);
 hipError_t launch_splats(hipStream_t s, Shape sh, const float *px, const float *py, const float *pz,
                          const vs_body *bodies, SplatView sp, TileView t);
 

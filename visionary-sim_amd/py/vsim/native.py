@@ -117,7 +117,7 @@ class VsServerReport(C.Structure):
 
 EXPORTS = [
     "vs_backend_create", "vs_backend_destroy", "vs_last_error", "vs_buffer_alloc", "vs_buffer_free",
-    "vs_buffer_write", "vs_buffer_read", "vs_kernel_id", "vs_dispatch", "vs_synchronize",
+    "vs_buffer_write", "vs_buffer_read", "vs_kernel_id", "vs_dispatch", "vs_synchronize", "vs_grip_noalloc",
     "vs_world_create", "vs_world_destroy", "vs_world_error", "vs_world_add_scene", "vs_world_remove_scene",
     "vs_world_reset_scene", "vs_world_set_target", "vs_world_frame", "vs_world_frame_scene",
     "vs_world_synchronize", "vs_world_record", "vs_world_elapsed", "vs_world_read_nodes",
@@ -175,6 +175,7 @@ def load(path=LIB_PATH):
         "vs_kernel_id": (C.c_int, [C.c_char_p]),
         "vs_dispatch": (C.c_int, [p, C.POINTER(VsLaunch), u32]),
         "vs_synchronize": (C.c_int, [p]),
+        "vs_grip_noalloc": (C.c_int, [p, u64, u32, u64, C.POINTER(VsGrabParams), u32, u32, u64]),
         "vs_world_create": (C.c_int, [C.POINTER(VsWorldConfig), C.POINTER(p)]),
         "vs_world_destroy": (None, [p]),
         "vs_world_error": (C.c_char_p, [p]),
@@ -291,6 +292,12 @@ class Backend:
 
     def sync(self):
         self._check(self.lib.vs_synchronize(self.h), "synchronize")
+
+    def grip_noalloc(self, pos, plane, mass, params, limit, grid, out):
+        """vs_grip_noalloc: the grip with the library's static candidate
+        array; `params` a VsGrabParams, buffers by handle."""
+        self._check(self.lib.vs_grip_noalloc(self.h, pos, plane, mass, C.byref(params), limit, grid, out),
+                    "grip_noalloc")
 
     def launch(self, kernel, grid, block=64, plane=0, items=0, flags=0, buckets=0, pair_cap=0, **buffers):
         """A vs_launch record; `buffers` by slot name (nodePos=..., out=...)."""
