@@ -387,7 +387,9 @@ int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, 
  * one round = the eight colour passes; splats: the outputs of the last frame) with a HIP event before the first and
  * after the last, so *ms / *launches is the kernel's average duration with no
  * per-launch event in between. Node state is saved first and restored after,
- * so the simulation is left where it was. Synchronous. */
+ * so the simulation is left where it was. The contact kinds (VS_P_BIN ..
+ * VS_P_SETTLE) are refused with VS_E_INVALID: their bucket counters would not
+ * be left where they were. Synchronous. */
 int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, uint32_t *launches);
 /* Re-times the colour-pass launch shapes on the admitted scenes and keeps the
  * fastest (block size x cells per tile x persistent grid x XCD mapping). */

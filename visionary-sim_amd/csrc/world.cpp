@@ -1500,6 +1500,12 @@ int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, 
 int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, uint32_t *launches)
 {
     if (!w || !ms || !launches || rounds == 0 || kind < 0 || kind >= VS_P_COUNT) return VS_E_INVALID;
+    // Repeated contact launches would leave the grid's counters and the
+    // overflow list counted up (only the node planes are restored), so the
+    // contact kinds are timed inside whole frames (vs_world_profile) only.
+    if (kind >= VS_P_BIN && kind <= VS_P_SETTLE) {
+        return fail(w, VS_E_INVALID, "time_pass: contact passes are timed in whole frames (vs_world_profile)");
+    }
     HIPW(w, hipSetDevice(w->cfg.device));
     if (int rc = prepare(w)) return rc;
     const int p = kind == VS_P_ELASTIC ? P_CELL
