@@ -61,8 +61,8 @@ typedef struct vs_body {
 } vs_body;
 
 /* Per-scene uniforms: the Params blocks of predict.wgsl:1-5,
- * finalize.wgsl:1-6, contacts.wgsl:1-8 and drive.wgsl:1-7 in one record.
- * 128 bytes. */
+ * finalize.wgsl:1-6, contacts.wgsl:1-8 and drive.wgsl:1-7 in one record,
+ * plus the sphere collider. 144 bytes. */
 typedef struct vs_scene_params {
     float dt;
     float gravity[3];
@@ -82,6 +82,10 @@ typedef struct vs_scene_params {
     uint32_t surf_count;
     uint32_t bucket_slots; /* 32, BUCKET_SLOTS of contacts.wgsl:75           */
     uint32_t flags;        /* VS_SCENE_*                                     */
+    float sphere[4];       /* sphere collider: centre xyz, radius (<= 0: none).
+                              No reference counterpart: finalize moves a node
+                              inside it out along the radius, with the tank
+                              planes' friction (finalize.wgsl:29-42)          */
 } vs_scene_params;
 
 /* A scene whose finalize follows its last colour directly (no contact and
@@ -312,6 +316,7 @@ typedef struct vs_scene_desc {
     const float *driven_offset;  /* 3 per record                            */
     uint32_t drive_law;
     float response_time;
+    float sphere[4];             /* sphere collider centre xyz, radius (<= 0: none) */
 } vs_scene_desc;
 
 int vs_world_create(const vs_world_config *cfg, vs_world **out);

@@ -666,6 +666,19 @@ static v3 resolve_plane(v3 p, v3 prev, v3 normal, real depth, real friction)
     return q;
 }
 
+/* The sphere collider, an extension with no reference counterpart: a node
+ * inside the ball is moved out along the radius to the surface (one on the
+ * centre leaves along +y), with resolvePlane's friction (finalize.wgsl:29-42)
+ * in the tangent plane there. */
+static v3 resolve_sphere(v3 p, v3 prev, v3 c, real r, real friction)
+{
+    v3 d = sub(p, c);
+    real len = length3(d);
+    if (!(len < r)) return p;
+    v3 n = len > (real)(float)1e-9 ? mk(d.x / len, d.y / len, d.z / len) : mk(0, 1, 0);
+    return resolve_plane(p, prev, n, r - len, friction);
+}
+
 /* bodyOfNode, state.wgsl:39-47 */
 static uint32_t body_of_node(const o_scene *sc, uint32_t node)
 {
@@ -700,6 +713,10 @@ void o_finalize(o_scene *sc)
         p = resolve_plane(p, prev, mk(-1, 0, 0), p.x - tmax[0], fr);
         p = resolve_plane(p, prev, mk(0, 0, 1), tmin[2] - p.z, fr);
         p = resolve_plane(p, prev, mk(0, 0, -1), p.z - tmax[2], fr);
+        if ((real)sc->sphere[3] > (real)0) {
+            p = resolve_sphere(p, prev, mk((real)sc->sphere[0], (real)sc->sphere[1], (real)sc->sphere[2]),
+                               (real)sc->sphere[3], fr);
+        }
         sc->px[n] = p.x;
         sc->py[n] = p.y;
         sc->pz[n] = p.z;

@@ -66,6 +66,8 @@ typedef struct {
     int32_t pair_cap;
     int32_t literal;             /* 1: elastic pass in project.wgsl's literal
                                     formulas; 0: the fused order of the GPU  */
+    int32_t pad1;
+    double sphere[4];            /* sphere collider: centre, radius (<= 0: none) */
 } o_scene;
 
 /* Splats of one scene; each splat rides one body's lattice. Outputs are SoA:

@@ -45,7 +45,7 @@ class OScene(C.Structure):
         ("d_node", C.c_void_p), ("d_off", C.c_void_p),
         ("target", C.c_double * 3), ("target_vel", C.c_double * 3), ("response_time", C.c_double),
         ("key_out", C.c_void_p), ("pair_count", C.c_void_p), ("pairs", C.c_void_p),
-        ("pair_cap", C.c_int32), ("literal", C.c_int32),
+        ("pair_cap", C.c_int32), ("literal", C.c_int32), ("pad1", C.c_int32), ("sphere", C.c_double * 4),
     ]
 
 
@@ -363,6 +363,8 @@ class OracleScene:
         sc.bucket_slots = 32
         sc.key_out = self.keys.ctypes.data
         sc.literal = int(literal)
+        if scene.sphere is not None:
+            sc.sphere[:] = [float(np.float32(v)) for v in scene.sphere]
         if pair_cap:
             sc.pair_count = self.pair_count.ctypes.data
             sc.pairs = self.pairs.ctypes.data

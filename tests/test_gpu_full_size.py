@@ -6,9 +6,12 @@
   folded node passes and node tiles against one scene's slice of the same
   tables).
 * 64^3 block with 2 M splats and a 27-node drag handle, and 512 bodies of 8^3
-  in contact: one frame against the f32 oracle, positions within 1e-5 of the
-  largest coordinate (the figures printed are the exact-match fraction), and
-  the contact grid's bucket keys and accepted pairs bit for bit.
+  in contact (buckets of 1 and of 2 voxels): one frame against the f32
+  oracle, positions within 1e-5 of the largest coordinate (the figures
+  printed are the exact-match fraction), and the contact grid's bucket keys
+  and accepted pairs bit for bit.
+* the 512 bodies piled up after 80 frames: one more frame against the oracle
+  from the GPU's state, with about a hundred overfull buckets.
 """
 import numpy as np
 import pytest
@@ -57,10 +60,33 @@ def test_256_scenes_batched_equals_plain():
         assert np.array_equal(m, m2) and np.array_equal(c, c2)
 
 
-@pytest.mark.parametrize("case", ["block64_drag", "bodies512_contact"])
+FULL_SIZE = {
+    "block64_drag": scenes.c2_scene,
+    "bodies512_contact": scenes.c3_scene,
+    # the configuration's hash grid: buckets of 2 voxels, up to 64 records
+    # at worst against 32 slots (broadphase.js:82-85)
+    "bodies512_contact_2voxel": lambda: scenes.c3_scene(bucket_voxels=2.0),
+}
+
+
+def check_contacts(w, sid, o, cap, case):
+    """The last substep's bucket keys and accepted pairs (up to `cap`
+    logged) against the oracle's."""
+    keys, counts, pairs = w.read_contacts(sid)
+    assert np.array_equal(keys, o.keys)
+    assert np.array_equal(counts, o.pair_count)
+    for s in np.nonzero(counts)[0]:
+        k = min(counts[s], cap)
+        assert np.array_equal(pairs[s, :k], o.pairs[s, :k])
+    overfull = int((np.bincount(keys) > 32).sum())
+    print(f"{case}: {int(counts.sum())} pairs in the last substep, {overfull} overfull buckets")
+    return overfull
+
+
+@pytest.mark.parametrize("case", sorted(FULL_SIZE))
 def test_full_size_frame_against_oracle(case):
     from oracle.oracle import OracleScene
-    sc = scenes.c2_scene() if case == "block64_drag" else scenes.c3_scene()
+    sc = FULL_SIZE[case]()
     print(f"{case}: {sc.n_nodes} nodes, {sc.n_splats} splats, {sc.n_surface_active} contact records, "
           f"{sc.substeps} substeps", flush=True)
     cap = 8 if sc.contact_active else 0
@@ -82,12 +108,35 @@ def test_full_size_frame_against_oracle(case):
         mean, cov = w.read_splats(sid)
         assert rel_err(mean, o.out_mean) <= RTOL and rel_err(cov, o.out_cov) <= RTOL
     if cap:
-        # the last substep's bucket keys and accepted pairs (up to `cap` logged)
-        keys, counts, pairs = w.read_contacts(sid)
-        assert np.array_equal(keys, o.keys)
-        assert np.array_equal(counts, o.pair_count)
-        for s in np.nonzero(counts)[0]:
-            k = min(counts[s], cap)
-            assert np.array_equal(pairs[s, :k], o.pairs[s, :k])
-        print(f"{case}: {int(counts.sum())} pairs in the last substep")
+        check_contacts(w, sid, o, cap, case)
+    w.close()
+
+
+@pytest.mark.parametrize("bucket_voxels", [1.0, 2.0])
+def test_piled_contact_frame_against_oracle(bucket_voxels):
+    """The 512 bodies after 80 frames on the GPU, fallen into a pile whose
+    crowded buckets hold far more than 32 records: the GPU's state is loaded
+    into the oracle and one more frame is compared (positions, velocities,
+    the last substep's keys and pairs). The overfull buckets keep their 32
+    lowest record ids (sort.wgsl:27-37, bodyContact.js:87-107)."""
+    from oracle.oracle import OracleScene
+    sc = scenes.c3_scene(bucket_voxels=bucket_voxels)
+    cap = 8
+    w = world_for([sc])
+    w.log_pairs(cap)
+    sid = w.add(sc)
+    for f in range(80):
+        w.frame(sc.substeps, False, True)
+    pos, vel = w.read_nodes(sid)
+    print(f"piled ({bucket_voxels:g}-voxel buckets): state after 80 frames read back", flush=True)
+    o = OracleScene(sc, "f32", pair_cap=cap)
+    o.p[:] = pos
+    o.q[:] = pos
+    o.v[:] = vel
+    w.frame(sc.substeps, False, True)
+    o.frame(sc.substeps)
+    pos, vel = w.read_nodes(sid)
+    print(f"piled: exact {np.mean(pos == o.p):.6f}, rel {rel_err(pos, o.p):.2e}")
+    assert rel_err(pos, o.p) <= RTOL and rel_err(vel, o.v) <= RTOL
+    assert check_contacts(w, sid, o, cap, f"piled {bucket_voxels:g}-voxel") > 0
     w.close()

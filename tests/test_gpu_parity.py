@@ -437,3 +437,29 @@ def test_colour_pairs_equal_separate_colours_and_oracle(case, block):
         print(f"{case}/{s.name}: exact vs oracle {np.mean(p == o.p):.6f}")
         assert rel_err(p, o.p) <= RTOL
         assert rel_err(v, o.v) <= RTOL
+
+
+def _sphere_scene(drive):
+    sc = scenes.c2_scene(side=12, splats=2000, seed=7, sphere=True)
+    if not drive:
+        sc.drive = None
+    return sc
+
+
+@pytest.mark.parametrize("drive", [False, True])
+@pytest.mark.parametrize("schedule", ["pairs", "separate_colours", "separate_node_passes", "plain"])
+def test_sphere_collider_frames(drive, schedule):
+    """A block draped over the sphere collider (its cap two cells into the
+    block's underside), with and without the drag handle, through every
+    schedule: the folded node passes of the colour pairs, one colour per
+    launch, separate finalize and predict, and the plain per-scene path."""
+    sc = _sphere_scene(drive)
+    kw = {"separate_colours": int(schedule == "separate_colours"),
+          "separate_node_passes": int(schedule == "separate_node_passes")}
+    w, sid, o, pos, vel = step_both(sc, 4, graph=schedule == "pairs", plain=schedule == "plain", splats=True, **kw)
+    print(f"sphere {schedule} drive={drive}: exact {np.mean(pos == o.p):.6f}, rel {rel_err(pos, o.p):.3e}")
+    assert rel_err(pos, o.p) <= RTOL and rel_err(vel, o.v) <= RTOL
+    mean, cov = w.read_splats(sid)
+    assert rel_err(mean, o.out_mean) <= RTOL and rel_err(cov, o.out_cov) <= RTOL
+    c = np.asarray(sc.sphere[:3], np.float64)[:, None]
+    assert np.all(np.linalg.norm(pos.astype(np.float64) - c, axis=0) >= sc.sphere[3] * (1 - 1e-5))

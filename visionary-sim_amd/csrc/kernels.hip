@@ -11,7 +11,10 @@
 //
 // Layout: nodes as SoA planes, one wavefront lane per cell / node / record,
 // workgroups over tiles (kernels.h). No MFMA: the work is 3x3 algebra per
-// cell and the passes are bound by HBM, not by math.
+// cell, not a contraction. The colour passes of large batches are bound by
+// VALU issue (about 1.4 k wave instructions a cell, 90% and more of the issue
+// rate), the node and splat passes by HBM, and contact by chains of dependent
+// loads.
 #include "kernels.h"
 
 namespace vsim {
@@ -275,22 +278,46 @@ __device__ __forceinline__ V3 plane(V3 p, V3 prev, float depth, float friction)
     return q;
 }
 
+// The sphere collider (no reference counterpart): a node inside the ball is
+// moved out along the radius to the surface (a node on the centre leaves
+// along +y), with resolvePlane's position-level Coulomb friction in the
+// tangent plane there (finalize.wgsl:29-42), in the oracle's order.
+__device__ __noinline__ V3 sphere(V3 p, V3 prev, float cx, float cy, float cz, float r, float friction)
+{
+    const V3 d = mk(p.x - cx, p.y - cy, p.z - cz);
+    const float len = sqrtf(dot(d, d));
+    if (!(len < r)) return p;
+    const V3 n = len > 1e-9f ? mk(d.x / len, d.y / len, d.z / len) : mk(0.0f, 1.0f, 0.0f);
+    const float depth = r - len;
+    V3 q = add(p, mul(n, depth));
+    const V3 delta = sub(q, prev);
+    const V3 tangent = sub(delta, mul(n, dot(delta, n)));
+    const float tl = sqrtf(dot(tangent, tangent));
+    if (tl > 1e-9f) {
+        const float lim = (friction * depth) / tl;
+        q = sub(q, mul(tangent, lim < 1.0f ? lim : 1.0f));
+    }
+    return q;
+}
+
 // The uniforms finalize reads for one body of one scene.
 // `rate` = exp(-damping dt) / dt, the f32 quotient of the two factors, formed
 // once per tile: the damped velocity is then one product a component.
 struct Tank {
     float lx, ly, lz, hx, hz, fr, rate;
+    float sx, sy, sz, sr;  // sphere collider; sr <= 0: none
 };
 
 __device__ __forceinline__ Tank tank_of(const vs_scene_params &sc, const vs_body &b)
 {
     return Tank{ sc.tank_min[0], sc.tank_min[1], sc.tank_min[2], sc.tank_max[0], sc.tank_max[2], b.friction,
-                 b.damp / sc.dt };
+                 b.damp / sc.dt, sc.sphere[0], sc.sphere[1], sc.sphere[2], sc.sphere[3] };
 }
 
 // finalize.wgsl:44-74 for one node with w != 0: the five tank planes (no
-// ceiling), then the damped velocity (p - prev) / dt * exp(-damping dt), as
-// (p - prev) * rate (the oracle's o_finalize order).
+// ceiling), the sphere collider when the scene has one, then the damped
+// velocity (p - prev) / dt * exp(-damping dt), as (p - prev) * rate (the
+// oracle's o_finalize order).
 __device__ __forceinline__ void finalize_node(V3 &p, V3 prev, V3 &v, const Tank &k)
 {
     p = plane<1, 1>(p, prev, k.ly - p.y, k.fr);
@@ -298,6 +325,7 @@ __device__ __forceinline__ void finalize_node(V3 &p, V3 prev, V3 &v, const Tank 
     p = plane<0, -1>(p, prev, p.x - k.hx, k.fr);
     p = plane<2, 1>(p, prev, k.lz - p.z, k.fr);
     p = plane<2, -1>(p, prev, p.z - k.hz, k.fr);
+    if (k.sr > 0.0f) p = sphere(p, prev, k.sx, k.sy, k.sz, k.sr, k.fr);
     v = mk((p.x - prev.x) * k.rate, (p.y - prev.y) * k.rate, (p.z - prev.z) * k.rate);
 }
 

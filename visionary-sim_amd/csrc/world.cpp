@@ -1032,6 +1032,7 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
         P.grid_origin[a] = (float)origin[a];
         P.grid_dims[a] = dims[a];
     }
+    for (int a = 0; a < 4; a++) P.sphere[a] = d->sphere[a];
     P.gain = 1.0f;  // GAIN, bodyContact.js:48
     P.bucket_size = (float)edge;
     P.bucket_count = (uint32_t)nbuckets;

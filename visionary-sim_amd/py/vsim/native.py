@@ -39,6 +39,7 @@ class VsSceneParams(C.Structure):
         ("target", C.c_float * 3), ("response_time", C.c_float),
         ("target_vel", C.c_float * 3), ("drive_law", C.c_uint32),
         ("surf_base", C.c_uint32), ("surf_count", C.c_uint32), ("bucket_slots", C.c_uint32), ("flags", C.c_uint32),
+        ("sphere", C.c_float * 4),
     ]
 
 
@@ -90,7 +91,7 @@ class VsSceneDesc(C.Structure):
         ("contact", C.c_int32), ("bucket_size", C.c_float),
         ("grid_origin", C.c_float * 3), ("grid_dims", C.c_uint32 * 3),
         ("n_driven", C.c_uint32), ("driven", C.POINTER(C.c_uint32)), ("driven_offset", C.POINTER(C.c_float)),
-        ("drive_law", C.c_uint32), ("response_time", C.c_float),
+        ("drive_law", C.c_uint32), ("response_time", C.c_float), ("sphere", C.c_float * 4),
     ]
 
 

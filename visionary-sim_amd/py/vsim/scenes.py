@@ -228,6 +228,7 @@ class Scene:
     bucket_size: float = 0.0      # 0: 2 x the widest contact radius (bodyContact.js:87)
     drive: Drive = None
     name: str = "scene"
+    sphere: tuple = None          # sphere collider (cx, cy, cz, radius); None: none
     _surface: list = field(default=None, repr=False)
 
     # -------- derived layout
@@ -394,6 +395,8 @@ class Scene:
             desc.driven_offset = native.ptr(do)
             desc.drive_law = self.drive.law
             desc.response_time = self.drive.response_time
+        if self.sphere is not None:
+            desc.sphere[:] = [float(v) for v in self.sphere]
         keep.append(bds)
         return desc, keep
 
@@ -466,12 +469,20 @@ def handle_drive(scene, seed, law=0, amplitude=0.1, frequency=1.0):
                  float(rng.uniform(0, 2 * math.pi)), law)
 
 
-def c2_scene(side=64, splats=2_000_000, seed=2):
+def c2_scene(side=64, splats=2_000_000, seed=2, sphere=True):
     """64^3 block resting on the tank floor (ground plane y = min), 2M
-    Gaussians, a 27-node handle on a seeded 0.1 m / 1 Hz sinusoid, hard law."""
+    Gaussians, a 27-node handle on a seeded 0.1 m / 1 Hz sinusoid, hard law,
+    and a sphere collider under the middle of the block: a ball of a quarter
+    of the block's width whose cap stands two cells above the floor, into the
+    block's underside."""
     sc = jelly_block(side, splats=splats, seed=seed, pinned=False, tank_margin=0.5, name="c2")
     sc.tank_min = (sc.tank_min[0], 0.0, sc.tank_min[2])
     sc.drive = handle_drive(sc, seed)
+    if sphere:
+        lat = sc.bodies[0].lattice
+        width = side * lat.h
+        r = 0.25 * width
+        sc.sphere = (lat.origin[0] + 0.5 * width, 2 * lat.h - r, lat.origin[2] + 0.5 * width, r)
     return sc
 
 
