@@ -170,6 +170,43 @@ export const tests = {
         assert.deepStrictEqual(sim.drive.heldNodes, [70]);
         sim.destroy();
     },
+    'idle and singleStep spend nothing from the bank; the bank stays below one step'() {
+        const c = new FixedTimestep(0.01, { maxStepsPerFrame: 3 });
+        c.consume(0.015);
+        close(c.snapshot().backlogTime, 0.005, 1e-12);
+        assert.strictEqual(c.idle().steps, 0);
+        const one = c.singleStep();
+        assert.strictEqual(one.steps, 1);
+        close(one.backlogTime, 0.005, 1e-12);
+        close(one.simulatedTime, 0.02, 1e-12);
+        for (const dt of [0.001, 0.07, 0.0333, 0.25, 0.004]) {
+            const r = c.consume(dt);
+            assert.ok(r.backlogTime >= 0 && r.backlogTime < c.stepDuration);
+            assert.ok(r.steps <= 3);
+        }
+        const t = c.snapshot();
+        close(t.simulatedTime + t.backlogTime + t.droppedTime, 0.015 + 0.01 + 0.001 + 0.07 + 0.0333 + 0.25 + 0.004,
+            1e-12);
+        c.reset();
+        assert.deepStrictEqual([c.snapshot().totalSteps, c.snapshot().backlogTime, c.snapshot().droppedTime], [0, 0, 0]);
+    },
+    'paused frames run only requested steps; driveLaw reads the drive'() {
+        const native = fakeAddon();
+        const backend = new ComputeBackend(0, native);
+        const sim = new Simulator(backend, [jelly(3)], { min: [-1, -1, -1], max: [1, 1, 1] });
+        sim.params.paused = true;
+        sim.step(0.5);
+        assert.strictEqual(native.launches.length, 0);
+        assert.strictEqual(sim.clock.snapshot().steps, 0);
+        sim.requestStep();
+        sim.step(0.5);
+        assert.strictEqual(sim.clock.snapshot().steps, 1);
+        assert.strictEqual(sim.clock.snapshot().backlogTime, 0);
+        sim.setDriveLaw('compliant');
+        assert.strictEqual(sim.driveLaw, 'compliant');
+        sim.destroy();
+        backend.destroy();
+    },
     'construction failure frees what it allocated'() {
         const native = fakeAddon();
         const backend = new ComputeBackend(0, native);

@@ -121,47 +121,56 @@ export class Simulator {
         }, timing);
     }
 
-    /** Advances by one render frame of `dt` seconds (simulator.js:239-278). */
+    /**
+     * Advances by one render frame of `dt` seconds (simulator.js:239-278).
+     * A scene with no nodes has nothing to advance: the clock idles. Input
+     * that is not a finite non-negative number counts as a frame of no time.
+     * A step asked for with requestStep() is taken in this frame: paused, as
+     * the one step of the frame; running, as a step forced on top of the
+     * frame's banked time. Paused with no request, the clock idles.
+     */
     step(dt) {
+        const requested = this.pendingStep;
+        this.pendingStep = false;
         if (this.particles.nodeCount === 0) {
-            this.pendingStep = false;
             this.clock.idle();
             return;
         }
-        const elapsed = Number.isFinite(dt) && dt >= 0 ? dt : 0;
-        const pendingStep = this.pendingStep;
-        this.pendingStep = false;
-        let timing;
-        if (this.params.paused) {
-            if (!pendingStep) {
-                this.clock.idle();
-                return;
-            }
-            timing = this.clock.singleStep();
-        } else {
-            timing = this.clock.consume(elapsed, { forceStep: pendingStep });
+        const frameTime = Number.isFinite(dt) && dt >= 0 ? dt : 0;
+        let timing = null;
+        if (!this.params.paused) timing = this.clock.consume(frameTime, { forceStep: requested });
+        else if (requested) timing = this.clock.singleStep();
+        if (timing === null) {
+            this.clock.idle();
+            return;
         }
         this.dispatch(timing.steps);
     }
 
     requestStep() { this.pendingStep = true; }
 
-    /** All `steps` substeps of a frame in one submission (simulator.js:292-318). */
+    /**
+     * All `steps` substeps of a frame in one submission (simulator.js:292-318).
+     * The uniforms of the frame are its substep, gravity and tank. When
+     * something is held, the drive is told where the grab target is and how
+     * fast it moved: its displacement since the last dispatch spread over
+     * this frame's simulated time.
+     */
     dispatch(steps) {
-        if (steps < 1 || this.particles.nodeCount === 0) return;
-        this.stepper.setFrameParameters({
-            substepDt: this.clock.stepDuration,
-            gravity: this.params.gravity,
-            tankMin: this.tank.min,
-            tankMax: this.tank.max
-        });
-        if (this.drive.drivenCount > 0) {
-            const { position, previousPosition } = this.grabTarget;
-            const frameDt = steps * this.clock.stepDuration;
-            this.drive.setGrabTarget(position, [0, 1, 2].map(a => (position[a] - previousPosition[a]) / frameDt));
-            this.grabTarget.previousPosition = position;
-        }
+        if (!(steps >= 1) || this.particles.nodeCount === 0) return;
+        const substepDt = this.clock.stepDuration;
+        this.stepper.setFrameParameters({ substepDt, gravity: this.params.gravity, tankMin: this.tank.min,
+            tankMax: this.tank.max });
+        if (this.drive.drivenCount > 0) this.aimDrive(steps * substepDt);
         this.backend.dispatch(this.stepper.sequence(steps), 'softbody-step');
+    }
+
+    /** The drive's target for a frame of `frameDt` seconds; the target's last position moves on. */
+    aimDrive(frameDt) {
+        const target = this.grabTarget;
+        const velocity = target.position.map((x, axis) => (x - target.previousPosition[axis]) / frameDt);
+        this.drive.setGrabTarget(target.position, velocity);
+        target.previousPosition = target.position;
     }
 
     /**
@@ -191,15 +200,16 @@ export class Simulator {
             });
         }
         if (generation !== this.grabGeneration || this.destroyed) return null;
-        if (grip.found > grip.nodes.length) {
+        const held = grip.nodes.length;
+        if (held < grip.found) {
             console.warn(`[grip] ${grip.found} nodes within ${this.grip.radius} m, only the nearest ` +
-                `${grip.nodes.length} held -- this grab's extent is resolution-dependent again.`);
+                `${held} held -- this grab's extent is resolution-dependent again.`);
         }
+        // The anchor's live position is where the hand starts: the target
+        // begins there, at rest.
         const position = await this.grabQueries.position(anchor);
-        this.drive.hold(grip.nodes, grip.offsets);
-        this.grabTarget.position = position;
-        this.grabTarget.previousPosition = position;
-        return { bodyIndex: bodyOfNode(this.particles.offsets, anchor), position, held: grip.nodes.length };
+        this.hold(grip.nodes, grip.offsets, position);
+        return { bodyIndex: bodyOfNode(this.particles.offsets, anchor), position, held };
     }
 
     /** Widest node spacing of any body, the aim tolerance's scale (simulator.js:420-423). */
@@ -230,28 +240,43 @@ export class Simulator {
 
     moveGrab(position) { this.grabTarget.position = position.slice(); }
 
-    releaseGrab() { if (!this.destroyed) this.drive.release(); }
+    /** Lets go of the held set; a grab still in flight is abandoned. */
+    releaseGrab() {
+        this.grabGeneration++;
+        if (!this.destroyed) this.drive.release();
+    }
 
     setDriveLaw(law) { this.drive.law = law; }
+
+    /** The drive law in use (simulator.js:487). */
+    get driveLaw() { return this.drive.law; }
 
     setDriveResponseTime(seconds) { this.drive.responseTime = seconds; }
 
     setContact(enabled) { this.contact.enabled = enabled; }
 
+    /**
+     * A material change (simulator.js:332-353). Density rescales the body's
+     * node masses (they belong to the discretisation); damping and friction
+     * are node uniforms; the rest are elastic uniforms. A change of
+     * stiffness or density changes the wave rate, so the clock's step is
+     * re-derived, its banked seconds kept.
+     */
     setMaterial(bodyIndex, changes) {
         if ('maxStretch' in changes) validateMaxStretch(changes.maxStretch);
         const { body, proxy } = this.simBodies[bodyIndex];
-        const rest = Object.assign({}, changes);
-        const density = rest.density;
-        delete rest.density;
-        Object.assign(body.material, rest);
-        if (density > 0 && density !== body.material.density) {
-            proxy.setDensity(body.material.density, density);
-            body.material.density = density;
+        const material = body.material;
+        for (const key of Object.keys(changes)) {
+            if (key !== 'density') material[key] = changes[key];
+        }
+        const density = changes.density;
+        if (density > 0 && density !== material.density) {
+            proxy.setDensity(material.density, density);
+            material.density = density;
             this.particles.uploadMasses();
         }
-        this.particles.setParams(bodyIndex, { damping: body.material.damping, friction: body.material.friction });
-        for (const elastic of this.elasticSets) elastic.uploadParams();
+        this.particles.setParams(bodyIndex, { damping: material.damping, friction: material.friction });
+        this.elasticSets.forEach(set => set.uploadParams());
         this.clock.setStepDuration(stableStepDuration(this.waveRate()));
     }
 
@@ -269,13 +294,15 @@ export class Simulator {
         };
     }
 
+    /** Every body back to rest, nothing held, the clock at zero (simulator.js:495-500). */
     reset() {
-        this.releaseGrab();
-        this.particles.reset();
-        this.clock.reset();
         this.pendingStep = false;
+        this.releaseGrab();
+        this.clock.reset();
+        this.particles.reset();
     }
 
+    /** Live node positions; each call returns its own array (simulator.js:502-503). */
     readPositions() { return this.particles.readPositions(); }
 
     destroy() {
