@@ -13,9 +13,13 @@ The autotuner (vs_world_autotune) picks the launch shapes and schedule first
 
 Prints ONE JSON line. `value` is the batched whole-job node-substeps/s;
 `plain`, `speedup`, frame latency p50/p99, the per-pass profile, the roofline
-of the dominant kernel (the elastic colour-pair pass), each pass's achieved
-GB/s against HBM peak, and the CPU baseline (the f32 oracle on a bounded
-sample, one core) ride along.
+of the dominant kernel (the colour-pair launches, timed inside a frame, with
+the back-to-back figure beside it), each pass's achieved GB/s against HBM
+peak, the single-scene workloads (16^3 block, 64^3 drag block with its
+sphere collider, 512 contact bodies at 2-voxel buckets), the serving loop at
+50 requests/s and saturated, and the CPU baseline (the f32 oracle, this
+repository's C restatement of the reference's WGSL, on a bounded sample,
+all cores) ride along.
 """
 import argparse
 import json
@@ -49,41 +53,69 @@ def _profile_rows(name, kernel):
 def _most_dispatched(name, kernel, column):
     """`column` of the row of `kernel` with the most dispatches in a committed
     profile summary: the plain colour-pair launch of the frame (the first and
-    last pairs, which also run node passes, are fewer), as the back-to-back
-    timing measures it. None when the summary is absent."""
+    last pairs, which also run node passes, are fewer). None when the summary
+    is absent."""
     rows = [r for r in _profile_rows(name, kernel) if len(r) > column]
     if not rows:
         return None
     return float(max(rows, key=lambda r: int(r[3]))[column])
 
 
+def profile_source(name):
+    """Where a committed counter figure comes from: the summary file and the
+    source state it names on its 'Measured at' line."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    stamp = next((l.strip() for l in open(path) if l.startswith("Measured at")), "Measured at: unstated")
+    return f"profiles/{name} ({stamp}); rocprofv3 counters, not measured in this run"
+
+
 def elastic_traffic_bytes(kernel):
-    """HBM bytes per batched launch of `kernel` from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (profiles/batched256_hbm_traffic.md, scaled
-    by the copy calibration)."""
+    """HBM bytes per plain colour-pair launch of `kernel` from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary, scaled by the copy calibration."""
     v = _most_dispatched("batched256_hbm_traffic.md", kernel, 6)
     return v * 1e6 if v is not None else None
 
 
 def elastic_valu_instructions(kernel):
-    """VALU wave instructions per batched launch of `kernel` (rocprofv3
-    SQ_INSTS_VALU, profiles/batched256_valu.md)."""
+    """VALU wave instructions per plain colour-pair launch of `kernel`
+    (rocprofv3 SQ_INSTS_VALU summary)."""
     v = _most_dispatched("batched256_valu.md", kernel, 4)
     return v * 1e6 if v is not None else None
 
 
-def pair_launch_bytes(scs):
-    """Algorithmic bytes of one colour-pair launch, averaged over the row
-    classes: 40 B (p, prev, w read; p written) per node of the rows it covers
-    (class (py, pz): cell rows y = py + 2a, z = pz + 2b, two node rows each)."""
+def pair_class_nodes(scs, cls):
+    """Nodes a colour-pair launch of row class `cls` covers: cell rows y =
+    py + 2a, z = pz + 2b of every dense body, two node rows each way."""
+    py, pz = cls & 1, cls >> 1
     total = 0
     for sc in scs:
         for b in sc.bodies:
             cx, cy, cz = b.lattice.cell_dims
-            for cls in range(4):
-                py, pz = cls & 1, cls >> 1
-                total += (cx + 1) * 2 * ((cy + 1 - py) // 2) * 2 * ((cz + 1 - pz) // 2)
-    return 40.0 * total / 4
+            total += (cx + 1) * 2 * ((cy + 1 - py) // 2) * 2 * ((cz + 1 - pz) // 2)
+    return total
+
+
+def pair_launch_bytes(scs):
+    """Algorithmic bytes of one colour-pair launch averaged over the four row
+    classes: 40 B (p, prev, w read; p written) per covered node."""
+    return 40.0 * sum(pair_class_nodes(scs, c) for c in range(4)) / 4
+
+
+def plain_pair_bytes(scs, substeps, merged=True):
+    """Average algorithmic bytes of the colour-pair launches of a frame that
+    run no node pass (the profile's "pair" kind): with merged node passes,
+    every class-3 launch finalizes and the frame's first class-0 launch
+    predicts, so those are left out."""
+    total, count = 0.0, 0
+    for s in range(substeps):
+        for cls in range(4):
+            if merged and (cls == 3 or (s == 0 and cls == 0)):
+                continue
+            total += 40.0 * pair_class_nodes(scs, cls)
+            count += 1
+    return total / count if count else 0.0
 
 
 def leftover_nodes(scs, pairs=True, merged=True):
@@ -226,10 +258,13 @@ def cpu_baseline(scs, budget_s, cores):
     return out
 
 
-def serve_leg(count, budget_nodes):
-    """Poisson arrivals (50/s) of scenes with log-uniform 4k-500k nodes, each
-    running 120 frames with a seeded drag handle, admitted while a world of
-    `budget_nodes` node capacity holds them; continuous batching."""
+def serve_leg(count, budget_nodes, realtime=True):
+    """Scenes with log-uniform 4k-500k nodes, each running 120 frames with a
+    seeded drag handle, admitted while a world of `budget_nodes` node
+    capacity holds them; continuous batching. `realtime`: the requests
+    arrive as a Poisson process (50/s), so the figure is mostly the arrival
+    rate; otherwise all are waiting at the start, and the server runs at
+    capacity (admission only by the budget)."""
     from vsim import native, scenes
     reqs = scenes.c5_requests(count)
     cells = int(budget_nodes * 1.05)
@@ -238,11 +273,11 @@ def serve_leg(count, budget_nodes):
     srv = native.Server(world, substeps=10, graph=True)
     for t, sc, frames in reqs:
         srv.submit(sc, t, frames)
-    rep = srv.run(realtime=True)
+    rep = srv.run(realtime=realtime)
     lat = srv.latencies()
     srv.close()
     world.close()
-    return {"requests": count, "arrival_rate_per_s": 50, "frames_per_request": 120,
+    return {"requests": count, "arrival_rate_per_s": 50 if realtime else None, "frames_per_request": 120,
             "budget_nodes": budget_nodes, "wall_s": rep["wall_s"],
             "node_substeps_per_s": rep["node_substeps"] / rep["wall_s"],
             "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
@@ -323,13 +358,92 @@ def ingest_leg(cpu):
     return out
 
 
+def workload_scenes(name):
+    """The seeded scenes and config of a named workload."""
+    from vsim import scenes
+    if name == "batched256":
+        return scenes.c4_scenes(256), {
+            "workload": "256 scenes, lattice side U{8..48}, 10k-500k Gaussians each, E 1e5 nu 0.3 rho 1000, "
+                        "h 0.016 m, bottom layer pinned, 10 substeps per 1/60 s frame", "scenes": 256}
+    if name == "c1":
+        return [scenes.c1_scene()], {"workload": "16^3 block, 50k Gaussians, bottom layer pinned, 10 substeps per "
+                                                 "frame", "scenes": 1}
+    if name == "c2":
+        return [scenes.c2_scene()], {"workload": "64^3 block, 2M Gaussians, ground plane and sphere collider, 27-node "
+                                                 "sinusoid handle, 10 substeps per frame", "scenes": 1}
+    voxels = 1.0 if name == "c3_1voxel" else 2.0
+    scs = [scenes.c3_scene(bucket_voxels=voxels)]
+    return scs, {"workload": f"512 bodies of 8^3 cells (h 0.01 m) in a 1 m tank, node-node and node-wall contact, "
+                             f"hash grid of {voxels:g}-voxel buckets, E log-uniform 1e4-1e6, {scs[0].substeps} "
+                             "substeps per frame", "scenes": 1}
+
+
+def timed_frames(world, ids, scs, substeps, steps, warmup, splats=True):
+    """Graph-replayed batched frames: warmup, then `steps` frames between a
+    synchronisation on each side. Returns wall seconds and per-frame ms from
+    HIP events recorded between frames on the world's stream."""
+    for f in range(warmup):
+        set_targets(world, ids, scs, f)
+        world.frame(substeps, splats=splats, graph=True)
+    sync(world)
+    marks = [world.record()]
+    t0 = time.perf_counter()
+    for f in range(steps):
+        set_targets(world, ids, scs, warmup + f)
+        world.frame(substeps, splats=splats, graph=True)
+        marks.append(world.record())
+    sync(world)
+    wall = time.perf_counter() - t0
+    return wall, np.array([world.elapsed_ms(marks[i], marks[i + 1]) for i in range(steps)])
+
+
+def plain_frames(world, ids, scs, substeps, steps, splats=True):
+    """The plain path: per-scene launches, one scene after another."""
+    for sid in ids:
+        world.frame_scene(sid, substeps, splats=splats)
+    sync(world)
+    t1 = time.perf_counter()
+    for f in range(steps):
+        set_targets(world, ids, scs, f)
+        for sid in ids:
+            world.frame_scene(sid, substeps, splats=splats)
+    sync(world)
+    return time.perf_counter() - t1
+
+
+def workload_leg(name, steps, warmup, plain_steps, autotune):
+    """One single-scene workload on the GPU: frame time, node-substeps/s,
+    frame latency p50/p99 (graph-replayed frames), the plain per-scene
+    launches beside it, and the per-pass profile of one instrumented frame."""
+    scs, config = workload_scenes(name)
+    nodes = sum(s.n_nodes for s in scs)
+    substeps = scs[0].substeps
+    splats = sum(s.n_splats for s in scs) > 0
+    world = make_world(scs)
+    ids = [world.add(sc) for sc in scs]
+    sync(world)
+    pick = json.loads(world.autotune(substeps))["best"] if autotune else None
+    wall, frame_ms = timed_frames(world, ids, scs, substeps, steps, warmup, splats)
+    prof = world.profile(substeps, splats=splats)
+    plain_wall = plain_frames(world, ids, scs, substeps, plain_steps, splats) if plain_steps else None
+    world.close()
+    out = {"config": config, "nodes": nodes, "substeps_per_frame": substeps, "steps": steps,
+           "ms_per_frame": wall * 1e3 / steps, "node_substeps_per_s": nodes * substeps * steps / wall,
+           "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
+           "plain_node_substeps_per_s": nodes * substeps * plain_steps / plain_wall if plain_wall else None,
+           "profile_ms": {k: {"total": v[0], "launches": v[1]} for k, v in prof.items() if v[1]},
+           "autotune": pick}
+    log(f"[bench] {name}: {out['ms_per_frame']:.3f} ms/frame, {out['node_substeps_per_s']:.4e} node-substeps/s, "
+        f"p50 {out['frame_latency_ms']['p50']:.3f} p99 {out['frame_latency_ms']['p99']:.3f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="batched256", choices=["batched256", "c1", "c2", "c3"])
-    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--workload", default="batched256", choices=["batched256", "c1", "c2", "c3", "c3_1voxel"])
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of the one-core CPU baseline, and again of the all-cores one")
@@ -337,6 +451,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of the autotuner's pick")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the single-scene workloads (16^3, 64^3 drag, 512 contact bodies) run beside the "
+                         "256-scene one")
     ap.add_argument("--separate-node-passes", action="store_true",
                     help="run each substep's finalize and the next one's predict as two node passes (same "
                          "results; the default merges them inside a frame)")
@@ -344,7 +461,7 @@ def main():
                     help="one launch per colour for every body (same results; the default solves the colours "
                          "of dense lattices in pairs, by rows of cells, one launch per pair)")
     ap.add_argument("--serve-requests", type=int, default=120,
-                    help="scene requests of the serving leg (Poisson 50/s, 4k-500k nodes, 120 frames); 0 skips it")
+                    help="scene requests of the serving legs (4k-500k nodes, 120 frames); 0 skips them")
     ap.add_argument("--serve-budget-nodes", type=float, default=6e6,
                     help="node capacity of the serving world (its HBM budget for node state)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the voxeliser leg")
@@ -372,24 +489,8 @@ def main():
               "communication, so the whole-job figure is N x the one-GPU figure by construction.", file=sys.stderr)
         sys.exit(2)
 
-    from vsim import scenes
-
     t_gen = time.perf_counter()
-    if args.workload == "batched256":
-        scs = scenes.c4_scenes(args.scenes)
-        config = {"workload": f"{args.scenes} scenes, lattice side U{{8..48}}, 10k-500k Gaussians each, "
-                              "E 1e5 nu 0.3 rho 1000, h 0.016 m, bottom layer pinned, 10 substeps per 1/60 s frame",
-                  "scenes": args.scenes}
-    elif args.workload == "c1":
-        scs = [scenes.c1_scene()]
-        config = {"workload": "16^3 block, 50k Gaussians, 10 substeps per frame", "scenes": 1}
-    elif args.workload == "c2":
-        scs = [scenes.c2_scene()]
-        config = {"workload": "64^3 block, 2M Gaussians, 27-node sinusoid handle, 10 substeps per frame", "scenes": 1}
-    else:
-        scs = [scenes.c3_scene()]
-        config = {"workload": "512 bodies of 8^3 cells (h 0.01 m) in a 1 m tank, node-node and node-wall contact, "
-                              f"E log-uniform 1e4-1e6, {scs[0].substeps} substeps per frame", "scenes": 1}
+    scs, config = workload_scenes(args.workload)
     nodes = sum(s.n_nodes for s in scs)
     cells = sum(s.n_cells for s in scs)
     splats = sum(s.n_splats for s in scs)
@@ -404,11 +505,10 @@ def main():
         cpu = cpu_baseline(scs, args.cpu_budget, min(16, os.cpu_count() or 1))
         log(f"[bench] cpu baseline: {cpu['value']:.4e} node-substeps/s on {cpu['cores']} cores ({cpu['sample']})")
 
+    merged = not args.separate_node_passes
     world = make_world(scs, separate_node_passes=int(args.separate_node_passes),
                        separate_colours=int(args.separate_colours))
-    ids = []
-    for sc in scs:
-        ids.append(world.add(sc))
+    ids = [world.add(sc) for sc in scs]
     sync(world)
     log("[bench] scenes resident")
     tune = None
@@ -419,74 +519,67 @@ def main():
             log(f"[bench] autotune {c}")
         log(f"[bench] autotune best {tune['best']}")
 
-    # ---- batched, graph-replayed frames
-    for f in range(args.warmup):
-        set_targets(world, ids, scs, f)
-        world.frame(substeps, splats=True, graph=True)
-    sync(world)
-    marks = [world.record()]
-    t0 = time.perf_counter()
-    for f in range(args.steps):
-        set_targets(world, ids, scs, args.warmup + f)
-        world.frame(substeps, splats=True, graph=True)
-        marks.append(world.record())
-    sync(world)
-    wall = time.perf_counter() - t0
-    frame_ms = np.array([world.elapsed_ms(marks[i], marks[i + 1]) for i in range(args.steps)])
+    # ---- batched, graph-replayed frames (the timed region)
+    wall, frame_ms = timed_frames(world, ids, scs, substeps, args.steps, args.warmup, splats > 0)
     value = nodes * substeps * args.steps / wall
     log(f"[bench] batched: {wall * 1e3 / args.steps:.3f} ms/frame, {value:.4e} node-substeps/s")
 
-    # ---- per-pass profile of one batched frame (HIP events around each launch)
-    prof = world.profile(substeps, splats=True)
+    # ---- per-pass profile of one batched frame, HIP events around each
+    # launch on the world's stream (the stream the kernels run on)
+    prof = world.profile(substeps, splats=splats > 0)
     log(f"[bench] profile: " + ", ".join(f"{k} {v[0]:.3f} ms/{v[1]}" for k, v in prof.items() if v[1]))
-    # ---- the colour pass alone: 8 x substeps launches back to back on the
-    # world's stream, HIP events only at the two ends (state restored after)
+    # ---- the colour passes alone: rounds of a substep's colour launches
+    # back to back, HIP events only at the two ends (state restored after)
     el_ms, el_n = world.time_pass("elastic", substeps)
-    el_avg_s = el_ms / el_n / 1e3
-    # One sweep is 8 colour launches, or 4 colour-pair launches (rows of a
-    # dense lattice read no cell list: 320 B a cell and colour). The bytes
-    # are the per-colour model's for the cells a launch solves, whatever the
-    # launch actually moves (a pair moves each node once for two colours).
+    burst_us = el_ms / el_n * 1e3
     per_sweep = el_n // substeps
     pairs = per_sweep == 4
     el_kernel = "k_elastic_rows" if pairs else "k_elastic"
     colour_model_bytes = cells * (ELASTIC_BYTES_PER_CELL - 4 if pairs else ELASTIC_BYTES_PER_CELL) / per_sweep
-    # A colour-pair launch reads p, prev, w (28 B) and writes p (12 B) of the
-    # nodes its rows cover, once for both colours: those are its algorithmic
-    # bytes, averaged over the four row classes.
-    el_bytes = pair_launch_bytes(scs) if pairs else colour_model_bytes
-    achieved = el_bytes / el_avg_s / 1e9
-    log(f"[bench] colour pass: {el_n} launches, {el_avg_s * 1e6:.2f} us avg, {achieved:.0f} GB/s algorithmic")
+    if pairs and prof["pair"][1]:
+        # The dominant kernel as the frame runs it: the colour-pair launches
+        # with no node pass on board, timed inside the frame.
+        kernel_bytes = plain_pair_bytes(scs, substeps, merged)
+        kernel_us = prof["pair"][0] / prof["pair"][1] * 1e3
+        burst_bytes = pair_launch_bytes(scs)
+    else:
+        kernel_bytes = burst_bytes = colour_model_bytes
+        kernel_us = prof["elastic"][0] / prof["elastic"][1] * 1e3
+    achieved = kernel_bytes / (kernel_us * 1e-6) / 1e9
+    burst_gbs = burst_bytes / (burst_us * 1e-6) / 1e9
+    log(f"[bench] colour pair in frame: {kernel_us:.2f} us avg, {achieved:.0f} GB/s algorithmic; back to back "
+        f"{burst_us:.2f} us, {burst_gbs:.0f} GB/s")
+    counted = args.workload == "batched256" and not args.separate_colours
+    valu = elastic_valu_instructions(el_kernel) if counted else None
+    valu_frac = valu / (kernel_us * 1e-6 * VALU_ISSUE_PER_S) if valu else None
+    hbm_frac = achieved / HBM_PEAK_GBS
     # ---- the splat pass alone, covariance output and with the engine format
     # (rotation + axis lengths by four Jacobi sweeps) as well
     from vsim import native
     splat_us = {}
-    for label, mode in (("cov", native.SPLAT_COV), ("cov+axes", native.SPLAT_COV | native.SPLAT_AXES)):
-        world.frame(0, mode, False)
-        ms, n = world.time_pass("splat", 3)
-        splat_us[label] = ms / n * 1e3 if n else None
-    log(f"[bench] splat pass: " + ", ".join(f"{k} {v:.1f} us" for k, v in splat_us.items() if v))
+    if splats:
+        for label, mode in (("cov", native.SPLAT_COV), ("cov+axes", native.SPLAT_COV | native.SPLAT_AXES)):
+            world.frame(0, mode, False)
+            ms, n = world.time_pass("splat", 3)
+            splat_us[label] = ms / n * 1e3 if n else None
+        log(f"[bench] splat pass: " + ", ".join(f"{k} {v:.1f} us" for k, v in splat_us.items() if v))
 
     # ---- plain per-scene launches (the reference's dispatch shape)
     plain = None
     if not args.no_plain and args.plain_steps > 0:
-        for sid in ids:
-            world.frame_scene(sid, substeps, splats=True)
-        sync(world)
-        t1 = time.perf_counter()
-        for f in range(args.plain_steps):
-            set_targets(world, ids, scs, f)
-            for sid in ids:
-                world.frame_scene(sid, substeps, splats=True)
-        sync(world)
-        plain_wall = time.perf_counter() - t1
+        plain_wall = plain_frames(world, ids, scs, substeps, args.plain_steps, splats > 0)
         plain = nodes * substeps * args.plain_steps / plain_wall
         log(f"[bench] plain: {plain_wall * 1e3 / args.plain_steps:.3f} ms/frame, {plain:.4e} node-substeps/s")
     world.close()
 
+    legs = None
+    if args.workload == "batched256" and not args.no_legs:
+        legs = {name: workload_leg(name, 20, 3, 3, not args.no_autotune) for name in ("c1", "c2", "c3")}
+
     serving = None
     if args.serve_requests > 0:
-        serving = serve_leg(args.serve_requests, int(args.serve_budget_nodes))
+        serving = {"poisson_50_per_s": serve_leg(args.serve_requests, int(args.serve_budget_nodes), realtime=True),
+                   "saturated": serve_leg(args.serve_requests, int(args.serve_budget_nodes), realtime=False)}
         log(f"[bench] serving: {serving}")
 
     ingest_out = None
@@ -494,7 +587,21 @@ def main():
         ingest_out = ingest_leg(cpu=not args.no_cpu)
         log(f"[bench] ingest: {ingest_out}")
 
-
+    roofline = {
+        "bound": "valu" if valu_frac and valu_frac > hbm_frac else "hbm",
+        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_frac,
+        "traffic": elastic_traffic_bytes(el_kernel) if counted else None,
+        "kernel": el_kernel + (" (colour pair, no node pass on board)" if pairs else " (colour pass)"),
+        "bytes_per_launch": kernel_bytes,
+        "avg_launch_us": kernel_us,
+        "timing": "HIP events around each launch of one batched frame on the world's stream",
+        "back_to_back": {"avg_launch_us": burst_us, "bytes_per_launch": burst_bytes, "achieved": burst_gbs,
+                         "frac": burst_gbs / HBM_PEAK_GBS},
+        "colour_model_bytes_per_launch": colour_model_bytes,
+        "valu_issue_frac": valu_frac,
+        "valu_wave_instructions_per_launch": valu,
+        "counters_source": profile_source("batched256_valu.md") if counted else None,
+    }
     out = {
         "metric": "node-substeps/s, batched vs plain, at \u22641e-5 rel error; p50/p99 frame latency",
         "value": value,
@@ -514,19 +621,10 @@ def main():
         "frame_latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99))},
         "nodes": nodes, "cells": cells, "splats": splats, "substeps_per_frame": substeps,
         "profile_ms": {k: {"total": v[0], "launches": v[1]} for k, v in prof.items() if v[1]},
-        "passes": pass_rates(prof, scs, substeps, splats, el_bytes, pairs, not args.separate_node_passes),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     # counter figures are profiled on the default workload only
-                     "traffic": elastic_traffic_bytes(el_kernel) if args.workload == "batched256" else None,
-                     "kernel": el_kernel + (" (colour pair)" if pairs else " (colour pass)"),
-                     "bytes_per_launch": el_bytes,
-                     "colour_model_bytes_per_launch": colour_model_bytes,
-                     "avg_launch_us": el_avg_s * 1e6,
-                     "avg_launch_us_with_events": prof["elastic"][0] / prof["elastic"][1] * 1e3,
-                     "valu_issue_frac": (lambda v: v / (el_avg_s * VALU_ISSUE_PER_S) if v else None)(
-                         elastic_valu_instructions(el_kernel) if args.workload == "batched256" else None)},
+        "passes": pass_rates(prof, scs, substeps, splats, burst_bytes, pairs, merged),
+        "roofline": roofline,
         "splat_pass_us": splat_us,
+        "workloads": legs,
         "serving": serving,
         "ingest": ingest_out,
         "cpu_baseline": cpu,

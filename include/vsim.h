@@ -383,7 +383,14 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out);
 #define VS_P_FINALIZE 7
 #define VS_P_SPLAT 8
 #define VS_P_FINALIZE_PREDICT 9 /* a substep's finalize and the next one's predict */
-#define VS_P_COUNT 10
+/* Breakdown of VS_P_ELASTIC in vs_world_profile (the same launches, counted
+ * again): colour-pair launches with no node pass on board, and colour-pair
+ * launches asked to run one (the first pair of a frame predicts, the last
+ * pair of a substep finalizes, in scenes flagged VS_SCENE_FOLD_PREDICT /
+ * VS_SCENE_FOLD). Not accepted by vs_world_time_pass. */
+#define VS_P_PAIR 10
+#define VS_P_PAIR_NODES 11
+#define VS_P_COUNT 12
 /* One frame of every admitted scene, batched but not captured, with HIP
  * events around every launch: ms[k] is the summed duration of the launches of
  * kind k, launches[k] their number. Synchronous. */

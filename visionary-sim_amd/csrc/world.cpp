@@ -608,6 +608,7 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
         const bool timed = prof && grid_for(w, p, scene) != 0;
         if (timed) {
             prof->marks.emplace_back(kind, prof->used);
+            if (kind == VS_P_ELASTIC && color >= 8) prof->marks.emplace_back(node_mode ? VS_P_PAIR_NODES : VS_P_PAIR, prof->used);
             HIPW(w, hipEventRecord(prof->next(), st));
         }
         if (int rc = enqueue_pass(w, kind, color, scene, limit_only, node_mode)) return rc;
@@ -1500,7 +1501,7 @@ int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, 
 
 int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, uint32_t *launches)
 {
-    if (!w || !ms || !launches || rounds == 0 || kind < 0 || kind >= VS_P_COUNT) return VS_E_INVALID;
+    if (!w || !ms || !launches || rounds == 0 || kind < 0 || kind >= VS_P_PAIR) return VS_E_INVALID;
     // Repeated contact launches would leave the grid's counters and the
     // overflow list counted up (only the node planes are restored), so the
     // contact kinds are timed inside whole frames (vs_world_profile) only.

@@ -133,7 +133,7 @@ EXPORTS = [
 SPLAT_COV, SPLAT_AXES = 1, 2  # VS_SPLAT_* output bits of a frame's `splats`
 
 PASS_KINDS = ["predict", "drive", "elastic", "bin", "sort", "collide", "settle", "finalize", "splat",
-              "finalize+predict"]
+              "finalize+predict", "pair", "pair+nodes"]
 
 class VsOccupancy(C.Structure):
     _fields_ = [("dims", C.c_uint32 * 3), ("pad0", C.c_uint32), ("origin", C.c_double * 3), ("h", C.c_double),
