@@ -463,3 +463,26 @@ def test_sphere_collider_frames(drive, schedule):
     assert rel_err(mean, o.out_mean) <= RTOL and rel_err(cov, o.out_cov) <= RTOL
     c = np.asarray(sc.sphere[:3], np.float64)[:, None]
     assert np.all(np.linalg.norm(pos.astype(np.float64) - c, axis=0) >= sc.sphere[3] * (1 - 1e-5))
+
+
+def test_poisson_stream_results_equal_each_request_served_alone():
+    """A shortened serving stream (Poisson arrivals at 50/s, seeded drag
+    handles, 4k-60k nodes, 20 frames each) served in real time: requests
+    that arrive while others run are admitted into the running batch, and
+    each result is bit-identical to the same request served alone."""
+    reqs = scenes.c5_requests(16, max_nodes=60_000, frames=20)
+    budget = 160_000
+    cells = int(budget * 1.05)
+    w = native.World(max_scenes=16, max_bodies=16, max_nodes=budget, max_cells=cells, max_driven=16 * 27,
+                     max_splats=1024, persistent=512)
+    srv = native.Server(w, substeps=10, graph=True, keep_results=True)
+    ids = [srv.submit(sc, t, frames) for t, sc, frames in reqs]
+    rep = srv.run(realtime=True)
+    print(rep)
+    assert rep["completed"] == len(reqs) and rep["max_active"] >= 2
+    for (t, sc, frames), rid in zip(reqs, ids):
+        alone_srv, _, alone_ids, _ = serve([(sc, frames)], sc.n_nodes + 16)
+        assert np.array_equal(alone_srv.result(alone_ids[0]), srv.result(rid)), sc.name
+        alone_srv.close()
+    srv.close()
+    w.close()
