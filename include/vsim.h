@@ -62,7 +62,7 @@ typedef struct vs_body {
 
 /* Per-scene uniforms: the Params blocks of predict.wgsl:1-5,
  * finalize.wgsl:1-6, contacts.wgsl:1-8 and drive.wgsl:1-7 in one record,
- * plus the sphere collider. 144 bytes. */
+ * plus the sphere collider and two contact bounds. 160 bytes. */
 typedef struct vs_scene_params {
     float dt;
     float gravity[3];
@@ -86,6 +86,12 @@ typedef struct vs_scene_params {
                               No reference counterpart: finalize moves a node
                               inside it out along the radius, with the tank
                               planes' friction (finalize.wgsl:29-42)          */
+    float reach_max;       /* contact: twice the largest record radius; 0 = no bound */
+    float skip_margin;     /* contact: rounding allowance of the bucket bound, metres.
+                              A neighbour bucket farther than reach_max +
+                              skip_margin from a record along an axis holds no
+                              mate in reach, and collide may pass it by        */
+    uint32_t pad1[2];
 } vs_scene_params;
 
 /* A scene whose finalize follows its last colour directly (no contact and

@@ -33,5 +33,5 @@ def test_kernel_ids_follow_the_reference_pass_labels():
 
 def test_struct_sizes_match_the_header():
     assert ctypes.sizeof(native.VsBody) == 80
-    assert ctypes.sizeof(native.VsSceneParams) == 144
+    assert ctypes.sizeof(native.VsSceneParams) == 160
     assert ctypes.sizeof(native.VsLaunch) == 32 + 80

@@ -306,7 +306,7 @@ int vs_dispatch(vs_backend *b, const vs_launch *launches, uint32_t count)
                 int32_t *c = reinterpret_cast<int32_t *>(l.buffers[VS_B_OUT]);
                 log = PairLog{ c, reinterpret_cast<uint32_t *>(c + I), l.pair_cap };
             }
-            e = launch_contact_collide(b->stream, sh, n, bodies, scenes, sv, g, t, log);
+            e = launch_contact_collide(b->stream, sh, n, bodies, scenes, sv, g, t, log, 256u);
             break;
         }
         case VS_K_CONTACT_SETTLE: e = launch_contact_settle(b->stream, sh, n, scenes, sv, g, t); break;

@@ -170,9 +170,11 @@ hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scen
                               SurfaceView sv, GridView g, TileView t);
 hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv,
                                GridView g, TileView t);
+// With sv.pay, nine lanes a record over tiles of at most `tile_max` records;
+// without, one lane a record through the node ids (the reference's form).
 hipError_t launch_contact_collide(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
                                   const vs_scene_params *scenes, SurfaceView sv, GridView g, TileView t,
-                                  PairLog log);
+                                  PairLog log, uint32_t tile_max);
 hipError_t launch_contact_settle(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes,
                                  SurfaceView sv, GridView g, TileView t);
 hipError_t launch_drive(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, DriveView d,

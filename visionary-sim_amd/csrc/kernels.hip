@@ -1035,15 +1035,50 @@ __device__ __forceinline__ void collide_mate(Collider &c, uint32_t other, uint32
     c.hits += 1.0f;
 }
 
-// collide.wgsl:16-125: per record, the 3x3x3 neighbour buckets in dz, dy, dx
-// order and their slots ascending; Jacobi (writes only its own record).
-//
-// With the bin pass's payload (sv.pay), a mate's position and radius come
-// with one load, the 27 bucket counts are loaded together first, and the
-// first slot of each bucket of a z-plane is loaded for all nine buckets
-// before any is resolved: a lane waits on a few rounds of independent loads
-// instead of a chain of dependent ones per bucket. The order in which mates
-// are resolved, and so the sum, is the same.
+// The reach test of collide.wgsl:52-57 (another body, closer than the two
+// radii), in collide_mate's arithmetic.
+__device__ __forceinline__ bool reaches(const Collider &c, uint32_t mateBody, V3 pj, float rj)
+{
+    if (mateBody == c.body) return false;
+    const V3 d = sub(c.pi, pj);
+    const float reach = c.radius + rj;
+    return dot(d, d) < reach * reach;
+}
+
+// A record's own data (collide.wgsl:20-40) and its bucket's coordinates.
+__device__ __forceinline__ void load_collider(Collider &c, uint32_t s, NodeView n, const vs_body *__restrict__ bodies,
+                                              SurfaceView sv, int dx, int dy, int &cx, int &cy, int &cz)
+{
+    c.s = s;
+    c.node = sv.node[s];
+    c.body = sv.body[s];
+    c.radius = sv.radius[s];
+    c.wi = n.w[c.node];
+    c.pi = ld3(n.px, n.py, n.pz, c.node);
+    c.movedI = sub(c.pi, ld3(n.qx, n.qy, n.qz, c.node));
+    c.frictionI = bodies[c.body].friction;
+    c.sum = mk(0.f, 0.f, 0.f);
+    c.hits = 0.f;
+    c.logged = 0;
+    const uint32_t key = sv.key[s];
+    cx = (int)(key % (uint32_t)dx);
+    cy = (int)((key / (uint32_t)dx) % (uint32_t)dy);
+    cz = (int)(key / (uint32_t)(dx * dy));
+}
+
+__device__ __forceinline__ void store_collider(const Collider &c, SurfaceView sv, PairLog log)
+{
+    if (log.cap) log.count[c.s] = c.logged;
+    sv.dx[c.s] = c.sum.x;
+    sv.dy[c.s] = c.sum.y;
+    sv.dz[c.s] = c.sum.z;
+    sv.hits[c.s] = c.hits;
+}
+
+// collide.wgsl:16-125 as the reference dispatches it: one lane per record,
+// the 3x3x3 neighbour buckets in dz, dy, dx order, their slots ascending,
+// each mate's position read through its node; Jacobi (a record writes only
+// its own correction). The layer-1 dispatch runs this form.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_collide(NodeView n, const vs_body *__restrict__ bodies,
                                                    const vs_scene_params *__restrict__ scenes, SurfaceView sv,
@@ -1057,112 +1092,177 @@ __global__ __launch_bounds__(BLOCK) void k_collide(NodeView n, const vs_body *__
         const int dx = (int)sc.grid_dims[0], dy = (int)sc.grid_dims[1], dz = (int)sc.grid_dims[2];
         for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
             Collider c;
-            c.s = tile.begin + i;
-            c.node = sv.node[c.s];
-            c.body = sv.body[c.s];
-            c.radius = sv.radius[c.s];
-            c.wi = n.w[c.node];
-            c.pi = ld3(n.px, n.py, n.pz, c.node);
-            c.movedI = sub(c.pi, ld3(n.qx, n.qy, n.qz, c.node));
-            c.frictionI = bodies[c.body].friction;
-            c.sum = mk(0.f, 0.f, 0.f);
-            c.hits = 0.f;
-            c.logged = 0;
-            const uint32_t key = sv.key[c.s];
-            const int cx = (int)(key % (uint32_t)dx), cy = (int)((key / (uint32_t)dx) % (uint32_t)dy),
-                      cz = (int)(key / (uint32_t)(dx * dy));
-            if (sv.pay) {
-                // Bucket m = 9 (oz+1) + 3 (oy+1) + (ox+1); out of the grid holds 0.
-                uint32_t held[27];
-#pragma unroll
-                for (int m = 0; m < 27; m++) {
-                    const int x = cx + m % 3 - 1, y = cy + (m / 3) % 3 - 1, z = cz + m / 9 - 1;
-                    const bool in = x >= 0 && x < dx && y >= 0 && y < dy && z >= 0 && z < dz;
-                    const uint32_t cnt = in ? g.count[gb + (uint32_t)(x + dx * (y + dy * z))] : 0u;
-                    held[m] = cnt < slots ? cnt : slots;
-                }
-#pragma unroll
-                for (int plane = 0; plane < 3; plane++) {
-                    uint32_t first[9], mb[9];
-                    float4 pl[9];
-#pragma unroll
-                    for (int j = 0; j < 9; j++) {
-                        const int m = plane * 9 + j;
-                        first[j] = 0u;
-                        if (held[m]) {
-                            const int x = cx + m % 3 - 1, y = cy + (m / 3) % 3 - 1, z = cz + m / 9 - 1;
-                            first[j] = g.slots[(size_t)(gb + (uint32_t)(x + dx * (y + dy * z))) * slots];
-                        }
-                    }
-#pragma unroll
-                    for (int j = 0; j < 9; j++) {
-                        const int m = plane * 9 + j;
-                        mb[j] = 0u;
-                        pl[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (held[m]) {
-                            mb[j] = sv.body[first[j]];
-                            pl[j] = sv.pay[first[j]];
-                        }
-                    }
-                    // Buckets to visit in order: the first slot within reach
-                    // of another body, or more than one slot held. Contacts
-                    // are rare next to reach tests, so the visits reload what
-                    // they need and run through one copy of the contact code.
-                    uint32_t visit = 0u;
-#pragma unroll
-                    for (int j = 0; j < 9; j++) {
-                        const int m = plane * 9 + j;
-                        const V3 d = sub(c.pi, mk(pl[j].x, pl[j].y, pl[j].z));
-                        const float reach = c.radius + pl[j].w;
-                        const bool near = held[m] && mb[j] != c.body && dot(d, d) < reach * reach;
-                        visit |= (near || held[m] > 1u ? 1u : 0u) << j;
-                    }
-                    while (visit) {
-                        const int j = __builtin_ctz(visit);
-                        visit &= visit - 1u;
-                        const int m = plane * 9 + j;
-                        const int x = cx + m % 3 - 1, y = cy + (m / 3) % 3 - 1, z = cz + m / 9 - 1;
+            int cx, cy, cz;
+            load_collider(c, tile.begin + i, n, bodies, sv, dx, dy, cx, cy, cz);
+            for (int oz = -1; oz <= 1; oz++) {
+                const int z = cz + oz;
+                if (z < 0 || z >= dz) continue;
+                for (int oy = -1; oy <= 1; oy++) {
+                    const int y = cy + oy;
+                    if (y < 0 || y >= dy) continue;
+                    for (int ox = -1; ox <= 1; ox++) {
+                        const int x = cx + ox;
+                        if (x < 0 || x >= dx) continue;
                         const uint32_t bk = gb + (uint32_t)(x + dx * (y + dy * z));
                         const uint32_t cnt = g.count[bk];
-                        const uint32_t hk = cnt < slots ? cnt : slots;
+                        const uint32_t held = cnt < slots ? cnt : slots;
                         const uint32_t *base = g.slots + (size_t)bk * slots;
-                        for (uint32_t k = 0; k < hk; k++) {
+                        for (uint32_t k = 0; k < held; k++) {
                             const uint32_t other = base[k];
+                            const uint32_t mate = sv.node[other];
+                            collide_mate(c, other, sv.body[other], ld3(n.px, n.py, n.pz, mate), sv.radius[other], n,
+                                         bodies, sv, log);
+                        }
+                    }
+                }
+            }
+            store_collider(c, sv, log);
+        }
+    }
+}
+
+// The same pass over bin's payload (sv.pay: a record's position and radius
+// beside it), nine lanes a record. A record's 27 neighbour buckets are nine
+// rows of three along x, (dz, dy) = row / 3 - 1, row % 3 - 1; lane `row` of
+// the record's group runs the reach tests of its row (loads four slots at a
+// time, ids then bodies and payloads) and lists, in LDS, the mates that pass.
+// The group's first lane then resolves the listed mates row by row, in
+// list order: the contacts of the one-lane pass, in its order, so the sums,
+// hit counts and logged pairs are the same. A row with more than ROW_KEEP
+// mates in reach is resolved by the first lane from the grid itself. A
+// wave holds seven records (lane 63 idles); a workgroup takes WAVES x 7
+// records of a tile at a time, so a tile is `parts` work items.
+constexpr uint32_t ROW_LANES = 9;
+constexpr uint32_t ROW_GROUPS = 64 / ROW_LANES;
+constexpr uint32_t ROW_KEEP = 16;
+
+template <int BLOCK>
+constexpr uint32_t collide_parts(uint32_t tile_max)
+{
+    return (tile_max + (BLOCK / 64) * ROW_GROUPS - 1) / ((BLOCK / 64) * ROW_GROUPS);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_collide_rows(NodeView n, const vs_body *__restrict__ bodies,
+                                                        const vs_scene_params *__restrict__ scenes, SurfaceView sv,
+                                                        GridView g, TileView tv, PairLog log, uint32_t parts)
+{
+    constexpr uint32_t WAVES = BLOCK / 64;
+    constexpr uint32_t PER_PART = WAVES * ROW_GROUPS;
+    __shared__ uint32_t s_mates[WAVES][64][ROW_KEEP];
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t group = lane / ROW_LANES, row = lane - group * ROW_LANES;
+    const uint32_t items = *tv.count * parts;
+    for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t t = it / parts, part = it - t * parts;
+        const vs_tile tile = tv.tiles[t];
+        const uint32_t first = part * PER_PART;
+        if (first >= tile.count) continue;  // the whole workgroup: `it` is its own
+        const vs_scene_params &sc = scenes[tile.scene];
+        const uint32_t slots = sc.bucket_slots, gb = sc.grid_base;
+        const int dx = (int)sc.grid_dims[0], dy = (int)sc.grid_dims[1], dz = (int)sc.grid_dims[2];
+        const uint32_t i = first + wv * ROW_GROUPS + group;
+        const bool active = group < ROW_GROUPS && i < tile.count;
+        Collider c;
+        int cx = 0, cy = 0, cz = 0;
+        uint32_t listed = 0;
+        if (active) {
+            load_collider(c, tile.begin + i, n, bodies, sv, dx, dy, cx, cy, cz);
+            // Neighbour buckets beyond reach: along an axis, the bucket below
+            // when the record sits farther than the largest reach (plus the
+            // rounding margin) above its bucket's lower face, the bucket
+            // above likewise. Mates there fail the reach test, so passing
+            // them by changes nothing (with the reference's default grid,
+            // buckets twice the widest radius, nothing is ever passed by).
+            bool lo[3] = { false, false, false }, hi[3] = { false, false, false };
+            if (sc.reach_max > 0.0f) {
+                const float bound = sc.reach_max + sc.skip_margin, size = sc.bucket_size;
+                const float pc[3] = { c.pi.x, c.pi.y, c.pi.z };
+                const int cell[3] = { cx, cy, cz };
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const float f = (pc[a] - sc.grid_origin[a]) / size - (float)cell[a];
+                    lo[a] = f * size > bound;
+                    hi[a] = (1.0f - f) * size > bound;
+                }
+            }
+            const int oz = (int)(row / 3u) - 1, oy = (int)(row % 3u) - 1;
+            const int z = cz + oz, y = cy + oy;
+            const bool in_row = z >= 0 && z < dz && y >= 0 && y < dy && !(oz < 0 && lo[2]) && !(oz > 0 && hi[2]) &&
+                                !(oy < 0 && lo[1]) && !(oy > 0 && hi[1]);
+            uint32_t held[3], bk[3];
+#pragma unroll
+            for (int ox = 0; ox < 3; ox++) {
+                const int x = cx + ox - 1;
+                const bool in = in_row && x >= 0 && x < dx && !(ox == 0 && lo[0]) && !(ox == 2 && hi[0]);
+                bk[ox] = in ? gb + (uint32_t)(x + dx * (y + dy * z)) : 0u;
+                const uint32_t cnt = in ? g.count[bk[ox]] : 0u;
+                held[ox] = cnt < slots ? cnt : slots;
+            }
+#pragma unroll
+            for (int ox = 0; ox < 3; ox++) {
+                const uint32_t *base = g.slots + (size_t)bk[ox] * slots;
+                for (uint32_t k0 = 0; k0 < held[ox]; k0 += 4u) {
+                    uint32_t o[4], mb[4];
+                    float4 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) o[u] = k0 + u < held[ox] ? base[k0 + u] : 0u;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        mb[u] = c.body;  // past the bucket's end: the record's own body, never in reach
+                        q[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (k0 + u < held[ox]) {
+                            mb[u] = sv.body[o[u]];
+                            q[u] = sv.pay[o[u]];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        if (reaches(c, mb[u], mk(q[u].x, q[u].y, q[u].z), q[u].w)) {
+                            if (listed < ROW_KEEP) s_mates[wv][lane][listed] = o[u];
+                            listed++;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // Every lane reads the group's nine counts (shuffles take the whole wave).
+        uint32_t counts[ROW_LANES];
+        const uint32_t lead = group * ROW_LANES;
+#pragma unroll
+        for (uint32_t k = 0; k < ROW_LANES; k++) counts[k] = __shfl(listed, (int)min(lead + k, 63u));
+        if (active && row == 0) {
+#pragma unroll
+            for (uint32_t k = 0; k < ROW_LANES; k++) {
+                if (counts[k] <= ROW_KEEP) {
+                    for (uint32_t j = 0; j < counts[k]; j++) {
+                        const uint32_t other = s_mates[wv][lead + k][j];
+                        const float4 q = sv.pay[other];
+                        collide_mate(c, other, sv.body[other], mk(q.x, q.y, q.z), q.w, n, bodies, sv, log);
+                    }
+                } else {
+                    // (The row listed mates, so z and y are inside the grid;
+                    // buckets passed by above add only failed reach tests.)
+                    const int z = cz + (int)(k / 3u) - 1, y = cy + (int)(k % 3u) - 1;
+                    for (int ox = -1; ox <= 1; ox++) {
+                        const int x = cx + ox;
+                        if (x < 0 || x >= dx) continue;
+                        const uint32_t bkk = gb + (uint32_t)(x + dx * (y + dy * z));
+                        const uint32_t cnt = g.count[bkk];
+                        const uint32_t h = cnt < slots ? cnt : slots;
+                        const uint32_t *base = g.slots + (size_t)bkk * slots;
+                        for (uint32_t k2 = 0; k2 < h; k2++) {
+                            const uint32_t other = base[k2];
                             const float4 q = sv.pay[other];
                             collide_mate(c, other, sv.body[other], mk(q.x, q.y, q.z), q.w, n, bodies, sv, log);
                         }
                     }
                 }
-            } else {
-                for (int oz = -1; oz <= 1; oz++) {
-                    const int z = cz + oz;
-                    if (z < 0 || z >= dz) continue;
-                    for (int oy = -1; oy <= 1; oy++) {
-                        const int y = cy + oy;
-                        if (y < 0 || y >= dy) continue;
-                        for (int ox = -1; ox <= 1; ox++) {
-                            const int x = cx + ox;
-                            if (x < 0 || x >= dx) continue;
-                            const uint32_t bk = gb + (uint32_t)(x + dx * (y + dy * z));
-                            const uint32_t cnt = g.count[bk];
-                            const uint32_t held = cnt < slots ? cnt : slots;
-                            const uint32_t *base = g.slots + (size_t)bk * slots;
-                            for (uint32_t k = 0; k < held; k++) {
-                                const uint32_t other = base[k];
-                                const uint32_t mate = sv.node[other];
-                                collide_mate(c, other, sv.body[other], ld3(n.px, n.py, n.pz, mate), sv.radius[other], n,
-                                             bodies, sv, log);
-                            }
-                        }
-                    }
-                }
             }
-            if (log.cap) log.count[c.s] = c.logged;
-            sv.dx[c.s] = c.sum.x;
-            sv.dy[c.s] = c.sum.y;
-            sv.dz[c.s] = c.sum.z;
-            sv.hits[c.s] = c.hits;
+            store_collider(c, sv, log);
         }
+        __syncthreads();  // the lists are rewritten by the next item
     }
 }
 
@@ -1452,9 +1552,18 @@ hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *s
 }
 
 hipError_t launch_contact_collide(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
-                                  const vs_scene_params *scenes, SurfaceView sv, GridView g, TileView t, PairLog log)
+                                  const vs_scene_params *scenes, SurfaceView sv, GridView g, TileView t, PairLog log,
+                                  uint32_t tile_max)
 {
-    VSIM_DISPATCH_BLOCK(k_collide, sh, s, n, bodies, scenes, sv, g, t, log);
+    if (!sv.pay) VSIM_DISPATCH_BLOCK(k_collide, sh, s, n, bodies, scenes, sv, g, t, log);
+    // Row lanes: each tile is `parts` work items of WAVES x 7 records.
+    switch (sh.block) {
+    case 64: sh.grid *= collide_parts<64>(tile_max); VSIM_DISPATCH_BLOCK(k_collide_rows, sh, s, n, bodies, scenes, sv, g, t, log, collide_parts<64>(tile_max));
+    case 128: sh.grid *= collide_parts<128>(tile_max); VSIM_DISPATCH_BLOCK(k_collide_rows, sh, s, n, bodies, scenes, sv, g, t, log, collide_parts<128>(tile_max));
+    case 256: sh.grid *= collide_parts<256>(tile_max); VSIM_DISPATCH_BLOCK(k_collide_rows, sh, s, n, bodies, scenes, sv, g, t, log, collide_parts<256>(tile_max));
+    case 512: sh.grid *= collide_parts<512>(tile_max); VSIM_DISPATCH_BLOCK(k_collide_rows, sh, s, n, bodies, scenes, sv, g, t, log, collide_parts<512>(tile_max));
+    default: return hipErrorInvalidConfiguration;
+    }
 }
 
 hipError_t launch_contact_settle(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,

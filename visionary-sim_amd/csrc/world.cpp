@@ -576,7 +576,7 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
         const PairLog log{ w->d_pairlog,
                            w->d_pairlog ? reinterpret_cast<uint32_t *>(w->d_pairlog + w->cfg.max_surface) : nullptr,
                            w->pair_cap };
-        HIPW(w, launch_contact_collide(st, sh, n, w->d_bodies, w->d_scenes, sv, g, t, log));
+        HIPW(w, launch_contact_collide(st, sh, n, w->d_bodies, w->d_scenes, sv, g, t, log, w->tile_surf));
         break;
     }
     case VS_P_SETTLE: HIPW(w, launch_contact_settle(st, sh, n, w->d_scenes, sv, g, t)); break;
@@ -1034,6 +1034,16 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
         P.grid_dims[a] = dims[a];
     }
     for (int a = 0; a < 4; a++) P.sphere[a] = d->sphere[a];
+    // Contact's bucket bound: no pair reaches farther than twice the widest
+    // record radius; the margin covers the f32 rounding of bucket coordinates
+    // and of the reach test over a grid of this extent (a few hundred ulps).
+    {
+        double widest = 0, extent = 0;
+        for (uint32_t b = 0; b < d->n_bodies; b++) widest = std::max(widest, (double)d->bodies[b].contact_radius);
+        for (int a = 0; a < 3; a++) extent = std::max(extent, std::fabs(origin[a]) + edge * dims[a]);
+        P.reach_max = (float)(2.0 * widest);
+        P.skip_margin = (float)(1e-5 * (extent + 2.0 * widest));
+    }
     P.gain = 1.0f;  // GAIN, bodyContact.js:48
     P.bucket_size = (float)edge;
     P.bucket_count = (uint32_t)nbuckets;
@@ -1052,6 +1062,44 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
     // body's inv_mass arrives with them zeroed, a device-built one gets them
     // zeroed here, with its static nodes.
     const bool hard = d->n_driven && d->drive_law == 0;
+    // Host-built bodies that follow each other are staged together and go up
+    // as one copy per plane: a scene of many small bodies is a handful of
+    // copies, not a dozen per body.
+    struct Staging {
+        uint64_t node0 = 0, cell0 = 0, surf0 = 0;
+        std::vector<float> pos[3], vel[3], inv, radius;
+        std::vector<uint32_t> cells, snode, sbody;
+    } st;
+    auto flush = [&]() -> int {
+        const uint64_t nn = st.inv.size();
+        if (nn) {
+            for (int a = 0; a < 3; a++) {
+                const float *hx = keep(w, std::move(st.pos[a]));
+                for (int k : { 0, 3, 10 })
+                    HIPW(w, hipMemcpyAsync(w->d_nodes + (k + a) * N + st.node0, hx, nn * 4, hipMemcpyHostToDevice, up));
+                const float *hv = keep(w, std::move(st.vel[a]));
+                HIPW(w, hipMemcpyAsync(w->d_nodes + (6 + a) * N + st.node0, hv, nn * 4, hipMemcpyHostToDevice, up));
+            }
+            const float *hw = keep(w, std::move(st.inv));
+            HIPW(w, hipMemcpyAsync(w->d_nodes + 9 * N + st.node0, hw, nn * 4, hipMemcpyHostToDevice, up));
+        }
+        if (!st.cells.empty()) {
+            const size_t nc = st.cells.size();
+            const uint32_t *hc = keep(w, std::move(st.cells));
+            HIPW(w, hipMemcpyAsync(w->d_cells + st.cell0, hc, nc * 4, hipMemcpyHostToDevice, up));
+        }
+        if (!st.snode.empty()) {
+            const size_t ns = st.snode.size();
+            const uint32_t *hn = keep(w, std::move(st.snode));
+            const uint32_t *hb = keep(w, std::move(st.sbody));
+            const float *hr = keep(w, std::move(st.radius));
+            HIPW(w, hipMemcpyAsync(w->d_surf + st.surf0, hn, ns * 4, hipMemcpyHostToDevice, up));
+            HIPW(w, hipMemcpyAsync(w->d_surf + S + st.surf0, hb, ns * 4, hipMemcpyHostToDevice, up));
+            HIPW(w, hipMemcpyAsync(w->d_surf + 2 * S + st.surf0, hr, ns * 4, hipMemcpyHostToDevice, up));
+        }
+        st = Staging{};
+        return VS_OK;
+    };
     r.bodies.resize(d->n_bodies);
     for (uint32_t b = 0; b < d->n_bodies; b++) {
         const vs_body_desc &bd = d->bodies[b];
@@ -1084,6 +1132,7 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
         vb.damp = damp_factor(bd.damping, d->dt);
         const bool surface = d->contact && pl.n_surface;
         if (pl.device) {
+            if (int rc = flush()) return rc;
             vsim::LatticeView L = lattice_view(bd, pl, node_at, r.body_begin + b);
             L.cells = w->d_cells + cell_at;
             if (surface) {
@@ -1110,43 +1159,24 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
                 HIPW(w, vsim::launch_zero_mass(up, plane(w, 9), dz, (uint32_t)nz));
             }
         } else {
-            // Node planes: p = prev = rest, vel, w, rest.
-            std::vector<float> xyz(3 * nn);
-            for (uint64_t i = 0; i < nn; i++) {
-                xyz[i] = bd.rest[3 * i];
-                xyz[nn + i] = bd.rest[3 * i + 1];
-                xyz[2 * nn + i] = bd.rest[3 * i + 2];
-            }
-            const float *hx = keep(w, std::move(xyz));
-            for (int k : { 0, 3, 10 }) {
-                for (int a = 0; a < 3; a++)
-                    HIPW(w, hipMemcpyAsync(w->d_nodes + (k + a) * N + node_at, hx + a * nn, nn * 4, hipMemcpyHostToDevice, up));
+            // Node planes: p = prev = rest, vel, w, rest; the cell list; the
+            // surface records, body by body (bodyContact.js:72-85).
+            if (st.inv.empty()) {
+                st.node0 = node_at;
+                st.cell0 = cell_at;
+                st.surf0 = surf_at;
             }
             for (int a = 0; a < 3; a++) {
-                if (bd.velocity[a] == 0.0f) {
-                    HIPW(w, hipMemsetAsync(w->d_nodes + (6 + a) * N + node_at, 0, nn * 4, up));
-                } else {
-                    const float *hv = keep(w, std::vector<float>(nn, bd.velocity[a]));
-                    HIPW(w, hipMemcpyAsync(w->d_nodes + (6 + a) * N + node_at, hv, nn * 4, hipMemcpyHostToDevice, up));
-                }
+                std::vector<float> &x = st.pos[a];
+                for (uint64_t i = 0; i < nn; i++) x.push_back(bd.rest[3 * i + a]);
+                st.vel[a].insert(st.vel[a].end(), nn, bd.velocity[a]);
             }
-            const float *hw = keep(w, std::vector<float>(bd.inv_mass, bd.inv_mass + nn));
-            HIPW(w, hipMemcpyAsync(w->d_nodes + 9 * N + node_at, hw, nn * 4, hipMemcpyHostToDevice, up));
-            if (bd.n_cells) {
-                const uint32_t *hc = keep(w, std::vector<uint32_t>(bd.cells, bd.cells + bd.n_cells));
-                HIPW(w, hipMemcpyAsync(w->d_cells + cell_at, hc, (size_t)bd.n_cells * 4, hipMemcpyHostToDevice, up));
-            }
-            // Surface records, body by body (bodyContact.js:72-85).
+            st.inv.insert(st.inv.end(), bd.inv_mass, bd.inv_mass + nn);
+            st.cells.insert(st.cells.end(), bd.cells, bd.cells + bd.n_cells);
             if (surface) {
-                std::vector<uint32_t> sn(pl.n_surface), sb(pl.n_surface, (uint32_t)(r.body_begin + b));
-                std::vector<float> sr(pl.n_surface, bd.contact_radius);
-                for (uint32_t i = 0; i < pl.n_surface; i++) sn[i] = (uint32_t)(node_at + bd.surface[i]);
-                const uint32_t *hn = keep(w, std::move(sn));
-                const uint32_t *hb = keep(w, std::move(sb));
-                const float *hr = keep(w, std::move(sr));
-                HIPW(w, hipMemcpyAsync(w->d_surf + surf_at, hn, pl.n_surface * 4ull, hipMemcpyHostToDevice, up));
-                HIPW(w, hipMemcpyAsync(w->d_surf + S + surf_at, hb, pl.n_surface * 4ull, hipMemcpyHostToDevice, up));
-                HIPW(w, hipMemcpyAsync(w->d_surf + 2 * S + surf_at, hr, pl.n_surface * 4ull, hipMemcpyHostToDevice, up));
+                for (uint32_t i = 0; i < pl.n_surface; i++) st.snode.push_back((uint32_t)(node_at + bd.surface[i]));
+                st.sbody.insert(st.sbody.end(), pl.n_surface, (uint32_t)(r.body_begin + b));
+                st.radius.insert(st.radius.end(), pl.n_surface, bd.contact_radius);
             }
         }
         // Cells, colour by colour.
@@ -1203,6 +1233,7 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
         splat_at += bd.n_splats;
         node_at += nn;
     }
+    if (int rc = flush()) return rc;
     const vs_body *hb_rec = keep(w, std::vector<vs_body>(w->h_bodies.begin() + r.body_begin,
                                                          w->h_bodies.begin() + r.body_begin + d->n_bodies));
     HIPW(w, hipMemcpyAsync(w->d_bodies + r.body_begin, hb_rec, sizeof(vs_body) * d->n_bodies, hipMemcpyHostToDevice, up));

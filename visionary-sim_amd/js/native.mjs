@@ -36,14 +36,15 @@ export const BODY = Object.freeze({
     relaxation: 12, maxStretch: 13, friction: 14, damp: 15, origin: 16, nodeCount: 19
 });
 
-/** vs_scene_params: 144 bytes, word offsets and types. */
-export const SCENE_BYTES = 144;
+/** vs_scene_params: 160 bytes, word offsets and types. */
+export const SCENE_BYTES = 160;
 export const SCENE = Object.freeze({
     dt: [0, 'f32', 1], gravity: [1, 'f32', 3], tankMin: [4, 'f32', 3], gain: [7, 'f32', 1],
     tankMax: [8, 'f32', 3], bucketSize: [11, 'f32', 1], gridOrigin: [12, 'f32', 3], bucketCount: [15, 'u32', 1],
     gridDims: [16, 'u32', 3], gridBase: [19, 'u32', 1], grabTarget: [20, 'f32', 3], responseTime: [23, 'f32', 1],
     grabTargetVelocity: [24, 'f32', 3], driveLaw: [27, 'u32', 1], surfBase: [28, 'u32', 1],
-    surfaceCount: [29, 'u32', 1], bucketSlots: [30, 'u32', 1], flags: [31, 'u32', 1], sphere: [32, 'f32', 4]
+    surfaceCount: [29, 'u32', 1], bucketSlots: [30, 'u32', 1], flags: [31, 'u32', 1], sphere: [32, 'f32', 4],
+    reachMax: [36, 'f32', 1], skipMargin: [37, 'f32', 1]
 });
 
 /** vs_tile: begin, count, body, scene (16 bytes); a tile buffer starts with a 16-byte header. */

@@ -39,7 +39,7 @@ class VsSceneParams(C.Structure):
         ("target", C.c_float * 3), ("response_time", C.c_float),
         ("target_vel", C.c_float * 3), ("drive_law", C.c_uint32),
         ("surf_base", C.c_uint32), ("surf_count", C.c_uint32), ("bucket_slots", C.c_uint32), ("flags", C.c_uint32),
-        ("sphere", C.c_float * 4),
+        ("sphere", C.c_float * 4), ("reach_max", C.c_float), ("skip_margin", C.c_float), ("pad1", C.c_uint32 * 2),
     ]
 
 
