@@ -941,6 +941,36 @@ __device__ __forceinline__ bool overflow_members(GridView g, uint32_t *base, uin
 // sort.wgsl:14-53, run by the record that arrived first in each bucket; the
 // overfull buckets are taken by those records' wave, from their members when
 // bin listed them, else by the scan.
+// Sorts the first `held` (<= N) ids of `base` ascending: a bitonic network
+// over N registers, unrolled.
+template <int N>
+__device__ __forceinline__ void sort_slots(uint32_t *base, uint32_t held)
+{
+    uint32_t v[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = (uint32_t)k < held ? base[k] : ~0u;
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                const int o = k ^ stride;
+                if (o > k) {
+                    const bool up = (k & size) == 0;
+                    const uint32_t a = v[k], c = v[o];
+                    const bool swap = up ? a > c : a < c;
+                    v[k] = swap ? c : a;
+                    v[o] = swap ? a : c;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        if ((uint32_t)k < held) base[k] = v[k];
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restrict__ scenes, SurfaceView sv, GridView g,
                                                 TileView tv)
@@ -961,7 +991,18 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restric
                 const uint32_t held = g.count[gb + b];
                 uint32_t *base = g.slots + (size_t)(gb + b) * slots;
                 over = held > slots;
-                for (uint32_t a = 1; a < held && !over; a++) {
+                // The bucket's ids ascend (sort.wgsl:17-26's insertion sort:
+                // ids are distinct, so any sort gives its order): loaded
+                // together, sorted in registers by a network of 8, 16 or 32
+                // (ids past the bucket's count pad with ~0 and sort last),
+                // written back together.
+                if (!over && held > 1u && held <= 32u) {
+                    if (held <= 8u) sort_slots<8>(base, held);
+                    else if (held <= 16u) sort_slots<16>(base, held);
+                    else sort_slots<32>(base, held);
+                }
+                for (uint32_t a = 1; a < held && !over && held > 32u; a++) {
+                    // slot arrays wider than 32 (a layer-1 caller's choice)
                     const uint32_t value = base[a];
                     uint32_t at = a;
                     while (at > 0u) {
