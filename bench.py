@@ -373,9 +373,13 @@ def workload_scenes(name):
                                                  "sinusoid handle, 10 substeps per frame", "scenes": 1}
     voxels = 1.0 if name == "c3_1voxel" else 2.0
     scs = [scenes.c3_scene(bucket_voxels=voxels)]
+    piled = ", fallen into a pile (timed after 80 frames)" if name == "c3_pile" else ""
     return scs, {"workload": f"512 bodies of 8^3 cells (h 0.01 m) in a 1 m tank, node-node and node-wall contact, "
                              f"hash grid of {voxels:g}-voxel buckets, E log-uniform 1e4-1e6, {scs[0].substeps} "
-                             "substeps per frame", "scenes": 1}
+                             f"substeps per frame{piled}", "scenes": 1}
+
+
+PRE_ROLL = {"c3_pile": 80}  # untimed frames before a workload is timed
 
 
 def timed_frames(world, ids, scs, substeps, steps, warmup, splats=True):
@@ -421,6 +425,8 @@ def workload_leg(name, steps, warmup, plain_steps, autotune):
     splats = sum(s.n_splats for s in scs) > 0
     world = make_world(scs)
     ids = [world.add(sc) for sc in scs]
+    for f in range(PRE_ROLL.get(name, 0)):
+        world.frame(substeps, splats=False, graph=True)
     sync(world)
     pick = json.loads(world.autotune(substeps))["best"] if autotune else None
     wall, frame_ms = timed_frames(world, ids, scs, substeps, steps, warmup, splats)
@@ -443,7 +449,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="batched256", choices=["batched256", "c1", "c2", "c3", "c3_1voxel"])
+    ap.add_argument("--workload", default="batched256", choices=["batched256", "c1", "c2", "c3", "c3_1voxel", "c3_pile"])
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of the one-core CPU baseline, and again of the all-cores one")
@@ -509,6 +515,8 @@ def main():
     world = make_world(scs, separate_node_passes=int(args.separate_node_passes),
                        separate_colours=int(args.separate_colours))
     ids = [world.add(sc) for sc in scs]
+    for f in range(PRE_ROLL.get(args.workload, 0)):
+        world.frame(substeps, splats=False, graph=True)
     sync(world)
     log("[bench] scenes resident")
     tune = None
@@ -574,7 +582,7 @@ def main():
 
     legs = None
     if args.workload == "batched256" and not args.no_legs:
-        legs = {name: workload_leg(name, 20, 3, 3, not args.no_autotune) for name in ("c1", "c2", "c3")}
+        legs = {name: workload_leg(name, 20, 3, 3, not args.no_autotune) for name in ("c1", "c2", "c3", "c3_pile")}
 
     serving = None
     if args.serve_requests > 0:
