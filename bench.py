@@ -14,9 +14,11 @@ The autotuner (vs_world_autotune) picks the launch shapes and schedule first
 Prints ONE JSON line. `value` is the batched whole-job node-substeps/s;
 `plain`, `speedup`, frame latency p50/p99, the per-pass profile, the roofline
 of the dominant kernel (the colour-pair launches, timed inside a frame, with
-the back-to-back figure beside it), each pass's achieved GB/s against HBM
-peak, the single-scene workloads (16^3 block, 64^3 drag block with its
-sphere collider, 512 contact bodies at 2-voxel buckets), the serving loop at
+the back-to-back figure beside it; its bound is the VALU when the committed
+counter summary's VALUBusy for the launch exceeds its HBM fraction), each
+pass's achieved GB/s against HBM peak, the single-scene workloads (16^3
+block, 64^3 drag block with its sphere collider, 512 contact bodies at
+2-voxel buckets, falling and piled), the serving loop at
 50 requests/s and saturated, and the CPU baseline (the f32 oracle, this
 repository's C restatement of the reference's WGSL, on a bounded sample,
 all cores) ride along.
@@ -83,6 +85,15 @@ def elastic_valu_instructions(kernel):
     (rocprofv3 SQ_INSTS_VALU summary)."""
     v = _most_dispatched("batched256_valu.md", kernel, 4)
     return v * 1e6 if v is not None else None
+
+
+def elastic_valu_busy(kernel):
+    """(VALUBusy, VALUUtilization) of the plain colour-pair launch of
+    `kernel` as fractions, from the same summary: the share of cycles its
+    VALUs issue and the share of lanes active per VALU instruction."""
+    busy = _most_dispatched("batched256_valu.md", kernel, 7)
+    lanes = _most_dispatched("batched256_valu.md", kernel, 8)
+    return (busy / 100 if busy is not None else None), (lanes / 100 if lanes is not None else None)
 
 
 def pair_class_nodes(scs, cls):
@@ -560,6 +571,7 @@ def main():
     counted = args.workload == "batched256" and not args.separate_colours
     valu = elastic_valu_instructions(el_kernel) if counted else None
     valu_frac = valu / (kernel_us * 1e-6 * VALU_ISSUE_PER_S) if valu else None
+    valu_busy, valu_lanes = elastic_valu_busy(el_kernel) if counted else (None, None)
     hbm_frac = achieved / HBM_PEAK_GBS
     # ---- the splat pass alone, covariance output and with the engine format
     # (rotation + axis lengths by four Jacobi sweeps) as well
@@ -596,7 +608,7 @@ def main():
         log(f"[bench] ingest: {ingest_out}")
 
     roofline = {
-        "bound": "valu" if valu_frac and valu_frac > hbm_frac else "hbm",
+        "bound": "valu" if (valu_busy or valu_frac or 0) > hbm_frac else "hbm",
         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_frac,
         "traffic": elastic_traffic_bytes(el_kernel) if counted else None,
         "kernel": el_kernel + (" (colour pair, no node pass on board)" if pairs else " (colour pass)"),
@@ -607,6 +619,10 @@ def main():
                          "frac": burst_gbs / HBM_PEAK_GBS},
         "colour_model_bytes_per_launch": colour_model_bytes,
         "valu_issue_frac": valu_frac,
+        "valu_issue_note": "wave instructions per CU per cycle at 2.4 GHz, this run's launch time; gfx950 "
+                           "dual-issues VALU instructions, so 1.0 is not a ceiling",
+        "valu_busy_frac": valu_busy,
+        "valu_lanes_active_frac": valu_lanes,
         "valu_wave_instructions_per_launch": valu,
         "counters_source": profile_source("batched256_valu.md") if counted else None,
     }

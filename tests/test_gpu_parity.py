@@ -372,6 +372,17 @@ def _pair_case(case):
         # workgroup and runs per colour beside the first
         return [scenes.jelly_block((71, 3, 4), h=0.004, seed=6, name="w71"),
                 scenes.jelly_block((130, 2, 3), h=0.002, seed=7, name="w130")]
+    if case == "short_rows":
+        # rows of two and three lanes, 128 and 85 of them to a 256-lane
+        # workgroup
+        return [scenes.jelly_block((2, 40, 40), h=0.01, seed=14, name="s2"),
+                scenes.jelly_block((4, 30, 30), h=0.01, seed=15, name="s4")]
+    if case == "even_rows":
+        # even rows (a lane at each row's end holds its last node alone)
+        # in the folded last pair and the predicting first pair, one block
+        # pinned, one falling onto the tank floor
+        return [scenes.jelly_block((16, 5, 6), seed=16, name="e16"),
+                scenes.jelly_block((10, 7, 4), seed=17, pinned=False, tank_margin=0.01, name="e10")]
     if case == "contact":
         return [scenes.contact_stack(count=3, side=4)]
     if case == "stretch":
@@ -399,8 +410,8 @@ def _pair_case(case):
 
 
 @pytest.mark.parametrize("block", [64, 256])
-@pytest.mark.parametrize("case", ["c1", "odd_block", "thin", "wide_rows", "contact", "stretch", "hard_drive", "mixed",
-                                  "mixed_fold"])
+@pytest.mark.parametrize("case", ["c1", "odd_block", "thin", "wide_rows", "short_rows", "even_rows", "contact", "stretch",
+                                  "hard_drive", "mixed", "mixed_fold"])
 def test_colour_pairs_equal_separate_colours_and_oracle(case, block):
     # Colours 2c and 2c+1 of a dense lattice solved by rows in one launch:
     # the same frames, bit for bit, as one launch per colour, batched, from a

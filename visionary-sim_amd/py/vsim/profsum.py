@@ -3,7 +3,7 @@
     python -m vsim.profsum trace <dir>/..._kernel_trace.csv
     python -m vsim.profsum pmc <fetch_counter_collection.csv> <write_counter_collection.csv> \
         <copy_fetch.csv> <copy_write.csv>
-    python -m vsim.profsum valu <counter_collection.csv with SQ_INSTS_VALU>
+    python -m vsim.profsum valu <counter_collection.csv with SQ_INSTS_VALU [VALUBusy ...]>
 
 `trace` prints per (kernel, launch grid) call counts and average durations, so
 the batched launches (large grids) separate from the per-scene ones. `pmc`
@@ -45,12 +45,15 @@ def trace(path, top=20, batched_min=1 << 20):
     return "\n".join(out)
 
 
-def _per_kernel(path):
-    """Counter values per (kernel, grid). The splat pass runs in two output
-    modes with one kernel: dispatches that take over 1.5 times the fastest
-    one of their grid (the engine axes' Jacobi sweeps) get their own row."""
+def _dispatches(path, counter=None):
+    """(counter value, duration in ns) per dispatch, keyed by (kernel, grid).
+    The splat pass runs in two output modes with one kernel: dispatches that
+    take over 1.5 times the fastest one of their grid (the engine axes'
+    Jacobi sweeps) get their own row."""
     rows = collections.defaultdict(list)
     for r in csv.DictReader(open(path)):
+        if counter and r["Counter_Name"] != counter:
+            continue
         dur = int(r.get("End_Timestamp") or 0) - int(r.get("Start_Timestamp") or 0)
         rows[(short(r["Kernel_Name"]), int(r.get("Grid_Size") or 0))].append((float(r["Counter_Value"]), dur))
     d = collections.defaultdict(list)
@@ -58,8 +61,12 @@ def _per_kernel(path):
         fastest = min(t for _, t in v)
         for value, t in v:
             mode = " (cov+axes)" if k.startswith("k_splats") and fastest > 0 and t > 1.5 * fastest else ""
-            d[(k + mode, g)].append(value)
+            d[(k + mode, g)].append((value, t))
     return d
+
+
+def _per_kernel(path):
+    return {key: [value for value, _ in v] for key, v in _dispatches(path).items()}
 
 
 def pmc(fetch, write, cfetch, cwrite, copy_bytes=1 << 30):
@@ -85,22 +92,35 @@ VALU_ISSUE_PER_S = 256 * 2.4e9  # wave instructions per second, MI355X
 
 
 def valu(path, batched_min=1 << 19):
-    d = collections.defaultdict(list)
-    for r in csv.DictReader(open(path)):
-        if r["Counter_Name"] != "SQ_INSTS_VALU":
-            continue
-        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
-        d[(short(r["Kernel_Name"]), int(r.get("Grid_Size") or 0))].append((float(r["Counter_Value"]), dur))
-    out = ["| kernel | grid (threads) | dispatches | VALU wave instructions (M) | avg us | VALU issue / peak |",
-           "|---|---:|---:|---:|---:|---:|"]
+    """Per (kernel, grid): SQ_INSTS_VALU per dispatch against one wave
+    instruction per CU per cycle, and, where the run collected them, the
+    derived VALUBusy (share of cycles the VALUs issue) and VALUUtilization
+    (share of a wave's lanes active per VALU instruction), the quad-cycles in
+    which a SIMD issued two VALU instructions, and F32 transcendentals."""
+    counters = {}
+    for name in ("SQ_INSTS_VALU", "VALUBusy", "VALUUtilization", "SQ_ACTIVE_INST_VALU2", "SQ_INSTS_VALU_TRANS_F32"):
+        d = _dispatches(path, name)
+        if d:
+            counters[name] = d
+    d = counters["SQ_INSTS_VALU"]
+    extra = [c for c in ("VALUBusy", "VALUUtilization", "SQ_ACTIVE_INST_VALU2", "SQ_INSTS_VALU_TRANS_F32") if c in counters]
+    head = {"VALUBusy": "VALUBusy %", "VALUUtilization": "lanes active %",
+            "SQ_ACTIVE_INST_VALU2": "dual-issue quad-cycles (M)", "SQ_INSTS_VALU_TRANS_F32": "F32 transcendentals (M)"}
+    out = ["| kernel | grid (threads) | dispatches | VALU wave instructions (M) | avg us | VALU instructions / (CU x cycle) |"
+           + "".join(f" {head[c]} |" for c in extra),
+           "|---|---:|---:|---:|---:|---:|" + "---:|" * len(extra)]
+    mean = lambda v: sum(x for x, _ in v) / len(v)
     for key in sorted(d, key=lambda k: -sum(v for v, _ in d[k])):
         if key[1] < batched_min or "rocclr" in key[0]:
             continue
         v = d[key]
-        inst = sum(x for x, _ in v) / len(v)
-        dur = sum(y for _, y in v) / len(v)
-        out.append(f"| {key[0]} | {key[1]} | {len(v)} | {inst / 1e6:.1f} | {dur * 1e6:.1f} | "
-                   f"{inst / (dur * VALU_ISSUE_PER_S):.2f} |")
+        inst = mean(v)
+        dur = sum(y for _, y in v) / len(v) * 1e-9
+        row = f"| {key[0]} | {key[1]} | {len(v)} | {inst / 1e6:.1f} | {dur * 1e6:.1f} | {inst / (dur * VALU_ISSUE_PER_S):.2f} |"
+        for c in extra:
+            x = mean(counters[c][key]) if key in counters[c] else float("nan")
+            row += f" {x:.1f} |" if c in ("VALUBusy", "VALUUtilization") else f" {x / 1e6:.2f} |"
+        out.append(row)
     return "\n".join(out)
 
 
