@@ -331,7 +331,9 @@ const char *vs_world_error(const vs_world *w);
 /* Admits a scene: allocates its ranges in every pool and uploads its state.
  * VS_E_NOMEM when a pool (the HBM budget) cannot hold it. */
 int vs_world_add_scene(vs_world *w, const vs_scene_desc *desc, uint32_t *scene_id);
-/* Releases a scene's ranges. */
+/* Releases a scene's ranges. The host does not wait for the device: the
+ * frames enqueued before the removal still run over the scene, and the next
+ * admission writes into its ranges only after them (stream-ordered). */
 int vs_world_remove_scene(vs_world *w, uint32_t scene_id);
 /* Restores a scene to rest (particles.js:245-249). */
 int vs_world_reset_scene(vs_world *w, uint32_t scene_id);
@@ -443,7 +445,9 @@ int vs_server_submit(vs_server *s, const vs_scene_desc *desc, double arrival_s, 
                      const float *targets, uint32_t *request_id);
 /* Serves every request: admits arrivals while the world's pools hold them,
  * steps all admitted scenes one batched frame per iteration, retires finished
- * scenes. `realtime` = 0 ignores arrival times (all requests are ready). */
+ * scenes. Frames are pipelined (the next is enqueued before the host waits
+ * for the current one); a frame's completion is the GPU time of an event
+ * after it. `realtime` = 0 ignores arrival times (all requests are ready). */
 int vs_server_run(vs_server *s, int32_t realtime, vs_server_report *report);
 /* Every frame latency (ms) of the last run, in completion order. */
 int vs_server_latencies(const vs_server *s, float *ms, uint64_t cap, uint64_t *count);

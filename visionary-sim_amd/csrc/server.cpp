@@ -3,13 +3,18 @@
 // Requests (a scene, a frame count, optional per-frame grab targets and an
 // arrival time) wait in arrival order. Every iteration the server admits the
 // requests that have arrived while the world's pools -- the HBM budget -- can
-// hold them, enqueues ONE batched frame of every admitted scene, and, while
-// the device works, admits newcomers on the upload stream; then it waits,
-// books frame latencies and retires the scenes that are done, freeing their
-// ranges for the next admissions. A scene's frame latency runs from the
-// moment the frame became eligible (the scene admitted, or its previous frame
-// done) to the completion of the batch that ran it. The reference steps one
-// scene per browser tab and has no counterpart.
+// hold them and enqueues ONE batched frame of every admitted scene. Frames
+// are pipelined: frame k+1 is enqueued while frame k runs, and only then does
+// the host wait, on frame k's event, to book it, so the device always has the
+// next frame queued. A scene whose last frame has been enqueued is removed at
+// once (its final positions copied out first, stream-ordered, when results
+// are kept): the world's removal does not wait for the device, and the next
+// admission into its ranges waits on the device for the frames before it.
+// Newcomers are admitted on the upload stream while frames run. A scene's
+// frame latency runs from the moment the frame became eligible (the scene
+// admitted, or its previous frame done) to the completion of the batch that
+// ran it. The reference steps one scene per browser tab and has no
+// counterpart.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +28,7 @@
 #include <vector>
 
 #include "../../include/vsim.h"
+#include "world_internal.h"
 
 namespace {
 
@@ -84,11 +90,18 @@ struct SceneCopy {
 struct Request {
     std::unique_ptr<SceneCopy> scene;
     double arrival = 0;
-    uint32_t frames = 0, done = 0;
+    uint32_t frames = 0, done = 0, enqueued = 0;
     std::vector<float> targets;  // (frames + 1) x 3, or empty
     int64_t scene_id = -1;
     double admitted = -1, finished = -1, eligible = 0;
     std::vector<float> result;   // final positions, 3 planes
+    float *staging = nullptr;    // pinned: the final positions' copy in flight
+};
+
+// A frame in flight: the requests it steps, the event after it.
+struct Flight {
+    std::vector<size_t> batch;
+    uint32_t slot = 0;
 };
 
 }  // namespace
@@ -168,9 +181,61 @@ int vs_server_run(vs_server *s, int32_t realtime, vs_server_report *report)
         return VS_OK;
     };
 
+    // Frames complete at the GPU time of their events, measured from an
+    // epoch event recorded on the idle stream: the host may book a frame
+    // well after it ran (an admission in between), and its latency must not
+    // count that.
+    if (int rc = vs_world_synchronize(w)) return rc;
+    struct Epoch {
+        hipEvent_t e = nullptr;
+        ~Epoch()
+        {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } epoch;
+    if (hipEventCreate(&epoch.e) != hipSuccess || hipEventRecord(epoch.e, vsim::world_stream(w)) != hipSuccess) {
+        s->error = "server epoch event";
+        return VS_E_DEVICE;
+    }
+    const double epoch_t = now();
+    std::deque<Flight> flights;  // oldest first; at most two
+    // Waits for the oldest frame in flight and books it.
+    auto book = [&]() -> int {
+        const Flight f = std::move(flights.front());
+        flights.pop_front();
+        const hipEvent_t ev = vsim::world_event(w, f.slot);
+        float ms = 0.f;
+        if (!ev || hipEventSynchronize(ev) != hipSuccess || hipEventElapsedTime(&ms, epoch.e, ev) != hipSuccess) {
+            s->error = "waiting for a frame";
+            return VS_E_DEVICE;
+        }
+        const double done_t = epoch_t + 1e-3 * (double)ms;
+        iterations++;
+        max_active = std::max<uint32_t>(max_active, (uint32_t)f.batch.size());
+        for (size_t k : f.batch) {
+            Request &r = s->requests[k];
+            s->latencies_ms.push_back((float)((done_t - r.eligible) * 1e3));
+            r.eligible = done_t;
+            r.done++;
+            frames++;
+            node_substeps += r.scene->nodes * s->cfg.substeps;
+            if (r.done < r.frames) continue;
+            r.finished = done_t;
+            if (r.staging) r.result.assign(r.staging, r.staging + 3 * r.scene->nodes);
+        }
+        return VS_OK;
+    };
+    struct Staging {
+        std::vector<float *> p;
+        ~Staging()
+        {
+            for (float *q : p) (void)hipHostFree(q);
+        }
+    } staging;
+
     if (int rc = admit(now())) return rc;
-    while (next < s->requests.size() || !waiting.empty() || !active.empty()) {
-        if (active.empty()) {
+    while (next < s->requests.size() || !waiting.empty() || !active.empty() || !flights.empty()) {
+        if (active.empty() && flights.empty()) {
             if (waiting.empty() && realtime && next < s->requests.size()) {
                 const double wait = s->requests[next].arrival - now();
                 if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
@@ -184,52 +249,60 @@ int vs_server_run(vs_server *s, int32_t realtime, vs_server_report *report)
                 continue;
             }
         }
-        // Grab targets of this frame.
-        for (size_t k : active) {
-            Request &r = s->requests[k];
-            if (r.targets.empty()) continue;
-            const float *cur = &r.targets[3 * (size_t)(r.done + 1)], *prev = &r.targets[3 * (size_t)r.done];
-            const double fdt = (double)s->cfg.substeps * (double)r.scene->desc.dt;
-            float vel[3];
-            for (int a = 0; a < 3; a++) vel[a] = (float)(((double)cur[a] - (double)prev[a]) / fdt);
-            if (int rc = vs_world_set_target(w, (uint32_t)r.scene_id, cur, vel)) return rc;
-        }
-        const std::vector<size_t> batch = active;
-        if (int rc = vs_world_frame(w, s->cfg.substeps, s->cfg.splats, s->cfg.graph)) {
-            s->error = vs_world_error(w);
-            return rc;
-        }
-        // Newcomers are admitted while the device runs the frame; their first
-        // frame is the next batch.
-        if (int rc = admit(now())) return rc;
-        if (int rc = vs_world_synchronize(w)) return rc;
-        const double done_t = now();
-        iterations++;
-        max_active = std::max<uint32_t>(max_active, (uint32_t)batch.size());
-        std::vector<size_t> still;
-        for (size_t k : batch) {
-            Request &r = s->requests[k];
-            s->latencies_ms.push_back((float)((done_t - r.eligible) * 1e3));
-            r.eligible = done_t;
-            r.done++;
-            frames++;
-            node_substeps += r.scene->nodes * s->cfg.substeps;
-            if (r.done < r.frames) {
-                still.push_back(k);
-                continue;
+        if (!active.empty()) {
+            // Grab targets of the frame about to be enqueued.
+            for (size_t k : active) {
+                Request &r = s->requests[k];
+                if (r.targets.empty()) continue;
+                const float *cur = &r.targets[3 * (size_t)(r.enqueued + 1)], *prev = &r.targets[3 * (size_t)r.enqueued];
+                const double fdt = (double)s->cfg.substeps * (double)r.scene->desc.dt;
+                float vel[3];
+                for (int a = 0; a < 3; a++) vel[a] = (float)(((double)cur[a] - (double)prev[a]) / fdt);
+                if (int rc = vs_world_set_target(w, (uint32_t)r.scene_id, cur, vel)) return rc;
             }
-            r.finished = done_t;
-            if (s->cfg.keep_results) {
-                r.result.resize(3 * r.scene->nodes);
-                if (int rc = vs_world_read_nodes(w, (uint32_t)r.scene_id, r.result.data(), nullptr)) return rc;
+            if (int rc = vs_world_frame(w, s->cfg.substeps, s->cfg.splats, s->cfg.graph)) {
+                s->error = vs_world_error(w);
+                return rc;
             }
-            if (int rc = vs_world_remove_scene(w, (uint32_t)r.scene_id)) return rc;
-            r.scene_id = -1;
+            // Scenes whose last frame this is leave the world now; their
+            // ranges are reused only after this frame (and the copy of their
+            // final positions) has run.
+            std::vector<size_t> still;
+            for (size_t k : active) {
+                Request &r = s->requests[k];
+                if (++r.enqueued < r.frames) {
+                    still.push_back(k);
+                    continue;
+                }
+                if (s->cfg.keep_results) {
+                    float *buf = nullptr;
+                    if (hipHostMalloc(reinterpret_cast<void **>(&buf), sizeof(float) * 3 * r.scene->nodes) != hipSuccess) {
+                        s->error = "pinned staging of a result";
+                        return VS_E_NOMEM;
+                    }
+                    staging.p.push_back(buf);
+                    r.staging = buf;
+                    if (int rc = vsim::world_read_positions_async(w, (uint32_t)r.scene_id, buf)) return rc;
+                }
+                if (int rc = vs_world_remove_scene(w, (uint32_t)r.scene_id)) return rc;
+                r.scene_id = -1;
+            }
+            Flight f;
+            f.batch = active;
+            if (int rc = vs_world_record(w, &f.slot)) return rc;
+            flights.push_back(std::move(f));
+            active = still;
+            // Newcomers are admitted while the device runs; their first frame
+            // is the next batch.
+            if (int rc = admit(now())) return rc;
         }
-        for (size_t k : active)
-            if (std::find(batch.begin(), batch.end(), k) == batch.end()) still.push_back(k);
-        active = still;
-        if (int rc = admit(now())) return rc;
+        // Keep one frame queued behind the one running; with nothing to
+        // enqueue, drain.
+        while (flights.size() > 1 || (!flights.empty() && active.empty())) {
+            if (int rc = book()) return rc;
+            if (int rc = admit(now())) return rc;
+            if (!active.empty()) break;
+        }
     }
     const double wall = now();
     std::vector<float> sorted = s->latencies_ms;

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "world_internal.h"
 
 namespace {
 
@@ -133,6 +134,10 @@ struct vs_world {
     std::string error;
     hipStream_t stream = nullptr, upload = nullptr;
     hipEvent_t upload_done = nullptr, tables_done = nullptr, params_done = nullptr;
+    // Recorded on `stream` by each removal: the work enqueued before it may
+    // still read the removed scene's ranges, so admissions, which may hand
+    // those ranges out again, wait for it on the upload stream.
+    hipEvent_t released = nullptr;
     std::vector<hipEvent_t> events;
     hipEvent_t tune[2] = {};
     uint32_t next_event = 0;
@@ -832,6 +837,8 @@ static int world_init(vs_world *w)
     HIPW(W, hipEventCreateWithFlags(&W->upload_done, hipEventDisableTiming));
     HIPW(W, hipEventCreateWithFlags(&W->tables_done, hipEventDisableTiming));
     HIPW(W, hipEventCreateWithFlags(&W->params_done, hipEventDisableTiming));
+    HIPW(W, hipEventCreateWithFlags(&W->released, hipEventDisableTiming));
+    HIPW(W, hipEventRecord(W->released, W->stream));
     HIPW(W, hipEventRecord(W->upload_done, W->upload));
     HIPW(W, hipEventRecord(W->tables_done, W->stream));
     HIPW(W, hipEventRecord(W->params_done, W->stream));
@@ -852,7 +859,7 @@ void vs_world_destroy(vs_world *w)
     if (w->graph) (void)hipGraphDestroy(w->graph);
     for (auto &ev : w->events)
         if (ev) (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : { w->upload_done, w->tables_done, w->params_done, w->tune[0], w->tune[1] })
+    for (hipEvent_t ev : { w->upload_done, w->tables_done, w->params_done, w->released, w->tune[0], w->tune[1] })
         if (ev) (void)hipEventDestroy(ev);
     void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf, w->d_surf_pay,
                      w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_drive, w->d_splat_in, w->d_splat_out,
@@ -877,6 +884,9 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
     if (!(d->dt > 0.0f)) return fail(w, VS_E_INVALID, "stepDuration must be a finite positive number");
     HIPW(w, hipSetDevice(w->cfg.device));
     hipStream_t up = w->upload;
+    // Ranges of removed scenes are written only after the frames enqueued
+    // before their removal.
+    HIPW(w, hipStreamWaitEvent(up, w->released, 0));
     // Retire host copies whose uploads have landed.
     if (hipEventQuery(w->upload_done) == hipSuccess) w->pending.clear();
     // Device temporaries of lattice builds, released in stream order on the
@@ -1265,9 +1275,11 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
 int vs_world_remove_scene(vs_world *w, uint32_t id)
 {
     if (!w || id >= w->cfg.max_scenes || !w->recs[id].live) return VS_E_INVALID;
-    // Frames already enqueued may still read the scene's ranges, and the next
-    // admission may hand them out again on the upload stream: drain first.
-    HIPW(w, hipStreamSynchronize(w->stream));
+    // Frames already enqueued may still read the scene's ranges; the next
+    // admission may hand them out again, and its uploads wait for this event.
+    // The host does not wait: removal keeps the device busy.
+    HIPW(w, hipSetDevice(w->cfg.device));
+    HIPW(w, hipEventRecord(w->released, w->stream));
     release_ranges(w, w->recs[id]);
     w->recs[id] = SceneRec{};
     w->tables_dirty = true;
@@ -1672,3 +1684,23 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
 }
 
 }  // extern "C"
+
+namespace vsim {
+
+int world_read_positions_async(vs_world *w, uint32_t id, float *pos)
+{
+    if (!w || id >= w->cfg.max_scenes || !w->recs[id].live || !pos) return VS_E_INVALID;
+    HIPW(w, hipSetDevice(w->cfg.device));
+    const SceneRec &r = w->recs[id];
+    const size_t N = w->cfg.max_nodes;
+    for (int a = 0; a < 3; a++)
+        HIPW(w, hipMemcpyAsync(pos + a * r.node_count, w->d_nodes + a * N + r.node_begin, r.node_count * 4,
+                               hipMemcpyDeviceToHost, w->stream));
+    return VS_OK;
+}
+
+hipStream_t world_stream(vs_world *w) { return w ? w->stream : nullptr; }
+
+hipEvent_t world_event(vs_world *w, uint32_t slot) { return w && slot < EVENT_POOL ? w->events[slot] : nullptr; }
+
+}  // namespace vsim
