@@ -14,7 +14,9 @@ for bit, positions within 1e-5 of the largest coordinate.
 * several overfull contact scenes share the world's overflow list in one
   batched, graph-replayed launch and still equal each scene alone;
 * timing a pass leaves the simulation where it was, and contact passes are
-  refused by vs_world_time_pass.
+  refused by vs_world_time_pass;
+* a removal does not wait for the frames already queued, and the scene
+  admitted into the freed ranges is written only after them.
 """
 import numpy as np
 import pytest
@@ -182,3 +184,29 @@ def test_time_pass_leaves_the_simulation_and_refuses_contact():
         w.frame(sc.substeps, False, False)
         worlds.append(w.read_nodes(sid))
     assert np.array_equal(worlds[0][0], worlds[1][0]) and np.array_equal(worlds[0][1], worlds[1][1])
+
+
+def test_removal_leaves_queued_frames_and_admission_waits_for_them():
+    # Scene A's frames are queued and A is removed without waiting; B is
+    # admitted into the ranges A held (the world's only scene) while A's
+    # frames may still be running, then stepped. B must equal B stepped in a
+    # world of its own, and A's queued frames must not see B's uploads.
+    a = scenes.jelly_block(20, seed=31, name="a")
+    b = scenes.jelly_block(20, seed=32, pinned=False, name="b")
+    w = world_for(a)                 # room for one of the two: B takes A's ranges
+    sa = w.add(a)
+    for f in range(6):
+        w.frame(a.substeps, False, True)
+    w.remove(sa)
+    sb = w.add(b)
+    for f in range(3):
+        w.frame(b.substeps, False, True)
+    got = w.read_nodes(sb)
+    w.close()
+    alone = world_for(b)
+    s = alone.add(b)
+    for f in range(3):
+        alone.frame(b.substeps, False, True)
+    ref = alone.read_nodes(s)
+    alone.close()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])

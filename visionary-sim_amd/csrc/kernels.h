@@ -47,12 +47,21 @@ struct SurfaceView {
 // this substep (appends past `ovf_cap` are dropped: sort then finds fewer
 // members than the bucket counted and rescans); sort finds an overfull
 // bucket's members there instead of rescanning the scene's records.
+// Optional too: `ovb`, the overfull buckets as (grid bucket, scene), one entry
+// per bucket, appended by the record whose arrival makes it overfull; sort's
+// waves take them from it by wave index, so that a few waves do not each sort
+// several crowded buckets in turn. With `ovb`, `ovf_count` holds three words:
+// [0] records listed in `ovf`, [1] buckets listed in `ovb`, [2] set when
+// `ovb` ran out of room (sort then takes every overfull bucket in the wave of
+// its first record).
 struct GridView {
     uint32_t *count;
     uint32_t *slots;
     uint2 *ovf = nullptr;
     uint32_t *ovf_count = nullptr;
     uint32_t ovf_cap = 0;
+    uint2 *ovb = nullptr;
+    uint32_t ovb_cap = 0;
 };
 
 // Optional log of the pairs collide accepts (record, mate), `cap` per record.

@@ -826,6 +826,12 @@ __global__ __launch_bounds__(BLOCK) void k_bin(NodeView n, const vs_scene_params
                 const uint32_t at = atomicAdd(g.ovf_count, 1u);
                 if (at < g.ovf_cap) g.ovf[at] = make_uint2(s, gb + b);
             }
+            if (slot == slots && g.ovb) {
+                // This arrival made the bucket overfull: list the bucket once.
+                const uint32_t at = atomicAdd(&g.ovf_count[1], 1u);
+                if (at < g.ovb_cap) g.ovb[at] = make_uint2(gb + b, tile.scene);
+                else atomicOr(&g.ovf_count[2], 1u);
+            }
         }
     }
 }
@@ -884,28 +890,26 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v)
 
 // An overfull bucket (grid bucket `gbk`, `held` records) from its members:
 // the `slots` that arrived first (in `base`) and the records bin listed in
-// g.ovf. Each lane keeps the members it reads (at most K), then `slots`
-// rounds of wave minimum write the lowest ids ascending: the slots the
-// scene-wide scan would give. Returns false, having written nothing, when a
-// lane would hold more than K members or the members found are not `held`
-// (the caller then scans). Called by every lane of the wave with the same
-// bucket.
+// g.ovf, gathered in list order into the wave's LDS list `mem` (64 K
+// entries: a ballot gives each match its place), so that lane l then holds
+// members l, l + 64, ... (at most K). Then `slots` rounds of wave minimum
+// write the lowest ids ascending: the slots the scene-wide scan would give.
+// Returns false, having written nothing, when the members found are not
+// `held` or more than 64 K (the caller then scans). Called by every lane of
+// the wave with the same bucket.
+constexpr uint32_t MEMBERS_K = 8;
+
 __device__ __forceinline__ bool overflow_members(GridView g, uint32_t *base, uint32_t gbk, uint32_t held,
-                                                 uint32_t slots)
+                                                 uint32_t slots, uint32_t *mem_lds)
 {
-    constexpr uint32_t K = 8, U = 8;
+    constexpr uint32_t K = MEMBERS_K, U = 8;
     const uint32_t lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
     if (slots > 64u || held > 64u * K) return false;
-    uint32_t mem[K], n = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < K; k++) mem[k] = ~0u;
-    if (lane < slots) {
-        mem[0] = base[lane];
-        n = 1;
-    }
-    bool spill = false;
+    if (lane < slots) mem_lds[lane] = base[lane];
+    uint32_t n = slots;  // wave-uniform
     const uint32_t total = min(*g.ovf_count, g.ovf_cap);
-    for (uint32_t o0 = 0; o0 < total; o0 += 64u * U) {
+    for (uint32_t o0 = 0; o0 < total && n <= held; o0 += 64u * U) {
         uint2 e[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
@@ -914,18 +918,22 @@ __device__ __forceinline__ bool overflow_members(GridView g, uint32_t *base, uin
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            if (e[u].y != gbk) continue;
-#pragma unroll
-            for (uint32_t k = 0; k < K; k++)
-                if (k == n) mem[k] = e[u].x;
-            spill |= n >= K;
-            n++;
+            const bool m = e[u].y == gbk;
+            const uint64_t mask = __ballot(m);
+            const uint32_t at = n + (uint32_t)__popcll(mask & below);
+            if (m && at < 64u * K) mem_lds[at] = e[u].x;
+            n += (uint32_t)__popcll(mask);
         }
     }
-    uint32_t found = n;
+    if (n != held) return false;
+    // The list is written by other lanes of this wave: order its reads after
+    // those writes.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t mem[K];
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) found += (uint32_t)__shfl_xor((int)found, d);
-    if (__ballot(spill) != 0ull || found != held) return false;
+    for (uint32_t k = 0; k < K; k++) mem[k] = lane + 64u * k < n ? mem_lds[lane + 64u * k] : ~0u;
     uint32_t prev = 0u;
     for (uint32_t r = 0; r < slots; r++) {
         uint32_t best = ~0u;
@@ -935,12 +943,15 @@ __device__ __forceinline__ bool overflow_members(GridView g, uint32_t *base, uin
         prev = wave_min(best);
         if (lane == 0u) base[r] = prev;
     }
+    // The next bucket this wave takes rewrites the list.
+    __builtin_amdgcn_wave_barrier();
     return true;
 }
 
 // sort.wgsl:14-53, run by the record that arrived first in each bucket; the
-// overfull buckets are taken by those records' wave, from their members when
-// bin listed them, else by the scan.
+// overfull buckets are taken from bin's list of them by whichever wave asks
+// next (or, without the list, by their first record's wave), from their
+// members when bin listed them, else by the scan.
 // Sorts the first `held` (<= N) ids of `base` ascending: a bitonic network
 // over N registers, unrolled.
 template <int N>
@@ -975,6 +986,10 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restrict__ scenes, SurfaceView sv, GridView g,
                                                 TileView tv)
 {
+    __shared__ uint32_t s_members[BLOCK / 64][64 * MEMBERS_K];
+    uint32_t *const members = s_members[threadIdx.x >> 6];
+    // Overfull buckets go through bin's list unless it ran out of room.
+    const bool queued = g.ovb && g.ovf_count[2] == 0u;
     const Range r = tile_range(*tv.count, 0);
     for (uint32_t t = r.first; t < r.end; t += r.stride) {
         const vs_tile tile = tv.tiles[t];
@@ -1014,17 +1029,31 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restric
                     base[at] = value;
                 }
             }
-            uint64_t pending = __ballot(over);
+            uint64_t pending = queued ? 0ull : __ballot(over);
             if (pending && g.ovf) {
                 for (uint64_t left = pending; left; left &= left - 1ull) {
                     const int j = __builtin_ctzll(left);
                     const uint32_t bb = (uint32_t)__shfl((int)b, j);
-                    if (overflow_members(g, g.slots + (size_t)(gb + bb) * slots, gb + bb, g.count[gb + bb], slots))
+                    if (overflow_members(g, g.slots + (size_t)(gb + bb) * slots, gb + bb, g.count[gb + bb], slots,
+                                         members))
                         pending &= ~(1ull << j);
                 }
             }
             if (pending) overflow_scan(sc, sv.key, g.slots + (size_t)gb * slots, b, pending, slots);
         }
+    }
+    if (!queued) return;
+    // Wave w of the grid takes listed buckets w, w + W, ... (W waves in
+    // all): one each while there are fewer than waves. Every lane of the
+    // wave is here (the tile loop is uniform over the workgroup).
+    const uint32_t listed = min(g.ovf_count[1], g.ovb_cap);
+    const uint32_t W = gridDim.x * (BLOCK / 64u);
+    for (uint32_t j = blockIdx.x * (BLOCK / 64u) + (threadIdx.x >> 6); j < listed; j += W) {
+        const uint2 e = g.ovb[j];
+        const vs_scene_params &so = scenes[e.y];
+        const uint32_t sl = so.bucket_slots;
+        if (!overflow_members(g, g.slots + (size_t)e.x * sl, e.x, g.count[e.x], sl, members))
+            overflow_scan(so, sv.key, g.slots + (size_t)so.grid_base * sl, e.x - so.grid_base, 1ull, sl);
     }
 }
 
@@ -1313,7 +1342,10 @@ __global__ __launch_bounds__(BLOCK) void k_settle(NodeView n, const vs_scene_par
 {
     // The overflow list is empty again for the next substep's bin (which
     // runs after this launch).
-    if (g.ovf_count && blockIdx.x == 0 && threadIdx.x == 0) *g.ovf_count = 0u;
+    if (g.ovf_count && blockIdx.x == 0 && threadIdx.x == 0) {
+        g.ovf_count[0] = 0u;
+        if (g.ovb) g.ovf_count[1] = g.ovf_count[2] = 0u;
+    }
     const Range r = tile_range(*tv.count, 0);
     for (uint32_t t = r.first; t < r.end; t += r.stride) {
         const vs_tile tile = tv.tiles[t];

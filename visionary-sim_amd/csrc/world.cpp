@@ -151,7 +151,8 @@ struct vs_world {
     float4 *d_surf_pay = nullptr;  // per record: node position and radius at bin time
     uint32_t *d_grid = nullptr;
     uint2 *d_ovf = nullptr;            // records that found their bucket full: (record, grid bucket)
-    uint32_t *d_ovf_count = nullptr;   // how many this substep (settle empties it)
+    uint32_t *d_ovf_count = nullptr;   // 4 words: records listed, buckets listed, list full (settle empties them), spare
+    uint2 *d_ovb = nullptr;            // overfull buckets this substep: (grid bucket, scene)
     uint32_t *d_drive = nullptr;
     float *d_splat_in = nullptr, *d_splat_out = nullptr;
     int32_t *d_pairlog = nullptr;
@@ -239,10 +240,14 @@ SurfaceView surf_view(vs_world *w)
                         w->d_surf_pay };
 }
 
+// A bucket is overfull past at least one slot, so no more than half the
+// records can make buckets overfull.
+uint32_t ovb_cap(const vs_world *w) { return (uint32_t)(w->cfg.max_surface / 2u + 1u); }
+
 GridView grid_view(vs_world *w)
 {
     return GridView{ w->d_grid, w->d_grid + w->cfg.max_buckets, w->d_ovf, w->d_ovf_count,
-                     (uint32_t)w->cfg.max_surface };
+                     (uint32_t)w->cfg.max_surface, w->d_ovb, ovb_cap(w) };
 }
 
 vsim::DriveView drive_view(vs_world *w)
@@ -773,14 +778,15 @@ static int world_init(vs_world *w)
     ALLOC(W->d_surf_pay, sizeof(float4) * c.max_surface);
     ALLOC(W->d_grid, sizeof(uint32_t) * (1 + BUCKET_SLOTS) * c.max_buckets);
     ALLOC(W->d_ovf, sizeof(uint2) * c.max_surface);
-    ALLOC(W->d_ovf_count, sizeof(uint32_t));
+    ALLOC(W->d_ovf_count, 4 * sizeof(uint32_t));
+    ALLOC(W->d_ovb, sizeof(uint2) * ovb_cap(W));
     ALLOC(W->d_drive, sizeof(uint32_t) * DRIVE_PLANES * c.max_driven);
     ALLOC(W->d_splat_in, sizeof(float) * 10 * c.max_splats);
     ALLOC(W->d_splat_out, sizeof(float) * SPLAT_OUT_PLANES * c.max_splats);
     ALLOC(W->d_scene_counts, sizeof(uint32_t) * NPASS * c.max_scenes);
     HIPW(W, hipMemset(W->d_nodes, 0, sizeof(float) * NODE_PLANES * c.max_nodes));
     HIPW(W, hipMemset(W->d_grid, 0, sizeof(uint32_t) * c.max_buckets));
-    HIPW(W, hipMemset(W->d_ovf_count, 0, sizeof(uint32_t)));
+    HIPW(W, hipMemset(W->d_ovf_count, 0, 4 * sizeof(uint32_t)));
     HIPW(W, hipMemset(W->d_scenes, 0, sizeof(vs_scene_params) * c.max_scenes));
     W->nodes.init(c.max_nodes);
     W->cells.init(c.max_cells);
@@ -862,7 +868,7 @@ void vs_world_destroy(vs_world *w)
     for (hipEvent_t ev : { w->upload_done, w->tables_done, w->params_done, w->released, w->tune[0], w->tune[1] })
         if (ev) (void)hipEventDestroy(ev);
     void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf, w->d_surf_pay,
-                     w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_drive, w->d_splat_in, w->d_splat_out,
+                     w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_ovb, w->d_drive, w->d_splat_in, w->d_splat_out,
                      w->d_pairlog, w->d_scene_counts };
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
