@@ -370,7 +370,8 @@ int vs_world_read_lattice(vs_world *w, uint32_t scene_id, uint32_t body, float *
  * rotations (4 planes, xyzw) and axis lengths (3 planes). */
 int vs_world_read_splat_axes(vs_world *w, uint32_t scene_id, float *mean, float *rot, float *axis);
 /* Contact results of the last substep: bucket of each record and, when the
- * world was asked to log pairs (vs_world_log_pairs), up to `cap` mates each. */
+ * world was asked to log pairs (vs_world_log_pairs), up to `cap` mates each,
+ * as the scene's own record numbers (0 = its first record). */
 int vs_world_log_pairs(vs_world *w, uint32_t cap);
 int vs_world_read_contacts(vs_world *w, uint32_t scene_id, uint32_t *keys, int32_t *counts, uint32_t *pairs);
 /* Counters for the measurement harness. */

@@ -1495,10 +1495,21 @@ int vs_world_read_contacts(vs_world *w, uint32_t id, uint32_t *keys, int32_t *co
     const size_t S = w->cfg.max_surface;
     if (keys) HIPW(w, hipMemcpy(keys, w->d_surf + 3 * S + r.surf_begin, r.surf_count * 4, hipMemcpyDeviceToHost));
     if (w->d_pairlog) {
+        std::vector<int32_t> own;
+        if (pairs && !counts) {
+            own.resize(r.surf_count);
+            counts = own.data();
+        }
         if (counts) HIPW(w, hipMemcpy(counts, w->d_pairlog + r.surf_begin, r.surf_count * 4, hipMemcpyDeviceToHost));
-        if (pairs)
+        if (pairs) {
             HIPW(w, hipMemcpy(pairs, reinterpret_cast<uint32_t *>(w->d_pairlog + S) + r.surf_begin * w->pair_cap,
                               r.surf_count * w->pair_cap * 4, hipMemcpyDeviceToHost));
+            // The log holds world record ids: hand back the scene's own.
+            for (uint64_t s = 0; s < r.surf_count; s++) {
+                const uint32_t n = std::min<uint32_t>((uint32_t)std::max(counts[s], 0), w->pair_cap);
+                for (uint32_t k = 0; k < n; k++) pairs[s * w->pair_cap + k] -= (uint32_t)r.surf_begin;
+            }
+        }
     }
     return VS_OK;
 }
