@@ -477,6 +477,9 @@ def main():
     ap.add_argument("--separate-colours", action="store_true",
                     help="one launch per colour for every body (same results; the default solves the colours "
                          "of dense lattices in pairs, by rows of cells, one launch per pair)")
+    ap.add_argument("--colour-launches", action="store_true",
+                    help="small bodies of contact, stretch and drive scenes through the colour-pair launches too "
+                         "(same results; the default runs each such body's eight colours in one launch, in LDS)")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="scene requests of the serving legs (4k-500k nodes, 120 frames); 0 skips them")
     ap.add_argument("--serve-budget-nodes", type=float, default=6e6,
@@ -524,7 +527,7 @@ def main():
 
     merged = not args.separate_node_passes
     world = make_world(scs, separate_node_passes=int(args.separate_node_passes),
-                       separate_colours=int(args.separate_colours))
+                       separate_colours=int(args.separate_colours), colour_launches=int(args.colour_launches))
     ids = [world.add(sc) for sc in scs]
     for f in range(PRE_ROLL.get(args.workload, 0)):
         world.frame(substeps, splats=False, graph=True)

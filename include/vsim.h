@@ -271,6 +271,13 @@ typedef struct vs_world_config {
                                (the default) runs the colours of a dense body in
                                pairs (2c, 2c+1) over whole x-rows of cells, one
                                launch per pair (same arithmetic, same results) */
+    uint32_t colour_launches; /* 1: every body's colours go through the colour-pair
+                               and colour launches; 0 (the default) runs the
+                               eight colours of a small dense body (at most 11^3
+                               nodes) of a scene whose node passes do not fold
+                               into its colour passes (contact, tensile sweeps
+                               or drive) in one launch, the body in LDS (same
+                               arithmetic, same results)                        */
 } vs_world_config;
 
 /* One body of a scene, in the reference's host layout: the dense node grid

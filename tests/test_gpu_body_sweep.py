@@ -1,0 +1,65 @@
+"""The eight colours of a small dense body in one launch, the body in LDS
+(k_elastic_bodies), against the same bodies through the colour-pair launches
+(`colour_launches=1`) bit for bit, and against the f32 oracle: contact
+bodies of 8^3 cells (64-lane workgroups), a dragged 10^3 block (11^3 nodes,
+the largest such body, 125 cells a colour: 128-lane workgroups), an 11^3
+block beside it that is too large and keeps the colour pairs, tensile-limit
+sweeps, and a folded scene (no contact, no drive) whose bodies keep the
+colour pairs, all in one world; batched from a graph and per scene."""
+import numpy as np
+import pytest
+
+from vsim import scenes
+
+from test_gpu_parity import RTOL, rel_err, world_for
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix():
+    bodies = scenes.c3_scene(bodies=27, box=0.3, seed=5)
+    drag = scenes.jelly_block(10, seed=5, pinned=False, tank_margin=0.3, name="drag10")
+    drag.drive = scenes.handle_drive(drag, seed=6, law=0)
+    big = scenes.jelly_block(11, seed=7, pinned=False, tank_margin=0.3, name="drag11")
+    big.drive = scenes.handle_drive(big, seed=8, law=1)
+    st = scenes.jelly_block((6, 5, 7), seed=8, name="st")
+    st.bodies[0].material.max_stretch = 1.05
+    fold = scenes.jelly_block(9, seed=9, name="fold")
+    return [bodies, drag, big, st, fold]
+
+
+def _run(scs, colour_launches, plain, frames):
+    w = world_for(*scs, colour_launches=colour_launches)
+    sids = [w.add(s) for s in scs]
+    n = 8
+    for f in range(frames):
+        for s, sid in zip(scs, sids):
+            if s.drive is not None:
+                w.set_target(sid, *s.drive.frame_target(f, n * s.dt))
+        if plain:
+            for sid in sids:
+                w.frame_scene(sid, n, False)
+        else:
+            w.frame(n, False, True)
+    out = [w.read_nodes(sid) for sid in sids]
+    w.close()
+    return out
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_body_sweep_equals_colour_launches_and_oracle(plain):
+    from oracle.oracle import OracleScene
+    scs = _mix()
+    frames = 2
+    swept = _run(scs, 0, plain, frames)
+    pairs = _run(scs, 1, plain, frames)
+    for s, (p, v), (rp, rv) in zip(scs, swept, pairs):
+        assert np.array_equal(p, rp) and np.array_equal(v, rv), s.name
+    for s, (p, v) in zip(scs, swept):
+        o = OracleScene(s, "f32")
+        for f in range(frames):
+            if s.drive is not None:
+                o.set_target(*s.drive.frame_target(f, 8 * s.dt))
+            o.frame(8)
+        print(f"{s.name}: exact vs oracle {np.mean(p == o.p):.6f}")
+        assert rel_err(p, o.p) <= RTOL and rel_err(v, o.v) <= RTOL

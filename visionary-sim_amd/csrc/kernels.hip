@@ -782,6 +782,83 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
+// The colour passes of whole small dense bodies, one workgroup a body: its
+// nodes' positions, start-of-substep positions and inverse masses are
+// loaded into LDS once, colours 0..7 run from LDS in order (a barrier
+// between colours), and the positions go back once. Colour k's cells are
+// the body's cells of x, y, z parities (k & 1, (k >> 1) & 1, k >> 2)
+// (cellColor, lattice.js:44-46); cells of one colour share no corner, so
+// their order within the colour does not matter. Bodies share no node, so
+// a body's eight colours in its own launch give the colour-by-colour
+// launches' results (elasticCells.js:210-214 order), bit for bit.
+// `limit_only` runs the tensile-limit sweep alone (elasticCells.js:197-214).
+constexpr uint32_t BODY_SWEEP_NODES = 11 * 11 * 11;  // bodies of up to 10^3 cells
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_elastic_bodies(NodeView n, const vs_body *__restrict__ bodies,
+                                                          const vs_scene_params *__restrict__ scenes, TileView tv,
+                                                          uint32_t limit_only)
+{
+    __shared__ float sp[3][BODY_SWEEP_NODES], sq[3][BODY_SWEEP_NODES], sw[BODY_SWEEP_NODES];
+    const uint32_t tid = threadIdx.x;
+    const Range r = tile_range(*tv.count, 0u);
+    for (uint32_t t = r.first; t < r.end; t += r.stride) {
+        const vs_tile tile = tv.tiles[t];
+        const vs_body bd = bodies[tile.body];
+        const vs_scene_params &sc = scenes[bd.scene];
+        if (limit_only && !(bd.max_stretch > 1.0f)) continue;
+        const uint32_t nx = bd.node_dims[0], nxy = bd.node_dims[0] * bd.node_dims[1];
+        const uint32_t N = nxy * bd.node_dims[2];
+        if (N > BODY_SWEEP_NODES) continue;  // the host lists no such body
+        const uint32_t o = bd.node_offset;
+        for (uint32_t i = tid; i < N; i += BLOCK) {
+            sp[0][i] = n.px[o + i];
+            sp[1][i] = n.py[o + i];
+            sp[2][i] = n.pz[o + i];
+            sq[0][i] = n.qx[o + i];
+            sq[1][i] = n.qy[o + i];
+            sq[2][i] = n.qz[o + i];
+            sw[i] = n.w[o + i];
+        }
+        __syncthreads();
+        const CellConsts kc = cell_consts(bd, sc.dt);
+        for (uint32_t col = 0; col < 8u; col++) {
+            const uint32_t ox = col & 1u, oy = (col >> 1) & 1u, oz = col >> 2;
+            const uint32_t mx = (bd.cell_dims[0] + 1u - ox) / 2u, my = (bd.cell_dims[1] + 1u - oy) / 2u,
+                           mz = (bd.cell_dims[2] + 1u - oz) / 2u;
+            const uint32_t cells = mx * my * mz;
+            for (uint32_t i = tid; i < cells; i += BLOCK) {
+                const uint32_t cx = ox + 2u * (i % mx), cy = oy + 2u * ((i / mx) % my), cz = oz + 2u * (i / (mx * my));
+                const uint32_t n0 = cx + nx * cy + nxy * cz;
+                V3 p[8], q[8];
+                float w[8];
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    const uint32_t k = n0 + (uint32_t)(c & 1) + (c & 2 ? nx : 0u) + (c & 4 ? nxy : 0u);
+                    p[c] = mk(sp[0][k], sp[1][k], sp[2][k]);
+                    q[c] = mk(sq[0][k], sq[1][k], sq[2][k]);
+                    w[c] = sw[k];
+                }
+                solve_cell(p, q, w, kc, limit_only);
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    const uint32_t k = n0 + (uint32_t)(c & 1) + (c & 2 ? nx : 0u) + (c & 4 ? nxy : 0u);
+                    sp[0][k] = p[c].x;
+                    sp[1][k] = p[c].y;
+                    sp[2][k] = p[c].z;
+                }
+            }
+            __syncthreads();
+        }
+        for (uint32_t i = tid; i < N; i += BLOCK) {
+            n.px[o + i] = sp[0][i];
+            n.py[o + i] = sp[1][i];
+            n.pz[o + i] = sp[2][i];
+        }
+        __syncthreads();  // the next body reuses the LDS
+    }
+}
+
 // -------------------------------------------------------------- contact
 // gridCoord (contacts.wgsl:84-87)
 __device__ __forceinline__ void grid_coord(const vs_scene_params &sc, V3 p, uint32_t &x, uint32_t &y, uint32_t &z)
@@ -1610,6 +1687,12 @@ hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_bod
                                uint32_t node_mode)
 {
     VSIM_DISPATCH_BLOCK(k_elastic_rows, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
+}
+
+hipError_t launch_elastic_bodies(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
+                                 const vs_scene_params *scenes, TileView t, uint32_t limit_only)
+{
+    VSIM_DISPATCH_BLOCK(k_elastic_bodies, sh, s, n, bodies, scenes, t, limit_only);
 }
 
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,

@@ -35,7 +35,8 @@ using vsim::TileView;
 
 constexpr uint32_t BUCKET_SLOTS = 32;  // contacts.wgsl:75, bodyContact.js:35
 constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, P_ROW = 12, P_REST = 16, P_REST0 = 17,
-              NPASS = 18;
+              P_BODY = 18, NPASS = 19;
+constexpr uint32_t BODY_SWEEP_NODES = 11 * 11 * 11;  // vsim::k_elastic_bodies' LDS
 
 // The four colour-pair (row class) tables.
 constexpr bool is_row_pass(int p) { return p >= P_ROW && p < P_ROW + 4; }
@@ -190,6 +191,7 @@ struct vs_world {
     uint32_t block_node = 256, block_cells = 256, block_surf = 256, block_drive = 64, block_splat = 256;
     uint32_t block_rest = 256;   // the node pass of P_REST: 64 when its runs are short (folded scenes)
     uint32_t block_rest0 = 256;  // the frame's first predict over P_REST0, likewise
+    uint32_t block_body = 64;    // P_BODY: 128 when a body has more than 64 cells of a colour
     uint32_t tile_nodes = 256, tile_cells = 256, tile_surf = 256, tile_drive = 64, tile_splat = 256;
 };
 
@@ -273,7 +275,8 @@ uint32_t pass_block(const vs_world *w, int p)
     if (p == P_NODE) return w->block_node;
     if (p == P_REST) return w->block_rest;
     if (p == P_REST0) return w->block_rest0;
-    if ((p >= P_CELL && p < P_CELL + 8) || p >= P_ROW) return w->block_cells;
+    if ((p >= P_CELL && p < P_CELL + 8) || is_row_pass(p)) return w->block_cells;
+    if (p == P_BODY) return w->block_body;
     if (p == P_SURF) return w->block_surf;
     if (p == P_DRIVE) return w->block_drive;
     return w->block_splat;
@@ -288,6 +291,15 @@ uint32_t pass_tile(const vs_world *w, int p)
     return w->tile_splat;
 }
 
+// A small dense body of a scene whose node passes do not ride on its colour
+// passes (contact, tensile sweeps or drive in between) runs its eight colours
+// in one launch, in LDS (k_elastic_bodies).
+bool sweep_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
+{
+    return !w->cfg.separate_colours && !w->cfg.colour_launches && !(r.params.flags & VS_SCENE_FOLD) && b.dense &&
+           b.node_count <= BODY_SWEEP_NODES;
+}
+
 // A dense body whose rows fit one workgroup runs its colours in pairs, by
 // rows (k_elastic_rows); any other body runs one colour per pass.
 bool row_body(const vs_world *w, const BodyRec &b)
@@ -296,22 +308,23 @@ bool row_body(const vs_world *w, const BodyRec &b)
 }
 
 // The tile table of an elastic launch: colours 0..7, then the four row
-// classes (colours 2c and 2c+1) as 8..11.
-int elastic_table(int color) { return color < 8 ? P_CELL + color : P_ROW + (color - 8); }
+// classes (colours 2c and 2c+1) as 8..11, then whole small bodies as 12.
+int elastic_table(int color) { return color < 8 ? P_CELL + color : color < 12 ? P_ROW + (color - 8) : P_BODY; }
 
 // A row body of a VS_SCENE_FOLD scene has the nodes its last colour pair
 // writes finalized and predicted inside that pass (unless the node passes
 // are kept separate).
 bool fold_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
 {
-    return !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD) && row_body(w, b);
+    return !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD) && row_body(w, b) && !sweep_body(w, r, b);
 }
 
 // A row body of a VS_SCENE_FOLD_PREDICT scene has the nodes its first colour
 // pair writes predicted inside that pass at a frame's first substep.
 bool fold_predict_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
 {
-    return !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD_PREDICT) && row_body(w, b);
+    return !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD_PREDICT) && row_body(w, b) &&
+           !sweep_body(w, r, b);
 }
 
 // The nodes of a row body that class `cls` leaves to a node pass: class 3
@@ -363,15 +376,18 @@ int rebuild_tables(vs_world *w)
                 } else if (p < P_CELL + 8) {
                     const int c = p - P_CELL;
                     for (size_t b = 0; b < r.bodies.size(); b++)
-                        if (!row_body(w, r.bodies[b]))
+                        if (!row_body(w, r.bodies[b]) && !sweep_body(w, r, r.bodies[b]))
                             ok &= emit(r.bodies[b].color_begin[c], r.bodies[b].color_count[c],
                                        (uint32_t)(r.body_begin + b));
-                } else if (p >= P_ROW) {
+                } else if (p == P_BODY) {
+                    for (size_t b = 0; b < r.bodies.size() && ok; b++)
+                        if (sweep_body(w, r, r.bodies[b])) ok &= emit(0, 1, (uint32_t)(r.body_begin + b));
+                } else if (is_row_pass(p)) {
                     // Whole rows of the class, as many as fit one workgroup.
                     const uint32_t cls = (uint32_t)(p - P_ROW);
                     for (size_t b = 0; b < r.bodies.size() && ok; b++) {
                         const BodyRec &br = r.bodies[b];
-                        if (!row_body(w, br)) continue;
+                        if (!row_body(w, br) || sweep_body(w, r, br)) continue;
                         const uint64_t ny = (br.cell_dims[1] + 1u - (cls & 1u)) / 2u;
                         const uint64_t nz = (br.cell_dims[2] + 1u - (cls >> 1)) / 2u;
                         const uint64_t per = w->block_cells / (br.cell_dims[0] / 2u + 1u);
@@ -414,6 +430,16 @@ int rebuild_tables(vs_world *w)
             w->h_scene_counts.p[(size_t)p * S + s] = (uint32_t)(n - start);
         }
         w->tile_count[p] = (uint32_t)n;
+        if (p == P_BODY) {
+            // Workgroups of 64 lanes when no listed body has more cells of a
+            // colour than that (8^3 cells and less), else 128.
+            uint32_t most = 0;
+            for (uint64_t i = 0; i < n; i++) {
+                const uint32_t *cd = w->h_bodies[tiles[i].body].cell_dims;
+                most = std::max(most, ((cd[0] + 1u) / 2u) * ((cd[1] + 1u) / 2u) * ((cd[2] + 1u) / 2u));
+            }
+            w->block_body = most > 64u ? 128u : 64u;
+        }
         if (p == P_REST || p == P_REST0) {
             // Tables of small bodies (tiles of under 128 nodes on average)
             // waste less with 64-lane workgroups than with 256.
@@ -545,7 +571,9 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
     case VS_P_ELASTIC:
         sh = shape(elastic_table(color));
         if (!sh.grid) return VS_OK;
-        if (color >= 8)
+        if (color == 12)
+            HIPW(w, launch_elastic_bodies(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_BODY, scene), limit_only));
+        else if (color >= 8)
             HIPW(w, launch_elastic_rows(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, elastic_table(color), scene),
                                         (uint32_t)(color - 8), limit_only, node_mode));
         else
@@ -618,7 +646,8 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
         const bool timed = prof && grid_for(w, p, scene) != 0;
         if (timed) {
             prof->marks.emplace_back(kind, prof->used);
-            if (kind == VS_P_ELASTIC && color >= 8) prof->marks.emplace_back(node_mode ? VS_P_PAIR_NODES : VS_P_PAIR, prof->used);
+            if (kind == VS_P_ELASTIC && color >= 8 && color < 12)
+                prof->marks.emplace_back(node_mode ? VS_P_PAIR_NODES : VS_P_PAIR, prof->used);
             HIPW(w, hipEventRecord(prof->next(), st));
         }
         if (int rc = enqueue_pass(w, kind, color, scene, limit_only, node_mode)) return rc;
@@ -639,13 +668,16 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
         if (int rc = pass(VS_P_DRIVE, 0, 0)) return rc;
         // Colours in order; row class c (colours 2c, 2c+1 of the row bodies)
         // goes with the two colour passes of the other bodies.
-        for (int sweep = 0; sweep < (stretch ? 4 : 1); sweep++)
+        for (int sweep = 0; sweep < (stretch ? 4 : 1); sweep++) {
+            // Small bodies' eight colours in one launch (their own nodes only).
+            if (int rc = pass(VS_P_ELASTIC, 12, sweep > 0 ? 1u : 0u)) return rc;
             for (int c = 0; c < 4; c++)
                 for (int e : { 8 + c, 2 * c, 2 * c + 1 }) {
                     uint32_t fold = merge && sweep == 0 && e == 11 ? (last ? 2u : 1u) : 0u;
                     if (merge && sweep == 0 && s == 0 && e == 8) fold = 3u;
                     if (int rc = pass(VS_P_ELASTIC, e, sweep > 0 ? 1u : 0u, fold)) return rc;
                 }
+        }
         if (contact)
             for (int kind : { VS_P_BIN, VS_P_SORT, VS_P_COLLIDE, VS_P_SETTLE })
                 if (int rc = pass(kind, 0, 0)) return rc;
@@ -802,6 +834,7 @@ static int world_init(vs_world *w)
     // take their node passes), like P_NODE.
     W->tile_cap[P_REST] = c.max_nodes / W->tile_nodes + c.max_bodies + 1;
     W->tile_cap[P_REST0] = W->tile_cap[P_REST];
+    W->tile_cap[P_BODY] = c.max_bodies + 1;
     W->tile_cap[P_SURF] = c.max_surface / W->tile_surf + c.max_scenes + 1;
     W->tile_cap[P_DRIVE] = c.max_driven / W->tile_drive + c.max_scenes + 1;
     W->tile_cap[P_SPLAT] = c.max_splats / W->tile_splat + c.max_bodies + 1;
@@ -1329,7 +1362,8 @@ int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph
     std::vector<uint32_t> key = { substeps, mode, (uint32_t)w->any_stretch, (uint32_t)w->any_contact,
                                   w->cfg.separate_node_passes, w->pair_cap,
                                   w->block_cells,
-                                  w->cfg.xcd_chunked, w->cfg.separate_colours, w->block_rest, w->block_rest0 };
+                                  w->cfg.xcd_chunked, w->cfg.separate_colours, w->block_rest, w->block_rest0,
+                                  w->cfg.colour_launches, w->block_body };
     for (int p = 0; p < NPASS; p++) key.push_back(grid_for(w, p, -1));
     if (!w->exec || key != w->graph_key) {
         if (w->exec) HIPW(w, hipGraphExecDestroy(w->exec));
@@ -1576,7 +1610,7 @@ int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, ui
                   : kind == VS_P_PREDICT ? P_NODE
                   : (kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_REST
                                                                              : P_SURF;
-    const int kinds = kind == VS_P_ELASTIC ? 12 : 1;  // elastic: colours 0..7 and row classes 8..11
+    const int kinds = kind == VS_P_ELASTIC ? 13 : 1;  // elastic: colours 0..7, row classes 8..11, bodies 12
     auto table = [&](int c) { return kind == VS_P_ELASTIC ? elastic_table(c) : p; };
     uint32_t per_round = 0;
     for (int c = 0; c < kinds; c++) per_round += grid_for(w, table(c), -1) != 0;
