@@ -177,8 +177,10 @@ hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_bod
                                uint32_t node_mode);
 // The eight colours of whole small dense bodies (at most 11^3 nodes), one
 // workgroup a body, in LDS; a tile is one body ({0, 0, body, scene}).
+// `node_mode` (scenes flagged VS_SCENE_FOLD_PREDICT): 3 predicts the nodes
+// first, 1 finalizes them (the previous substep) and predicts them first.
 hipError_t launch_elastic_bodies(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
-                                 const vs_scene_params *scenes, TileView t, uint32_t limit_only);
+                                 const vs_scene_params *scenes, TileView t, uint32_t limit_only, uint32_t node_mode);
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes,
                               SurfaceView sv, GridView g, TileView t);
 hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv,

@@ -792,12 +792,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 // a body's eight colours in its own launch give the colour-by-colour
 // launches' results (elasticCells.js:210-214 order), bit for bit.
 // `limit_only` runs the tensile-limit sweep alone (elasticCells.js:197-214).
+// In a scene flagged VS_SCENE_FOLD_PREDICT (no drive between predict and the
+// first colour), `node_mode` 3 predicts the body's nodes before the colours
+// (a frame's first substep) and `node_mode` 1 runs the previous substep's
+// finalize and then this one's predict (k_finalize_predict's arithmetic):
+// nothing runs between them but this pass's load.
 constexpr uint32_t BODY_SWEEP_NODES = 11 * 11 * 11;  // bodies of up to 10^3 cells
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_elastic_bodies(NodeView n, const vs_body *__restrict__ bodies,
                                                           const vs_scene_params *__restrict__ scenes, TileView tv,
-                                                          uint32_t limit_only)
+                                                          uint32_t limit_only, uint32_t node_mode)
 {
     __shared__ float sp[3][BODY_SWEEP_NODES], sq[3][BODY_SWEEP_NODES], sw[BODY_SWEEP_NODES];
     const uint32_t tid = threadIdx.x;
@@ -811,14 +816,25 @@ __global__ __launch_bounds__(BLOCK) void k_elastic_bodies(NodeView n, const vs_b
         const uint32_t N = nxy * bd.node_dims[2];
         if (N > BODY_SWEEP_NODES) continue;  // the host lists no such body
         const uint32_t o = bd.node_offset;
+        const uint32_t mode = !limit_only && (sc.flags & VS_SCENE_FOLD_PREDICT) ? node_mode : 0u;
+        const Tank tk = tank_of(sc, bd);
+        const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
         for (uint32_t i = tid; i < N; i += BLOCK) {
-            sp[0][i] = n.px[o + i];
-            sp[1][i] = n.py[o + i];
-            sp[2][i] = n.pz[o + i];
-            sq[0][i] = n.qx[o + i];
-            sq[1][i] = n.qy[o + i];
-            sq[2][i] = n.qz[o + i];
-            sw[i] = n.w[o + i];
+            V3 p = ld3(n.px, n.py, n.pz, o + i), q = ld3(n.qx, n.qy, n.qz, o + i);
+            const float w = n.w[o + i];
+            if (mode != 0u && w != 0.0f) {
+                V3 v;
+                if (mode == 3u) v = ld3(n.vx, n.vy, n.vz, o + i);
+                else finalize_node(p, q, v, tk);
+                predict_node(p, q, v, gx, gy, gz, dt);
+            }
+            sp[0][i] = p.x;
+            sp[1][i] = p.y;
+            sp[2][i] = p.z;
+            sq[0][i] = q.x;
+            sq[1][i] = q.y;
+            sq[2][i] = q.z;
+            sw[i] = w;
         }
         __syncthreads();
         const CellConsts kc = cell_consts(bd, sc.dt);
@@ -854,6 +870,11 @@ __global__ __launch_bounds__(BLOCK) void k_elastic_bodies(NodeView n, const vs_b
             n.px[o + i] = sp[0][i];
             n.py[o + i] = sp[1][i];
             n.pz[o + i] = sp[2][i];
+            if (mode != 0u) {
+                n.qx[o + i] = sq[0][i];
+                n.qy[o + i] = sq[1][i];
+                n.qz[o + i] = sq[2][i];
+            }
         }
         __syncthreads();  // the next body reuses the LDS
     }
@@ -1690,9 +1711,9 @@ hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_bod
 }
 
 hipError_t launch_elastic_bodies(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
-                                 const vs_scene_params *scenes, TileView t, uint32_t limit_only)
+                                 const vs_scene_params *scenes, TileView t, uint32_t limit_only, uint32_t node_mode)
 {
-    VSIM_DISPATCH_BLOCK(k_elastic_bodies, sh, s, n, bodies, scenes, t, limit_only);
+    VSIM_DISPATCH_BLOCK(k_elastic_bodies, sh, s, n, bodies, scenes, t, limit_only, node_mode);
 }
 
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,

@@ -35,7 +35,7 @@ using vsim::TileView;
 
 constexpr uint32_t BUCKET_SLOTS = 32;  // contacts.wgsl:75, bodyContact.js:35
 constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, P_ROW = 12, P_REST = 16, P_REST0 = 17,
-              P_BODY = 18, NPASS = 19;
+              P_BODY = 18, P_LAST = 19, NPASS = 20;
 constexpr uint32_t BODY_SWEEP_NODES = 11 * 11 * 11;  // vsim::k_elastic_bodies' LDS
 
 // The four colour-pair (row class) tables.
@@ -192,6 +192,7 @@ struct vs_world {
     uint32_t block_rest = 256;   // the node pass of P_REST: 64 when its runs are short (folded scenes)
     uint32_t block_rest0 = 256;  // the frame's first predict over P_REST0, likewise
     uint32_t block_body = 64;    // P_BODY: 128 when a body has more than 64 cells of a colour
+    uint32_t block_last = 256;   // the frame's last finalize over P_LAST, as block_rest
     uint32_t tile_nodes = 256, tile_cells = 256, tile_surf = 256, tile_drive = 64, tile_splat = 256;
 };
 
@@ -274,6 +275,7 @@ uint32_t pass_block(const vs_world *w, int p)
 {
     if (p == P_NODE) return w->block_node;
     if (p == P_REST) return w->block_rest;
+    if (p == P_LAST) return w->block_last;
     if (p == P_REST0) return w->block_rest0;
     if ((p >= P_CELL && p < P_CELL + 8) || is_row_pass(p)) return w->block_cells;
     if (p == P_BODY) return w->block_body;
@@ -284,7 +286,7 @@ uint32_t pass_block(const vs_world *w, int p)
 
 uint32_t pass_tile(const vs_world *w, int p)
 {
-    if (p == P_NODE || p == P_REST || p == P_REST0) return w->tile_nodes;
+    if (p == P_NODE || p == P_REST || p == P_REST0 || p == P_LAST) return w->tile_nodes;
     if (p >= P_CELL && p < P_CELL + 8) return w->tile_cells;
     if (p == P_SURF) return w->tile_surf;
     if (p == P_DRIVE) return w->tile_drive;
@@ -321,6 +323,14 @@ bool fold_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
 
 // A row body of a VS_SCENE_FOLD_PREDICT scene has the nodes its first colour
 // pair writes predicted inside that pass at a frame's first substep.
+// A sweep body of a scene with nothing between its predict and its first
+// colour (no drive) has its node passes in its body sweep: the previous
+// substep's finalize and this one's predict, or a frame's first predict.
+bool sweep_fold_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
+{
+    return sweep_body(w, r, b) && !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD_PREDICT);
+}
+
 bool fold_predict_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
 {
     return !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD_PREDICT) && row_body(w, b) &&
@@ -363,14 +373,19 @@ int rebuild_tables(vs_world *w)
                 if (p == P_NODE) {
                     for (size_t b = 0; b < r.bodies.size(); b++)
                         ok &= emit(r.bodies[b].node_begin, r.bodies[b].node_count, (uint32_t)(r.body_begin + b));
-                } else if (p == P_REST || p == P_REST0) {
-                    // P_REST: the nodes of bodies whose last pair does not
-                    // finalize them; P_REST0: of those whose first pair does
-                    // not predict them (a folded body's leftover nodes go to
-                    // node tiles of that pair's launch).
+                } else if (p == P_REST || p == P_REST0 || p == P_LAST) {
+                    // P_LAST: the nodes of bodies whose last pair does not
+                    // finalize them; P_REST: those, but for bodies whose body
+                    // sweep finalizes and predicts them at the next substep's
+                    // start; P_REST0: the nodes of bodies whose first pair or
+                    // body sweep does not predict them (a folded body's
+                    // leftover nodes go to node tiles of that pair's launch).
                     for (size_t b = 0; b < r.bodies.size() && ok; b++) {
                         const BodyRec &br = r.bodies[b];
-                        if (p == P_REST ? fold_body(w, r, br) : fold_predict_body(w, r, br)) continue;
+                        const bool skip = p == P_LAST ? fold_body(w, r, br)
+                                          : p == P_REST ? fold_body(w, r, br) || sweep_fold_body(w, r, br)
+                                                        : fold_predict_body(w, r, br) || sweep_fold_body(w, r, br);
+                        if (skip) continue;
                         ok &= emit(br.node_begin, br.node_count, (uint32_t)(r.body_begin + b));
                     }
                 } else if (p < P_CELL + 8) {
@@ -440,12 +455,13 @@ int rebuild_tables(vs_world *w)
             }
             w->block_body = most > 64u ? 128u : 64u;
         }
-        if (p == P_REST || p == P_REST0) {
+        if (p == P_REST || p == P_REST0 || p == P_LAST) {
             // Tables of small bodies (tiles of under 128 nodes on average)
             // waste less with 64-lane workgroups than with 256.
             uint64_t items = 0;
             for (uint64_t i = 0; i < n; i++) items += tiles[i].count;
-            (p == P_REST ? w->block_rest : w->block_rest0) = n && items < 128 * n ? 64u : 256u;
+            (p == P_REST ? w->block_rest : p == P_REST0 ? w->block_rest0 : w->block_last) =
+                n && items < 128 * n ? 64u : 256u;
         }
         w->h_tiles[p].p[0] = (uint32_t)n;
         w->h_tiles[p].p[1] = w->h_tiles[p].p[2] = w->h_tiles[p].p[3] = 0;
@@ -572,7 +588,8 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
         sh = shape(elastic_table(color));
         if (!sh.grid) return VS_OK;
         if (color == 12)
-            HIPW(w, launch_elastic_bodies(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_BODY, scene), limit_only));
+            HIPW(w, launch_elastic_bodies(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_BODY, scene), limit_only,
+                                          node_mode));
         else if (color >= 8)
             HIPW(w, launch_elastic_rows(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, elastic_table(color), scene),
                                         (uint32_t)(color - 8), limit_only, node_mode));
@@ -582,8 +599,8 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
         return VS_OK;
     case VS_P_FINALIZE:
         // Every node, or, after folded colour pairs, the ones they leave.
-        sh = shape(P_REST);
-        if (sh.grid) HIPW(w, launch_finalize(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_REST, scene)));
+        sh = shape(P_LAST);
+        if (sh.grid) HIPW(w, launch_finalize(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_LAST, scene)));
         return VS_OK;
     case VS_P_FINALIZE_PREDICT:
         sh = shape(P_REST);
@@ -641,8 +658,9 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
                       : kind == VS_P_DRIVE ? P_DRIVE
                       : kind == VS_P_SPLAT ? P_SPLAT
                       : kind == VS_P_PREDICT ? (color ? P_REST0 : P_NODE)
-                      : (kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_REST
-                                                                                 : P_SURF;
+                      : kind == VS_P_FINALIZE ? P_LAST
+                      : kind == VS_P_FINALIZE_PREDICT ? P_REST
+                                                      : P_SURF;
         const bool timed = prof && grid_for(w, p, scene) != 0;
         if (timed) {
             prof->marks.emplace_back(kind, prof->used);
@@ -669,8 +687,12 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
         // Colours in order; row class c (colours 2c, 2c+1 of the row bodies)
         // goes with the two colour passes of the other bodies.
         for (int sweep = 0; sweep < (stretch ? 4 : 1); sweep++) {
-            // Small bodies' eight colours in one launch (their own nodes only).
-            if (int rc = pass(VS_P_ELASTIC, 12, sweep > 0 ? 1u : 0u)) return rc;
+            // Small bodies' eight colours in one launch (their own nodes
+            // only); at a sweep's start the bodies that fold their node
+            // passes there predict (a frame's first substep) or finalize
+            // the previous substep and predict.
+            if (int rc = pass(VS_P_ELASTIC, 12, sweep > 0 ? 1u : 0u, merge && sweep == 0 ? (s == 0 ? 3u : 1u) : 0u))
+                return rc;
             for (int c = 0; c < 4; c++)
                 for (int e : { 8 + c, 2 * c, 2 * c + 1 }) {
                     uint32_t fold = merge && sweep == 0 && e == 11 ? (last ? 2u : 1u) : 0u;
@@ -834,6 +856,7 @@ static int world_init(vs_world *w)
     // take their node passes), like P_NODE.
     W->tile_cap[P_REST] = c.max_nodes / W->tile_nodes + c.max_bodies + 1;
     W->tile_cap[P_REST0] = W->tile_cap[P_REST];
+    W->tile_cap[P_LAST] = W->tile_cap[P_REST];
     W->tile_cap[P_BODY] = c.max_bodies + 1;
     W->tile_cap[P_SURF] = c.max_surface / W->tile_surf + c.max_scenes + 1;
     W->tile_cap[P_DRIVE] = c.max_driven / W->tile_drive + c.max_scenes + 1;
@@ -1363,7 +1386,7 @@ int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph
                                   w->cfg.separate_node_passes, w->pair_cap,
                                   w->block_cells,
                                   w->cfg.xcd_chunked, w->cfg.separate_colours, w->block_rest, w->block_rest0,
-                                  w->cfg.colour_launches, w->block_body };
+                                  w->cfg.colour_launches, w->block_body, w->block_last };
     for (int p = 0; p < NPASS; p++) key.push_back(grid_for(w, p, -1));
     if (!w->exec || key != w->graph_key) {
         if (w->exec) HIPW(w, hipGraphExecDestroy(w->exec));
@@ -1608,8 +1631,9 @@ int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, ui
                   : kind == VS_P_DRIVE ? P_DRIVE
                   : kind == VS_P_SPLAT ? P_SPLAT
                   : kind == VS_P_PREDICT ? P_NODE
-                  : (kind == VS_P_FINALIZE || kind == VS_P_FINALIZE_PREDICT) ? P_REST
-                                                                             : P_SURF;
+                  : kind == VS_P_FINALIZE ? P_LAST
+                  : kind == VS_P_FINALIZE_PREDICT ? P_REST
+                                                  : P_SURF;
     const int kinds = kind == VS_P_ELASTIC ? 13 : 1;  // elastic: colours 0..7, row classes 8..11, bodies 12
     auto table = [&](int c) { return kind == VS_P_ELASTIC ? elastic_table(c) : p; };
     uint32_t per_round = 0;
