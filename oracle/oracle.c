@@ -544,7 +544,13 @@ static uint32_t bucket_at(const o_scene *sc, const uint32_t c[3])
 
 /* The four contact passes (bodyContact.js:137-148): bin.wgsl:9-23 into
  * the canonical bucket order sort.wgsl:14-53 establishes (ascending record
- * index, the lowest `bucket_slots` kept), collide.wgsl:16-125, settle.wgsl:20-36. */
+ * index, the lowest `bucket_slots` kept), collide.wgsl:16-125, settle.wgsl:20-36.
+ * A record's corrections are summed as collide.wgsl sums them, one after
+ * another in mate order (literal), or (the GPU's order) row by row: the
+ * corrections of each row of three neighbour buckets (dz, dy), in mate
+ * order, into a row sum, and the nine row sums in row order into the
+ * record's. The same mates and corrections; only the sum's association
+ * differs. Hit counts are integers either way. */
 void o_contact(o_scene *sc)
 {
     int S = sc->n_surf;
@@ -586,6 +592,7 @@ void o_contact(o_scene *sc)
             for (int dy = -1; dy <= 1; dy++) {
                 int y = cyi + dy;
                 if (y < 0 || y >= dims[1]) continue;
+                v3 row = mk(0, 0, 0);
                 for (int dx = -1; dx <= 1; dx++) {
                     int x = cxi + dx;
                     if (x < 0 || x >= dims[0]) continue;
@@ -622,10 +629,12 @@ void o_contact(o_scene *sc)
                             real lim = (mu * depth) / slid;
                             corr = sub(corr, scale(tangent, share * (lim < (real)1 ? lim : (real)1)));
                         }
-                        sum = add(sum, corr);
+                        if (sc->literal) sum = add(sum, corr);
+                        else row = add(row, corr);
                         hits += (real)1;
                     }
                 }
+                if (!sc->literal) sum = add(sum, row);
             }
         }
         if (sc->pair_count) sc->pair_count[s] = npairs;

@@ -65,7 +65,8 @@ typedef struct {
     uint32_t *pairs;             /* mate record ids, pair_cap per record */
     int32_t pair_cap;
     int32_t literal;             /* 1: elastic pass in project.wgsl's literal
-                                    formulas; 0: the fused order of the GPU  */
+                                    formulas and contact sums in collide.wgsl's
+                                    order; 0: the GPU's fused forms and order */
     int32_t pad1;
     double sphere[4];            /* sphere collider: centre, radius (<= 0: none) */
 } o_scene;

@@ -356,6 +356,10 @@ WIDE_CASES = {
     "hard_drive": (_drag_block, 3),
     "stretch": (_stretch_block, 3),
     "bodies64_contact": (lambda: scenes.c3_scene(bodies=64), 2),
+    # 64 bodies packed into a 0.4 m tank: hundreds of contacts a substep,
+    # where the sum of a record's corrections row by row departs most from
+    # collide.wgsl's one-after-another sum
+    "bodies64_packed": (lambda: scenes.c3_scene(bodies=64, box=0.4), 1),
 }
 
 

@@ -1268,6 +1268,10 @@ __global__ __launch_bounds__(BLOCK) void k_collide(NodeView n, const vs_body *__
                 for (int oy = -1; oy <= 1; oy++) {
                     const int y = cy + oy;
                     if (y < 0 || y >= dy) continue;
+                    // The row's corrections into a row sum, added to the
+                    // record's in row order.
+                    const V3 before = c.sum;
+                    c.sum = mk(0.f, 0.f, 0.f);
                     for (int ox = -1; ox <= 1; ox++) {
                         const int x = cx + ox;
                         if (x < 0 || x >= dx) continue;
@@ -1282,6 +1286,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide(NodeView n, const vs_body *__
                                          bodies, sv, log);
                         }
                     }
+                    c.sum = add(before, c.sum);
                 }
             }
             store_collider(c, sv, log);
@@ -1292,22 +1297,50 @@ __global__ __launch_bounds__(BLOCK) void k_collide(NodeView n, const vs_body *__
 // The same pass over bin's payload (sv.pay: a record's position and radius
 // beside it), nine lanes a record. A record's 27 neighbour buckets are nine
 // rows of three along x, (dz, dy) = row / 3 - 1, row % 3 - 1; lane `row` of
-// the record's group runs the reach tests of its row (loads four slots at a
-// time, ids then bodies and payloads) and lists, in LDS, the mates that pass.
-// The group's first lane then resolves the listed mates row by row, in
-// list order: the contacts of the one-lane pass, in its order, so the sums,
-// hit counts and logged pairs are the same. A row with more than ROW_KEEP
-// mates in reach is resolved by the first lane from the grid itself. A
-// wave holds seven records (lane 63 idles); a workgroup takes WAVES x 7
-// records of a tile at a time, so a tile is `parts` work items.
+// the record's group walks its row (loads four slots at a time, ids then
+// bodies and payloads), resolves each mate in reach as it meets it, and sums
+// the row's corrections; the group's first lane adds the nine row sums in
+// row order (the oracle's order: the row sums of collide.wgsl's mates). With
+// a pair log, a first walk counts each row's mates in reach, so that every
+// row logs its mates at its place in the record's mate order. A wave holds
+// seven records (lane 63 idles); a workgroup takes WAVES x 7 records of a
+// tile at a time, so a tile is `parts` work items.
 constexpr uint32_t ROW_LANES = 9;
 constexpr uint32_t ROW_GROUPS = 64 / ROW_LANES;
-constexpr uint32_t ROW_KEEP = 16;
 
 template <int BLOCK>
 constexpr uint32_t collide_parts(uint32_t tile_max)
 {
     return (tile_max + (BLOCK / 64) * ROW_GROUPS - 1) / ((BLOCK / 64) * ROW_GROUPS);
+}
+
+// Walks the row's buckets bk[0..2] (held[ox] slots each) in mate order,
+// four slots at a time, and calls f(other, mateBody, payload) for each.
+template <typename F>
+__device__ __forceinline__ void walk_row(const uint32_t (&bk)[3], const uint32_t (&held)[3], uint32_t slots,
+                                         uint32_t ownBody, GridView g, SurfaceView sv, F f)
+{
+#pragma unroll
+    for (int ox = 0; ox < 3; ox++) {
+        const uint32_t *base = g.slots + (size_t)bk[ox] * slots;
+        for (uint32_t k0 = 0; k0 < held[ox]; k0 += 4u) {
+            uint32_t o[4], mb[4];
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) o[u] = k0 + u < held[ox] ? base[k0 + u] : 0u;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                mb[u] = ownBody;  // past the bucket's end: the record's own body, never in reach
+                q[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (k0 + u < held[ox]) {
+                    mb[u] = sv.body[o[u]];
+                    q[u] = sv.pay[o[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) f(o[u], mb[u], q[u]);
+        }
+    }
 }
 
 template <int BLOCK>
@@ -1317,9 +1350,9 @@ __global__ __launch_bounds__(BLOCK) void k_collide_rows(NodeView n, const vs_bod
 {
     constexpr uint32_t WAVES = BLOCK / 64;
     constexpr uint32_t PER_PART = WAVES * ROW_GROUPS;
-    __shared__ uint32_t s_mates[WAVES][64][ROW_KEEP];
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t group = lane / ROW_LANES, row = lane - group * ROW_LANES;
+    const uint32_t lead = group * ROW_LANES;
     const uint32_t items = *tv.count * parts;
     for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
         const uint32_t t = it / parts, part = it - t * parts;
@@ -1332,9 +1365,9 @@ __global__ __launch_bounds__(BLOCK) void k_collide_rows(NodeView n, const vs_bod
         const uint32_t i = first + wv * ROW_GROUPS + group;
         const bool active = group < ROW_GROUPS && i < tile.count;
         Collider c;
-        int cx = 0, cy = 0, cz = 0;
-        uint32_t listed = 0;
+        uint32_t held[3] = { 0u, 0u, 0u }, bk[3] = { 0u, 0u, 0u };
         if (active) {
+            int cx, cy, cz;
             load_collider(c, tile.begin + i, n, bodies, sv, dx, dy, cx, cy, cz);
             // Neighbour buckets beyond reach: along an axis, the bucket below
             // when the record sits farther than the largest reach (plus the
@@ -1358,7 +1391,6 @@ __global__ __launch_bounds__(BLOCK) void k_collide_rows(NodeView n, const vs_bod
             const int z = cz + oz, y = cy + oy;
             const bool in_row = z >= 0 && z < dz && y >= 0 && y < dy && !(oz < 0 && lo[2]) && !(oz > 0 && hi[2]) &&
                                 !(oy < 0 && lo[1]) && !(oy > 0 && hi[1]);
-            uint32_t held[3], bk[3];
 #pragma unroll
             for (int ox = 0; ox < 3; ox++) {
                 const int x = cx + ox - 1;
@@ -1367,70 +1399,47 @@ __global__ __launch_bounds__(BLOCK) void k_collide_rows(NodeView n, const vs_bod
                 const uint32_t cnt = in ? g.count[bk[ox]] : 0u;
                 held[ox] = cnt < slots ? cnt : slots;
             }
-#pragma unroll
-            for (int ox = 0; ox < 3; ox++) {
-                const uint32_t *base = g.slots + (size_t)bk[ox] * slots;
-                for (uint32_t k0 = 0; k0 < held[ox]; k0 += 4u) {
-                    uint32_t o[4], mb[4];
-                    float4 q[4];
-#pragma unroll
-                    for (int u = 0; u < 4; u++) o[u] = k0 + u < held[ox] ? base[k0 + u] : 0u;
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        mb[u] = c.body;  // past the bucket's end: the record's own body, never in reach
-                        q[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (k0 + u < held[ox]) {
-                            mb[u] = sv.body[o[u]];
-                            q[u] = sv.pay[o[u]];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        if (reaches(c, mb[u], mk(q[u].x, q[u].y, q[u].z), q[u].w)) {
-                            if (listed < ROW_KEEP) s_mates[wv][lane][listed] = o[u];
-                            listed++;
-                        }
-                    }
-                }
-            }
         }
-        __syncthreads();
-        // Every lane reads the group's nine counts (shuffles take the whole wave).
-        uint32_t counts[ROW_LANES];
-        const uint32_t lead = group * ROW_LANES;
-#pragma unroll
-        for (uint32_t k = 0; k < ROW_LANES; k++) counts[k] = __shfl(listed, (int)min(lead + k, 63u));
-        if (active && row == 0) {
+        if (log.cap) {
+            // Each row's mates in reach, so that the rows log in mate order.
+            uint32_t mine = 0;
+            if (active)
+                walk_row(bk, held, slots, c.body, g, sv, [&](uint32_t, uint32_t mb, float4 q) {
+                    if (reaches(c, mb, mk(q.x, q.y, q.z), q.w)) mine++;
+                });
+            uint32_t before = 0, total = 0;
 #pragma unroll
             for (uint32_t k = 0; k < ROW_LANES; k++) {
-                if (counts[k] <= ROW_KEEP) {
-                    for (uint32_t j = 0; j < counts[k]; j++) {
-                        const uint32_t other = s_mates[wv][lead + k][j];
-                        const float4 q = sv.pay[other];
-                        collide_mate(c, other, sv.body[other], mk(q.x, q.y, q.z), q.w, n, bodies, sv, log);
-                    }
-                } else {
-                    // (The row listed mates, so z and y are inside the grid;
-                    // buckets passed by above add only failed reach tests.)
-                    const int z = cz + (int)(k / 3u) - 1, y = cy + (int)(k % 3u) - 1;
-                    for (int ox = -1; ox <= 1; ox++) {
-                        const int x = cx + ox;
-                        if (x < 0 || x >= dx) continue;
-                        const uint32_t bkk = gb + (uint32_t)(x + dx * (y + dy * z));
-                        const uint32_t cnt = g.count[bkk];
-                        const uint32_t h = cnt < slots ? cnt : slots;
-                        const uint32_t *base = g.slots + (size_t)bkk * slots;
-                        for (uint32_t k2 = 0; k2 < h; k2++) {
-                            const uint32_t other = base[k2];
-                            const float4 q = sv.pay[other];
-                            collide_mate(c, other, sv.body[other], mk(q.x, q.y, q.z), q.w, n, bodies, sv, log);
-                        }
-                    }
-                }
+                const uint32_t m = (uint32_t)__shfl((int)mine, (int)min(lead + k, 63u));
+                before += k < row ? m : 0u;
+                total += m;
             }
-            store_collider(c, sv, log);
+            if (active) {
+                c.logged = (int32_t)before;
+                if (row == 0u) log.count[c.s] = (int32_t)total;
+            }
         }
-        __syncthreads();  // the lists are rewritten by the next item
+        if (active)
+            walk_row(bk, held, slots, c.body, g, sv, [&](uint32_t other, uint32_t mb, float4 q) {
+                collide_mate(c, other, mb, mk(q.x, q.y, q.z), q.w, n, bodies, sv, log);
+            });
+        // The row sums in row order (shuffles take the whole wave).
+        V3 sum = mk(0.f, 0.f, 0.f);
+        float hits = 0.f;
+#pragma unroll
+        for (uint32_t k = 0; k < ROW_LANES; k++) {
+            const int from = (int)min(lead + k, 63u);
+            const V3 r = mk(__shfl(active ? c.sum.x : 0.f, from), __shfl(active ? c.sum.y : 0.f, from),
+                            __shfl(active ? c.sum.z : 0.f, from));
+            sum = add(sum, r);
+            hits += __shfl(active ? c.hits : 0.f, from);
+        }
+        if (active && row == 0u) {
+            sv.dx[c.s] = sum.x;
+            sv.dy[c.s] = sum.y;
+            sv.dz[c.s] = sum.z;
+            sv.hits[c.s] = hits;
+        }
     }
 }
 
