@@ -54,6 +54,11 @@ struct SurfaceView {
 // [0] records listed in `ovf`, [1] buckets listed in `ovb`, [2] set when
 // `ovb` ran out of room (sort then takes every overfull bucket in the wave of
 // its first record).
+// Optional too: `ext`, `ext_slots` more slots a bucket past its `slots`: the
+// record that arrives at position k (slots <= k < slots + ext_slots) of a
+// bucket writes itself to ext[bucket * ext_slots + k - slots], and only later
+// arrivals go to `ovf`; sort then finds most overfull buckets' members in
+// their slots and extension alone.
 struct GridView {
     uint32_t *count;
     uint32_t *slots;
@@ -62,6 +67,8 @@ struct GridView {
     uint32_t ovf_cap = 0;
     uint2 *ovb = nullptr;
     uint32_t ovb_cap = 0;
+    uint32_t *ext = nullptr;
+    uint32_t ext_slots = 0;
 };
 
 // Optional log of the pairs collide accepts (record, mate), `cap` per record.

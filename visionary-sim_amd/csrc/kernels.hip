@@ -920,6 +920,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin(NodeView n, const vs_scene_params
             const uint32_t slot = atomicAdd(&g.count[gb + b], 1u);
             sv.rank[s] = slot;
             if (slot < slots) g.slots[(size_t)(gb + b) * slots + slot] = s;
+            else if (slot - slots < g.ext_slots) g.ext[(size_t)(gb + b) * g.ext_slots + (slot - slots)] = s;
             else if (g.ovf) {
                 const uint32_t at = atomicAdd(g.ovf_count, 1u);
                 if (at < g.ovf_cap) g.ovf[at] = make_uint2(s, gb + b);
@@ -987,14 +988,16 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v)
 }
 
 // An overfull bucket (grid bucket `gbk`, `held` records) from its members:
-// the `slots` that arrived first (in `base`) and the records bin listed in
-// g.ovf, gathered in list order into the wave's LDS list `mem` (64 K
-// entries: a ballot gives each match its place), so that lane l then holds
-// members l, l + 64, ... (at most K). Then `slots` rounds of wave minimum
-// write the lowest ids ascending: the slots the scene-wide scan would give.
-// Returns false, having written nothing, when the members found are not
-// `held` or more than 64 K (the caller then scans). Called by every lane of
-// the wave with the same bucket.
+// the `slots` that arrived first (in `base`), the next ones in the bucket's
+// extension slots (g.ext), and the records bin listed in g.ovf, gathered in
+// that order into the wave's LDS list `mem` (64 K entries: a ballot gives
+// each list match its place), so that lane l then holds members l, l + 64,
+// ... (at most K). The list is scanned only when the bucket holds more than
+// its slots and extension. Then `slots` rounds of wave minimum write the
+// lowest ids ascending: the slots the scene-wide scan would give. Returns
+// false, having written nothing, when the members found are not `held` or
+// more than 64 K (the caller then scans). Called by every lane of the wave
+// with the same bucket.
 constexpr uint32_t MEMBERS_K = 8;
 
 __device__ __forceinline__ bool overflow_members(GridView g, uint32_t *base, uint32_t gbk, uint32_t held,
@@ -1006,7 +1009,10 @@ __device__ __forceinline__ bool overflow_members(GridView g, uint32_t *base, uin
     if (slots > 64u || held > 64u * K) return false;
     if (lane < slots) mem_lds[lane] = base[lane];
     uint32_t n = slots;  // wave-uniform
-    const uint32_t total = min(*g.ovf_count, g.ovf_cap);
+    const uint32_t in_ext = min(held - slots, g.ext_slots);
+    for (uint32_t k = lane; k < in_ext; k += 64u) mem_lds[n + k] = g.ext[(size_t)gbk * g.ext_slots + k];
+    n += in_ext;
+    const uint32_t total = n < held ? min(*g.ovf_count, g.ovf_cap) : 0u;
     for (uint32_t o0 = 0; o0 < total && n <= held; o0 += 64u * U) {
         uint2 e[U];
 #pragma unroll

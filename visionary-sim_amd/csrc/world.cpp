@@ -34,6 +34,9 @@ using vsim::SurfaceView;
 using vsim::TileView;
 
 constexpr uint32_t BUCKET_SLOTS = 32;  // contacts.wgsl:75, bodyContact.js:35
+// Extension slots a bucket: an overfull bucket's first 32 + EXT_SLOTS
+// arrivals are found without the overflow list (384 B a bucket).
+constexpr uint32_t EXT_SLOTS = 96;
 constexpr int P_NODE = 0, P_CELL = 1, P_SURF = 9, P_DRIVE = 10, P_SPLAT = 11, P_ROW = 12, P_REST = 16, P_REST0 = 17,
               P_BODY = 18, P_LAST = 19, NPASS = 20;
 constexpr uint32_t BODY_SWEEP_NODES = 11 * 11 * 11;  // vsim::k_elastic_bodies' LDS
@@ -154,6 +157,7 @@ struct vs_world {
     uint2 *d_ovf = nullptr;            // records that found their bucket full: (record, grid bucket)
     uint32_t *d_ovf_count = nullptr;   // 4 words: records listed, buckets listed, list full (settle empties them), spare
     uint2 *d_ovb = nullptr;            // overfull buckets this substep: (grid bucket, scene)
+    uint32_t *d_ext = nullptr;         // EXT_SLOTS more slots a bucket (arrivals 32 .. 32 + EXT_SLOTS - 1)
     uint32_t *d_drive = nullptr;
     float *d_splat_in = nullptr, *d_splat_out = nullptr;
     int32_t *d_pairlog = nullptr;
@@ -250,7 +254,7 @@ uint32_t ovb_cap(const vs_world *w) { return (uint32_t)(w->cfg.max_surface / 2u 
 GridView grid_view(vs_world *w)
 {
     return GridView{ w->d_grid, w->d_grid + w->cfg.max_buckets, w->d_ovf, w->d_ovf_count,
-                     (uint32_t)w->cfg.max_surface, w->d_ovb, ovb_cap(w) };
+                     (uint32_t)w->cfg.max_surface, w->d_ovb, ovb_cap(w), w->d_ext, EXT_SLOTS };
 }
 
 vsim::DriveView drive_view(vs_world *w)
@@ -834,6 +838,7 @@ static int world_init(vs_world *w)
     ALLOC(W->d_ovf, sizeof(uint2) * c.max_surface);
     ALLOC(W->d_ovf_count, 4 * sizeof(uint32_t));
     ALLOC(W->d_ovb, sizeof(uint2) * ovb_cap(W));
+    ALLOC(W->d_ext, sizeof(uint32_t) * EXT_SLOTS * c.max_buckets);
     ALLOC(W->d_drive, sizeof(uint32_t) * DRIVE_PLANES * c.max_driven);
     ALLOC(W->d_splat_in, sizeof(float) * 10 * c.max_splats);
     ALLOC(W->d_splat_out, sizeof(float) * SPLAT_OUT_PLANES * c.max_splats);
@@ -924,7 +929,7 @@ void vs_world_destroy(vs_world *w)
     for (hipEvent_t ev : { w->upload_done, w->tables_done, w->params_done, w->released, w->tune[0], w->tune[1] })
         if (ev) (void)hipEventDestroy(ev);
     void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf, w->d_surf_pay,
-                     w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_ovb, w->d_drive, w->d_splat_in, w->d_splat_out,
+                     w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_ovb, w->d_ext, w->d_drive, w->d_splat_in, w->d_splat_out,
                      w->d_pairlog, w->d_scene_counts };
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1589,7 +1594,7 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out)
     const vs_world_config &c = w->cfg;
     out->bytes_reserved = 4ull * NODE_PLANES * c.max_nodes + 4ull * c.max_cells + sizeof(vs_body) * c.max_bodies +
                           sizeof(vs_scene_params) * c.max_scenes + (4ull * SURF_PLANES + 24) * c.max_surface +
-                          4ull * (1 + BUCKET_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
+                          4ull * (1 + BUCKET_SLOTS + EXT_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
                           4ull * (10 + SPLAT_OUT_PLANES) * c.max_splats;
     out->graph_captures = w->captures;
     return VS_OK;
