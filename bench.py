@@ -438,6 +438,8 @@ def workload_leg(name, steps, warmup, plain_steps, autotune):
     ids = [world.add(sc) for sc in scs]
     for f in range(PRE_ROLL.get(name, 0)):
         world.frame(substeps, splats=False, graph=True)
+        if f % 8 == 7:
+            sync(world)  # a few frames in flight at a time
     sync(world)
     pick = json.loads(world.autotune(substeps))["best"] if autotune else None
     wall, frame_ms = timed_frames(world, ids, scs, substeps, steps, warmup, splats)
@@ -531,6 +533,8 @@ def main():
     ids = [world.add(sc) for sc in scs]
     for f in range(PRE_ROLL.get(args.workload, 0)):
         world.frame(substeps, splats=False, graph=True)
+        if f % 8 == 7:
+            sync(world)  # a few frames in flight at a time
     sync(world)
     log("[bench] scenes resident")
     tune = None
