@@ -280,7 +280,7 @@ def serve_leg(count, budget_nodes, realtime=True):
     reqs = scenes.c5_requests(count)
     cells = int(budget_nodes * 1.05)
     world = native.World(max_scenes=256, max_bodies=256, max_nodes=budget_nodes, max_cells=cells,
-                         max_driven=256 * 27, max_splats=1024, persistent=2048)
+                         max_driven=256 * 27, max_splats=1024, persistent=2048, branches=2)
     srv = native.Server(world, substeps=10, graph=True)
     for t, sc, frames in reqs:
         srv.submit(sc, t, frames)
@@ -482,6 +482,9 @@ def main():
     ap.add_argument("--colour-launches", action="store_true",
                     help="small bodies of contact, stretch and drive scenes through the colour-pair launches too "
                          "(same results; the default runs each such body's eight colours in one launch, in LDS)")
+    ap.add_argument("--branches", type=int, default=0,
+                    help="1 or 2: run batched frames as that many chains of launches side by side and skip the "
+                         "autotuner (same results; by default the autotuner picks)")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="scene requests of the serving legs (4k-500k nodes, 120 frames); 0 skips them")
     ap.add_argument("--serve-budget-nodes", type=float, default=6e6,
@@ -529,7 +532,8 @@ def main():
 
     merged = not args.separate_node_passes
     world = make_world(scs, separate_node_passes=int(args.separate_node_passes),
-                       separate_colours=int(args.separate_colours), colour_launches=int(args.colour_launches))
+                       separate_colours=int(args.separate_colours), colour_launches=int(args.colour_launches),
+                       branches=args.branches)
     ids = [world.add(sc) for sc in scs]
     for f in range(PRE_ROLL.get(args.workload, 0)):
         world.frame(substeps, splats=False, graph=True)
@@ -539,7 +543,7 @@ def main():
     log("[bench] scenes resident")
     tune = None
     # The autotuner may pick either schedule; an explicit schedule flag keeps it.
-    if not (args.no_autotune or args.separate_colours or args.separate_node_passes):
+    if not (args.no_autotune or args.separate_colours or args.separate_node_passes or args.branches):
         tune = json.loads(world.autotune(substeps))
         for c in tune["candidates"]:
             log(f"[bench] autotune {c}")

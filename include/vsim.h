@@ -278,6 +278,12 @@ typedef struct vs_world_config {
                                into its colour passes (contact, tensile sweeps
                                or drive) in one launch, the body in LDS (same
                                arithmetic, same results)                        */
+    uint32_t branches;        /* 2: a batched frame runs as two chains of launches
+                               side by side (scenes split by work, contact scenes
+                               all in the first), captured as two branches of the
+                               frame's graph, so that one chain's launches fill
+                               the other's tails; 0 or 1: one chain (same
+                               results)                                          */
 } vs_world_config;
 
 /* One body of a scene, in the reference's host layout: the dense node grid
@@ -420,7 +426,9 @@ int vs_world_profile(vs_world *w, uint32_t substeps, int32_t splats, float *ms, 
  * be left where they were. Synchronous. */
 int vs_world_time_pass(vs_world *w, int32_t kind, uint32_t rounds, float *ms, uint32_t *launches);
 /* Re-times the colour-pass launch shapes on the admitted scenes and keeps the
- * fastest (block size x cells per tile x persistent grid x XCD mapping). */
+ * fastest (block size x cells per tile x persistent grid x XCD mapping x one
+ * or two chains), timing graph-replayed frames with the splat output of the
+ * world's last frame; node state is put back afterwards. */
 int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t report_bytes);
 
 /* ------------------------------------------------------ the server */

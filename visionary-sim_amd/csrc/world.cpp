@@ -117,6 +117,7 @@ struct SceneRec {
     uint64_t drive_begin = 0, drive_count = 0;
     uint64_t splat_begin = 0, splat_count = 0;
     bool contact = false, stretch = false;
+    uint32_t branch = 0;  // chain of a two-chain frame (vs_world_config.branches)
     std::vector<BodyRec> bodies;
     std::vector<uint32_t> splat_perm;  // stored splat -> caller's splat index
 };
@@ -137,6 +138,11 @@ struct vs_world {
     vs_world_config cfg{};
     std::string error;
     hipStream_t stream = nullptr, upload = nullptr;
+    // Two-chain frames: the second chain's stream, the fork and join events,
+    // and, while a chain is enqueued, its stream and index (-1: one chain).
+    hipStream_t stream2 = nullptr, cur_stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    int cur_branch = -1;
     hipEvent_t upload_done = nullptr, tables_done = nullptr, params_done = nullptr;
     // Recorded on `stream` by each removal: the work enqueued before it may
     // still read the removed scene's ranges, so admissions, which may hand
@@ -168,6 +174,9 @@ struct vs_world {
     uint32_t *d_tiles[NPASS] = {};
     uint64_t tile_cap[NPASS] = {};
     uint32_t *d_scene_counts = nullptr;  // [NPASS][max_scenes]
+    uint32_t *d_branch_counts = nullptr; // [NPASS][2]
+    HostPinned<uint32_t> h_branch_counts;
+    uint32_t branch_first[NPASS][2] = {}, branch_ntiles[NPASS][2] = {};
     HostPinned<uint32_t> h_tiles[NPASS];
     // The colour-pair tables once more, in the order the batched launches
     // take them (rows of the most lanes first, node tiles last); the tables
@@ -356,97 +365,128 @@ int rebuild_tables(vs_world *w)
     const uint32_t S = w->cfg.max_scenes;
     w->any_stretch = false;
     w->any_contact = false;
+    // Two-chain frames: scenes split into two groups of about equal work
+    // (cells and nodes), largest first to the lighter group. Contact scenes
+    // all go to chain 0: they share the grid's overflow lists and counters.
+    {
+        uint64_t load[2] = { 0, 0 };
+        std::vector<uint32_t> order;
+        for (uint32_t s = 0; s < S; s++) {
+            SceneRec &r = w->recs[s];
+            r.branch = 0;
+            if (!r.live) continue;
+            if (w->cfg.branches >= 2 && r.contact) load[0] += r.cell_count + r.node_count;
+            else order.push_back(s);
+        }
+        if (w->cfg.branches >= 2) {
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+                return w->recs[a].cell_count + w->recs[a].node_count > w->recs[b].cell_count + w->recs[b].node_count;
+            });
+            for (uint32_t s : order) {
+                const uint32_t b = load[1] < load[0] ? 1u : 0u;
+                w->recs[s].branch = b;
+                load[b] += w->recs[s].cell_count + w->recs[s].node_count;
+            }
+        }
+    }
     for (int p = 0; p < NPASS; p++) {
         vs_tile *tiles = reinterpret_cast<vs_tile *>(w->h_tiles[p].p + 4);
         uint64_t n = 0;
         const uint32_t T = pass_tile(w, p);
-        for (uint32_t s = 0; s < S; s++) {
-            SceneRec &r = w->recs[s];
-            w->scene_first[p][s] = (uint32_t)n;
-            const uint64_t start = n;
-            if (r.live) {
-                auto emit = [&](uint64_t begin, uint64_t count, uint32_t body) {
-                    for (uint64_t o = 0; o < count; o += T) {
-                        if (n >= w->tile_cap[p]) return false;
-                        tiles[n++] = vs_tile{ (uint32_t)(begin + o), (uint32_t)std::min<uint64_t>(T, count - o), body,
-                                              s };
-                    }
-                    return true;
-                };
-                bool ok = true;
-                if (p == P_NODE) {
-                    for (size_t b = 0; b < r.bodies.size(); b++)
-                        ok &= emit(r.bodies[b].node_begin, r.bodies[b].node_count, (uint32_t)(r.body_begin + b));
-                } else if (p == P_REST || p == P_REST0 || p == P_LAST) {
-                    // P_LAST: the nodes of bodies whose last pair does not
-                    // finalize them; P_REST: those, but for bodies whose body
-                    // sweep finalizes and predicts them at the next substep's
-                    // start; P_REST0: the nodes of bodies whose first pair or
-                    // body sweep does not predict them (a folded body's
-                    // leftover nodes go to node tiles of that pair's launch).
-                    for (size_t b = 0; b < r.bodies.size() && ok; b++) {
-                        const BodyRec &br = r.bodies[b];
-                        const bool skip = p == P_LAST ? fold_body(w, r, br)
-                                          : p == P_REST ? fold_body(w, r, br) || sweep_fold_body(w, r, br)
-                                                        : fold_predict_body(w, r, br) || sweep_fold_body(w, r, br);
-                        if (skip) continue;
-                        ok &= emit(br.node_begin, br.node_count, (uint32_t)(r.body_begin + b));
-                    }
-                } else if (p < P_CELL + 8) {
-                    const int c = p - P_CELL;
-                    for (size_t b = 0; b < r.bodies.size(); b++)
-                        if (!row_body(w, r.bodies[b]) && !sweep_body(w, r, r.bodies[b]))
-                            ok &= emit(r.bodies[b].color_begin[c], r.bodies[b].color_count[c],
-                                       (uint32_t)(r.body_begin + b));
-                } else if (p == P_BODY) {
-                    for (size_t b = 0; b < r.bodies.size() && ok; b++)
-                        if (sweep_body(w, r, r.bodies[b])) ok &= emit(0, 1, (uint32_t)(r.body_begin + b));
-                } else if (is_row_pass(p)) {
-                    // Whole rows of the class, as many as fit one workgroup.
-                    const uint32_t cls = (uint32_t)(p - P_ROW);
-                    for (size_t b = 0; b < r.bodies.size() && ok; b++) {
-                        const BodyRec &br = r.bodies[b];
-                        if (!row_body(w, br) || sweep_body(w, r, br)) continue;
-                        const uint64_t ny = (br.cell_dims[1] + 1u - (cls & 1u)) / 2u;
-                        const uint64_t nz = (br.cell_dims[2] + 1u - (cls >> 1)) / 2u;
-                        const uint64_t per = w->block_cells / (br.cell_dims[0] / 2u + 1u);
-                        const uint64_t rows = ny * nz;
-                        const uint32_t id = (uint32_t)(r.body_begin + b);
-                        for (uint64_t o = 0; o < rows; o += per) {
-                            if (n >= w->tile_cap[p]) {
-                                ok = false;
-                                break;
-                            }
-                            tiles[n++] = vs_tile{ (uint32_t)o, (uint32_t)std::min<uint64_t>(per, rows - o), id, s };
+        // Chain 0's scenes first, then chain 1's, each in scene-id order.
+        for (uint32_t pass_br = 0; pass_br < 2; pass_br++) {
+            w->branch_first[p][pass_br] = (uint32_t)n;
+            for (uint32_t s = 0; s < S; s++) {
+                SceneRec &r = w->recs[s];
+                if (r.branch != pass_br) continue;
+                w->scene_first[p][s] = (uint32_t)n;
+                const uint64_t start = n;
+                if (r.live) {
+                    auto emit = [&](uint64_t begin, uint64_t count, uint32_t body) {
+                        for (uint64_t o = 0; o < count; o += T) {
+                            if (n >= w->tile_cap[p]) return false;
+                            tiles[n++] = vs_tile{ (uint32_t)(begin + o), (uint32_t)std::min<uint64_t>(T, count - o), body,
+                                                  s };
                         }
-                        // The nodes the first pair predicts (class 0) or the
-                        // last pair finalizes (class 3) that its rows leave
-                        // out, as node tiles of the same launch: full
-                        // workgroups of entries of the body's leftover list.
-                        const bool nodes = cls == 3 ? fold_body(w, r, br) : cls == 0 && fold_predict_body(w, r, br);
-                        const uint32_t left = ok && nodes ? leftover_count(br, cls) : 0u;
-                        for (uint32_t o = 0; o < left; o += w->block_cells) {
-                            if (n >= w->tile_cap[p]) {
-                                ok = false;
-                                break;
-                            }
-                            tiles[n++] = vs_tile{ o, std::min(w->block_cells, left - o) | VS_NODE_TILE, id, s };
+                        return true;
+                    };
+                    bool ok = true;
+                    if (p == P_NODE) {
+                        for (size_t b = 0; b < r.bodies.size(); b++)
+                            ok &= emit(r.bodies[b].node_begin, r.bodies[b].node_count, (uint32_t)(r.body_begin + b));
+                    } else if (p == P_REST || p == P_REST0 || p == P_LAST) {
+                        // P_LAST: the nodes of bodies whose last pair does not
+                        // finalize them; P_REST: those, but for bodies whose body
+                        // sweep finalizes and predicts them at the next substep's
+                        // start; P_REST0: the nodes of bodies whose first pair or
+                        // body sweep does not predict them (a folded body's
+                        // leftover nodes go to node tiles of that pair's launch).
+                        for (size_t b = 0; b < r.bodies.size() && ok; b++) {
+                            const BodyRec &br = r.bodies[b];
+                            const bool skip = p == P_LAST ? fold_body(w, r, br)
+                                              : p == P_REST ? fold_body(w, r, br) || sweep_fold_body(w, r, br)
+                                                            : fold_predict_body(w, r, br) || sweep_fold_body(w, r, br);
+                            if (skip) continue;
+                            ok &= emit(br.node_begin, br.node_count, (uint32_t)(r.body_begin + b));
                         }
+                    } else if (p < P_CELL + 8) {
+                        const int c = p - P_CELL;
+                        for (size_t b = 0; b < r.bodies.size(); b++)
+                            if (!row_body(w, r.bodies[b]) && !sweep_body(w, r, r.bodies[b]))
+                                ok &= emit(r.bodies[b].color_begin[c], r.bodies[b].color_count[c],
+                                           (uint32_t)(r.body_begin + b));
+                    } else if (p == P_BODY) {
+                        for (size_t b = 0; b < r.bodies.size() && ok; b++)
+                            if (sweep_body(w, r, r.bodies[b])) ok &= emit(0, 1, (uint32_t)(r.body_begin + b));
+                    } else if (is_row_pass(p)) {
+                        // Whole rows of the class, as many as fit one workgroup.
+                        const uint32_t cls = (uint32_t)(p - P_ROW);
+                        for (size_t b = 0; b < r.bodies.size() && ok; b++) {
+                            const BodyRec &br = r.bodies[b];
+                            if (!row_body(w, br) || sweep_body(w, r, br)) continue;
+                            const uint64_t ny = (br.cell_dims[1] + 1u - (cls & 1u)) / 2u;
+                            const uint64_t nz = (br.cell_dims[2] + 1u - (cls >> 1)) / 2u;
+                            const uint64_t per = w->block_cells / (br.cell_dims[0] / 2u + 1u);
+                            const uint64_t rows = ny * nz;
+                            const uint32_t id = (uint32_t)(r.body_begin + b);
+                            for (uint64_t o = 0; o < rows; o += per) {
+                                if (n >= w->tile_cap[p]) {
+                                    ok = false;
+                                    break;
+                                }
+                                tiles[n++] = vs_tile{ (uint32_t)o, (uint32_t)std::min<uint64_t>(per, rows - o), id, s };
+                            }
+                            // The nodes the first pair predicts (class 0) or the
+                            // last pair finalizes (class 3) that its rows leave
+                            // out, as node tiles of the same launch: full
+                            // workgroups of entries of the body's leftover list.
+                            const bool nodes = cls == 3 ? fold_body(w, r, br) : cls == 0 && fold_predict_body(w, r, br);
+                            const uint32_t left = ok && nodes ? leftover_count(br, cls) : 0u;
+                            for (uint32_t o = 0; o < left; o += w->block_cells) {
+                                if (n >= w->tile_cap[p]) {
+                                    ok = false;
+                                    break;
+                                }
+                                tiles[n++] = vs_tile{ o, std::min(w->block_cells, left - o) | VS_NODE_TILE, id, s };
+                            }
+                        }
+                    } else if (p == P_SURF) {
+                        if (r.contact) ok &= emit(r.surf_begin, r.surf_count, (uint32_t)r.body_begin);
+                    } else if (p == P_DRIVE) {
+                        ok &= emit(r.drive_begin, r.drive_count, (uint32_t)r.body_begin);
+                    } else {
+                        for (size_t b = 0; b < r.bodies.size(); b++)
+                            ok &= emit(r.bodies[b].splat_begin, r.bodies[b].splat_count, (uint32_t)(r.body_begin + b));
                     }
-                } else if (p == P_SURF) {
-                    if (r.contact) ok &= emit(r.surf_begin, r.surf_count, (uint32_t)r.body_begin);
-                } else if (p == P_DRIVE) {
-                    ok &= emit(r.drive_begin, r.drive_count, (uint32_t)r.body_begin);
-                } else {
-                    for (size_t b = 0; b < r.bodies.size(); b++)
-                        ok &= emit(r.bodies[b].splat_begin, r.bodies[b].splat_count, (uint32_t)(r.body_begin + b));
+                    if (!ok) return fail(w, VS_E_NOMEM, "tile table overflow");
+                    w->any_stretch |= r.stretch;
+                    w->any_contact |= r.contact;
                 }
-                if (!ok) return fail(w, VS_E_NOMEM, "tile table overflow");
-                w->any_stretch |= r.stretch;
-                w->any_contact |= r.contact;
+                w->scene_ntiles[p][s] = (uint32_t)(n - start);
+                w->h_scene_counts.p[(size_t)p * S + s] = (uint32_t)(n - start);
             }
-            w->scene_ntiles[p][s] = (uint32_t)(n - start);
-            w->h_scene_counts.p[(size_t)p * S + s] = (uint32_t)(n - start);
+            w->branch_ntiles[p][pass_br] = (uint32_t)n - w->branch_first[p][pass_br];
+            w->h_branch_counts.p[2 * p + pass_br] = w->branch_ntiles[p][pass_br];
         }
         w->tile_count[p] = (uint32_t)n;
         if (p == P_BODY) {
@@ -483,12 +523,19 @@ int rebuild_tables(vs_world *w)
                 if (t.count & VS_NODE_TILE) return 0;
                 return (uint64_t)t.count * (w->h_bodies[t.body].cell_dims[0] / 2u + 1u);
             };
-            std::stable_sort(sorted, sorted + n, [&](const vs_tile &a, const vs_tile &b) { return lanes(a) > lanes(b); });
+            // Within each chain's run of the table.
+            for (int br = 0; br < 2; br++) {
+                vs_tile *first = sorted + w->branch_first[p][br];
+                std::stable_sort(first, first + w->branch_ntiles[p][br],
+                                 [&](const vs_tile &a, const vs_tile &b) { return lanes(a) > lanes(b); });
+            }
             HIPW(w, hipMemcpyAsync(w->d_rows_by_work[p - P_ROW], h, 16 + n * sizeof(vs_tile), hipMemcpyHostToDevice,
                                    w->stream));
         }
     }
     HIPW(w, hipMemcpyAsync(w->d_scene_counts, w->h_scene_counts.p, sizeof(uint32_t) * NPASS * S,
+                           hipMemcpyHostToDevice, w->stream));
+    HIPW(w, hipMemcpyAsync(w->d_branch_counts, w->h_branch_counts.p, sizeof(uint32_t) * NPASS * 2,
                            hipMemcpyHostToDevice, w->stream));
     HIPW(w, hipEventRecord(w->tables_done, w->stream));
     w->tables_dirty = false;
@@ -526,18 +573,25 @@ TileView tile_view(vs_world *w, int p, int scene)
 {
     // XCD-chunked launches give each XCD one contiguous run of the table, so
     // they keep the scene order (a region stays on one XCD from pass to pass).
+    const int br = scene < 0 ? w->cur_branch : -1;
     if (scene < 0 && is_row_pass(p) && !w->cfg.xcd_chunked) {
         uint32_t *d = w->d_rows_by_work[p - P_ROW];
+        if (br >= 0)
+            return TileView{ w->d_branch_counts + 2 * p + br,
+                             reinterpret_cast<const vs_tile *>(d + 4) + w->branch_first[p][br] };
         return TileView{ d, reinterpret_cast<const vs_tile *>(d + 4) };
     }
     const vs_tile *tiles = reinterpret_cast<const vs_tile *>(w->d_tiles[p] + 4);
+    if (br >= 0) return TileView{ w->d_branch_counts + 2 * p + br, tiles + w->branch_first[p][br] };
     if (scene < 0) return TileView{ w->d_tiles[p], tiles };
     return TileView{ w->d_scene_counts + (size_t)p * w->cfg.max_scenes + scene, tiles + w->scene_first[p][scene] };
 }
 
 uint32_t grid_for(vs_world *w, int p, int scene)
 {
-    const uint32_t n = scene < 0 ? w->tile_count[p] : w->scene_ntiles[p][scene];
+    const uint32_t n = scene >= 0 ? w->scene_ntiles[p][scene]
+                       : w->cur_branch >= 0 ? w->branch_ntiles[p][w->cur_branch]
+                                            : w->tile_count[p];
     if (n == 0) return 0;
     // Persistent: a fixed grid walks the table, so a captured frame stays
     // valid whatever the admitted scenes (only a pass appearing or vanishing
@@ -574,7 +628,7 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
     using namespace vsim;
     auto shape = [&](int p) { return Shape{ grid_for(w, p, scene), pass_block(w, p), scene < 0 ? w->cfg.xcd_chunked : 0u }; };
     const NodeView n = node_view(w);
-    hipStream_t st = w->stream;
+    hipStream_t st = w->cur_stream ? w->cur_stream : w->stream;
     Shape sh{};
     switch (kind) {
     case VS_P_PREDICT: {
@@ -655,7 +709,7 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
         contact = w->recs[scene].contact;
     }
     const bool merge = !w->cfg.separate_node_passes;
-    hipStream_t st = w->stream;
+    hipStream_t st = w->cur_stream ? w->cur_stream : w->stream;
     // With a profile, events bracket every launch of a non-empty pass.
     auto pass = [&](int kind, int color, uint32_t limit_only, uint32_t node_mode = 0) -> int {
         const int p = kind == VS_P_ELASTIC ? elastic_table(color)
@@ -843,6 +897,7 @@ static int world_init(vs_world *w)
     ALLOC(W->d_splat_in, sizeof(float) * 10 * c.max_splats);
     ALLOC(W->d_splat_out, sizeof(float) * SPLAT_OUT_PLANES * c.max_splats);
     ALLOC(W->d_scene_counts, sizeof(uint32_t) * NPASS * c.max_scenes);
+    ALLOC(W->d_branch_counts, sizeof(uint32_t) * NPASS * 2);
     HIPW(W, hipMemset(W->d_nodes, 0, sizeof(float) * NODE_PLANES * c.max_nodes));
     HIPW(W, hipMemset(W->d_grid, 0, sizeof(uint32_t) * c.max_buckets));
     HIPW(W, hipMemset(W->d_ovf_count, 0, 4 * sizeof(uint32_t)));
@@ -890,6 +945,8 @@ static int world_init(vs_world *w)
     }
     HIPW(W, hipHostMalloc(reinterpret_cast<void **>(&W->h_scene_counts.p), sizeof(uint32_t) * NPASS * c.max_scenes,
                           hipHostMallocDefault));
+    HIPW(W, hipHostMalloc(reinterpret_cast<void **>(&W->h_branch_counts.p), sizeof(uint32_t) * NPASS * 2,
+                          hipHostMallocDefault));
     HIPW(W, hipHostMalloc(reinterpret_cast<void **>(&W->h_scenes.p), sizeof(vs_scene_params) * c.max_scenes,
                           hipHostMallocDefault));
     std::memset(W->h_scenes.p, 0, sizeof(vs_scene_params) * c.max_scenes);
@@ -901,6 +958,9 @@ static int world_init(vs_world *w)
     W->h_bodies.resize(c.max_bodies);
     HIPW(W, hipStreamCreateWithFlags(&W->stream, hipStreamNonBlocking));
     HIPW(W, hipStreamCreateWithFlags(&W->upload, hipStreamNonBlocking));
+    HIPW(W, hipStreamCreateWithFlags(&W->stream2, hipStreamNonBlocking));
+    HIPW(W, hipEventCreateWithFlags(&W->fork, hipEventDisableTiming));
+    HIPW(W, hipEventCreateWithFlags(&W->join, hipEventDisableTiming));
     HIPW(W, hipEventCreateWithFlags(&W->upload_done, hipEventDisableTiming));
     HIPW(W, hipEventCreateWithFlags(&W->tables_done, hipEventDisableTiming));
     HIPW(W, hipEventCreateWithFlags(&W->params_done, hipEventDisableTiming));
@@ -921,16 +981,18 @@ void vs_world_destroy(vs_world *w)
     if (!w) return;
     (void)hipSetDevice(w->cfg.device);
     if (w->stream) (void)hipStreamSynchronize(w->stream);
+    if (w->stream2) (void)hipStreamSynchronize(w->stream2);
     if (w->upload) (void)hipStreamSynchronize(w->upload);
     if (w->exec) (void)hipGraphExecDestroy(w->exec);
     if (w->graph) (void)hipGraphDestroy(w->graph);
     for (auto &ev : w->events)
         if (ev) (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : { w->upload_done, w->tables_done, w->params_done, w->released, w->tune[0], w->tune[1] })
+    for (hipEvent_t ev : { w->upload_done, w->tables_done, w->params_done, w->released, w->tune[0], w->tune[1], w->fork,
+                           w->join })
         if (ev) (void)hipEventDestroy(ev);
     void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf, w->d_surf_pay,
                      w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_ovb, w->d_ext, w->d_drive, w->d_splat_in, w->d_splat_out,
-                     w->d_pairlog, w->d_scene_counts };
+                     w->d_pairlog, w->d_scene_counts, w->d_branch_counts };
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int p = 0; p < NPASS; p++)
@@ -938,6 +1000,7 @@ void vs_world_destroy(vs_world *w)
     for (uint32_t *d : w->d_rows_by_work)
         if (d) (void)hipFree(d);
     if (w->stream) (void)hipStreamDestroy(w->stream);
+    if (w->stream2) (void)hipStreamDestroy(w->stream2);
     if (w->upload) (void)hipStreamDestroy(w->upload);
     delete w;
 }
@@ -1380,26 +1443,60 @@ int vs_world_set_target(vs_world *w, uint32_t id, const float target[3], const f
     return VS_OK;
 }
 
+// Enqueues one batched frame: on the world's stream, or, with two chains,
+// each chain's scenes on a stream of its own between a fork and a join (the
+// scenes of one chain share nothing with the other's: their launches may run
+// side by side and fill each other's tails).
+static int enqueue_batched_frame(vs_world *w, uint32_t substeps, uint32_t mode)
+{
+    bool two = w->cfg.branches >= 2;
+    if (two) {
+        bool used[2] = { false, false };
+        for (const SceneRec &r : w->recs)
+            if (r.live) used[r.branch] = true;
+        two = used[0] && used[1];
+    }
+    if (!two) return enqueue_frame(w, substeps, mode, -1);
+    HIPW(w, hipEventRecord(w->fork, w->stream));
+    HIPW(w, hipStreamWaitEvent(w->stream2, w->fork, 0));
+    int rc = VS_OK;
+    for (int b = 0; b < 2 && !rc; b++) {
+        w->cur_branch = b;
+        w->cur_stream = b ? w->stream2 : w->stream;
+        rc = enqueue_frame(w, substeps, mode, -1);
+    }
+    w->cur_branch = -1;
+    w->cur_stream = nullptr;
+    // Joined even after a failed enqueue, so no work is left outside the
+    // world's stream.
+    HIPW(w, hipEventRecord(w->join, w->stream2));
+    HIPW(w, hipStreamWaitEvent(w->stream, w->join, 0));
+    return rc;
+}
+
 int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph)
 {
     if (!w) return VS_E_INVALID;
     HIPW(w, hipSetDevice(w->cfg.device));
     if (int rc = prepare(w)) return rc;
     const uint32_t mode = splat_mode(splats);
-    if (!graph) return enqueue_frame(w, substeps, mode, -1);
+    if (!graph) return enqueue_batched_frame(w, substeps, mode);
     std::vector<uint32_t> key = { substeps, mode, (uint32_t)w->any_stretch, (uint32_t)w->any_contact,
                                   w->cfg.separate_node_passes, w->pair_cap,
                                   w->block_cells,
                                   w->cfg.xcd_chunked, w->cfg.separate_colours, w->block_rest, w->block_rest0,
-                                  w->cfg.colour_launches, w->block_body, w->block_last };
-    for (int p = 0; p < NPASS; p++) key.push_back(grid_for(w, p, -1));
+                                  w->cfg.colour_launches, w->block_body, w->block_last, w->cfg.branches };
+    for (int p = 0; p < NPASS; p++) {
+        key.push_back(grid_for(w, p, -1));
+        for (int b = 0; b < 2; b++) key.push_back(w->branch_ntiles[p][b] != 0);
+    }
     if (!w->exec || key != w->graph_key) {
         if (w->exec) HIPW(w, hipGraphExecDestroy(w->exec));
         if (w->graph) HIPW(w, hipGraphDestroy(w->graph));
         w->exec = nullptr;
         w->graph = nullptr;
         HIPW(w, hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_frame(w, substeps, mode, -1);
+        const int rc = enqueue_batched_frame(w, substeps, mode);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(w->stream, &g);
         if (rc) {
@@ -1686,28 +1783,32 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
     HIPW(w, hipMalloc(&snap, bytes));
     HIPW(w, hipMemcpyAsync(snap, w->d_nodes, bytes, hipMemcpyDeviceToDevice, w->stream));
     struct Cand {
-        uint32_t block, tile, persistent, xcd, colours;
+        uint32_t block, tile, persistent, xcd, colours, branches;
         float ms;
     };
     // Launch shapes for one colour per pass; colour pairs by rows take a
     // whole number of rows per workgroup, so only the block size, the grid
     // and the tile order vary for them.
     std::vector<Cand> cands;
+    // Colour pairs also as one chain or two side by side.
     for (uint32_t colours : { 0u, 1u })
         for (uint32_t block : { 64u, 128u, 256u, 512u })
             for (uint32_t mult : { 1u, 2u, 4u })
                 for (uint32_t persistent : { 0u, 2048u })
-                    for (uint32_t xcd : { 0u, 1u }) {
-                        if ((xcd && !persistent) || (!colours && mult > 1)) continue;
-                        cands.push_back(Cand{ block, block * mult, persistent, xcd, colours, 0.f });
-                    }
-    const Cand saved{ w->block_cells, w->tile_cells, w->cfg.persistent, w->cfg.xcd_chunked, w->cfg.separate_colours, 0.f };
+                    for (uint32_t xcd : { 0u, 1u })
+                        for (uint32_t branches : { 1u, 2u }) {
+                            if ((xcd && !persistent) || (!colours && mult > 1) || (colours && branches > 1)) continue;
+                            cands.push_back(Cand{ block, block * mult, persistent, xcd, colours, branches, 0.f });
+                        }
+    const Cand saved{ w->block_cells, w->tile_cells, w->cfg.persistent, w->cfg.xcd_chunked, w->cfg.separate_colours,
+                      w->cfg.branches, 0.f };
     auto apply = [&](const Cand &c) {
         w->block_cells = c.block;
         w->tile_cells = c.tile;
         w->cfg.persistent = c.persistent;
         w->cfg.xcd_chunked = c.xcd;
         w->cfg.separate_colours = c.colours;
+        w->cfg.branches = c.branches;
         w->tables_dirty = true;
     };
     // Frames as the batched path runs them (captured once per shape, then
@@ -1718,13 +1819,17 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
         apply(c);
         if (int rc = prepare(w)) return rc;
         c.ms = 0.f;
-        if (int rc = vs_world_frame(w, substeps, 0, 1)) return rc;  // warm-up
+        // Frames with the splat output the world last produced (how a
+        // frame runs: in two chains each chain's splat pass overlaps the
+        // other's last substep).
+        const int32_t splats = (int32_t)w->splat_mode;
+        if (int rc = vs_world_frame(w, substeps, splats, 1)) return rc;  // warm-up
         for (int trial = 0; trial < 2; trial++) {
             HIPW(w, hipMemcpyAsync(w->d_nodes, snap, bytes, hipMemcpyDeviceToDevice, w->stream));
             hipEvent_t a = w->tune[0], b = w->tune[1];
             HIPW(w, hipEventRecord(a, w->stream));
             for (int rep = 0; rep < 3; rep++)
-                if (int rc = vs_world_frame(w, substeps, 0, 1)) return rc;
+                if (int rc = vs_world_frame(w, substeps, splats, 1)) return rc;
             HIPW(w, hipEventRecord(b, w->stream));
             HIPW(w, hipEventSynchronize(b));
             float ms = 0.f;
@@ -1751,8 +1856,9 @@ int vs_world_autotune(vs_world *w, uint32_t substeps, char *report, uint32_t rep
         auto one = [](const Cand &c) {
             char buf[224];
             std::snprintf(buf, sizeof buf,
-                          "{\"block\":%u,\"tile\":%u,\"persistent\":%u,\"xcd\":%u,\"separate_colours\":%u,\"ms\":%.4f}",
-                          c.block, c.tile, c.persistent, c.xcd, c.colours, c.ms);
+                          "{\"block\":%u,\"tile\":%u,\"persistent\":%u,\"xcd\":%u,\"separate_colours\":%u,"
+                          "\"branches\":%u,\"ms\":%.4f}",
+                          c.block, c.tile, c.persistent, c.xcd, c.colours, c.branches, c.ms);
             return std::string(buf);
         };
         std::string out = "{\"candidates\":[";

@@ -58,7 +58,7 @@ class VsWorldConfig(C.Structure):
         ("max_buckets", C.c_uint64), ("max_driven", C.c_uint64), ("max_splats", C.c_uint64),
         ("tile_nodes", C.c_uint32), ("tile_cells", C.c_uint32), ("block_cells", C.c_uint32),
         ("persistent", C.c_uint32), ("xcd_chunked", C.c_uint32), ("separate_node_passes", C.c_uint32),
-        ("separate_colours", C.c_uint32), ("colour_launches", C.c_uint32),
+        ("separate_colours", C.c_uint32), ("colour_launches", C.c_uint32), ("branches", C.c_uint32),
     ]
 
 
@@ -318,14 +318,14 @@ class World:
     def __init__(self, *, max_scenes=16, max_bodies=16, max_nodes=1 << 20, max_cells=1 << 20,
                  max_surface=1 << 16, max_buckets=1 << 16, max_driven=4096, max_splats=1 << 20,
                  device=0, block_cells=256, tile_cells=0, persistent=0, xcd_chunked=0, separate_node_passes=0,
-                 separate_colours=0, colour_launches=0):
+                 separate_colours=0, colour_launches=0, branches=0):
         self.lib = load()
         cfg = VsWorldConfig(device=device, max_scenes=max_scenes, max_bodies=max_bodies, max_nodes=max_nodes,
                             max_cells=max_cells, max_surface=max_surface, max_buckets=max_buckets,
                             max_driven=max_driven, max_splats=max_splats, block_cells=block_cells,
                             tile_cells=tile_cells, persistent=persistent, xcd_chunked=xcd_chunked,
                             separate_node_passes=separate_node_passes, separate_colours=separate_colours,
-                            colour_launches=colour_launches)
+                            colour_launches=colour_launches, branches=branches)
         h = C.c_void_p()
         rc = self.lib.vs_world_create(C.byref(cfg), C.byref(h))
         if rc != VS_OK:
