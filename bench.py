@@ -618,6 +618,16 @@ def main():
         ingest_out = ingest_leg(cpu=not args.no_cpu)
         log(f"[bench] ingest: {ingest_out}")
 
+    # ---- the whole frame as the timed loop runs it (one or two chains of
+    # launches side by side): the colour launches' and the splat pass's
+    # algorithmic bytes over the frame time; node passes and contact are left
+    # out, so this is a lower bound of the frame's rate.
+    frame_bytes = substeps * per_sweep * (burst_bytes if pairs else colour_model_bytes) + splats * SPLAT_BYTES
+    frame_s = wall / args.steps
+    frame_roof = {"bytes": frame_bytes, "ms": frame_s * 1e3, "achieved": frame_bytes / frame_s / 1e9,
+                  "frac": frame_bytes / frame_s / 1e9 / HBM_PEAK_GBS,
+                  "branches": tune["best"].get("branches") if tune else (args.branches or None),
+                  "note": "colour launches and splat pass only, over the timed frame"}
     roofline = {
         "bound": "valu" if (valu_busy or valu_frac or 0) > hbm_frac else "hbm",
         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_frac,
@@ -636,6 +646,7 @@ def main():
         "valu_lanes_active_frac": valu_lanes,
         "valu_wave_instructions_per_launch": valu,
         "counters_source": profile_source("batched256_valu.md") if counted else None,
+        "frame": frame_roof,
     }
     out = {
         "metric": "node-substeps/s, batched vs plain, at \u22641e-5 rel error; p50/p99 frame latency",
