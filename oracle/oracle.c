@@ -98,8 +98,8 @@ static const int KUHN_TETS[6][4] = {
 static m3 KUHN_INV[6];
 /* Every Kuhn tetrahedron is a path of three axis steps from corner 0 to
  * corner 7: PATH[t] lists its corners in path order, AXIS[t][j] the axis of
- * step j, POS[t][i] the path position of the tet's i-th vertex. */
-static int PATH[6][4], AXIS[6][3], POS[6][4];
+ * step j. */
+static int PATH[6][4], AXIS[6][3];
 static real STRETCH_LEN[28];
 static int STRETCH_A[28], STRETCH_B[28];
 static int tables_ready;
@@ -147,9 +147,6 @@ static void build_tables(void)
             }
             PATH[t][j + 1] = at;
         }
-        for (int i = 0; i < 4; i++)
-            for (int j = 0; j < 4; j++)
-                if (PATH[t][j] == KUHN_TETS[t][i]) POS[t][i] = j;
     }
     /* STRETCH_EDGES, stretch.js:5-7: every corner pair i<j with its unit-cell
      * length, stored as an f32 literal by the kernel generator. */
@@ -238,99 +235,105 @@ static void project_tet(v3 *p, const v3 *prev, const real *w, const int *tet, m3
  * mathematics is project.wgsl's, rewritten so that it needs no square root,
  * one division, and no 3x3 scaling:
  *
- * - Every Kuhn tetrahedron is an axis path, so F = Ds * Dm^-1 = s * E, with
- *   E the path's edge differences (axis by axis) and s = 1/h; the cofactor
- *   columns are H = s^2 * K, K = (E1 x E2, E2 x E0, E0 x E1), and det F =
- *   s^3 * det E.
+ * - Every Kuhn tetrahedron is an axis path P0 = corner 0, P1, P2, P3 = corner
+ *   7, so F = Ds * Dm^-1 = s * E, with E the path's edge differences A_j =
+ *   p(P_j+1) - p(P_j) placed by axis and s = 1/h; the cofactor columns are
+ *   H = s^2 * K, K = (E1 x E2, E2 x E0, E0 x E1) (K_j below: the column of
+ *   step j's axis), and det F = s^3 * det E.
  * - (dC/dF) * B^T gives each path corner the difference of the per-axis
- *   gradients into and out of it: d_i = c * e_i with c = s / |E| and e_i
- *   the edge differences of E, h_i = s^3 * f_i with f_i those of K
- *   (d0 = -(d1 + d2 + d3) holds by construction).
- * - With u = c * lambda_D and v = s^3 * lambda_H, and the first row of the
- *   2x2 block of xpbdCouple.wgsl:58-71 multiplied by 1/c^2 = |E|^2 h^2, the
- *   block becomes
- *       [Ge st + aD h^2 |E|^2,  Gef st        ] [u]   [-(|E|^2 + r Re)      ]
- *       [s^6 Gef st,            s^6 Gf st + aH] [v] = [-s^3 (C_H + r s^3 Rf)]
+ *   gradients into and out of it: d = c * e with c = s / |E| and e = -A_0,
+ *   A_0 - A_1, A_1 - A_2, A_2 by path position, h = s^3 * f with f the same
+ *   differences of the K_j.
+ * - Sums over the corners are taken over the path's edges instead:
+ *   sum_i w_i |e_i|^2 = (W0 + W1)|A0|^2 + (W1 + W2)|A1|^2 + (W2 + W3)|A2|^2
+ *   - 2 (W1 A0.A1 + W2 A1.A2) (W_j the weight at path position j), the same
+ *   for f over K (its dot products from the edges' by Binet-Cauchy), and sum_i e_i . (p_i - q_i) = sum_j A_j . D_j with D_j =
+ *   A_j - (q(P_j+1) - q(P_j)), the edge's own change since the substep's
+ *   start (a rigid translation drops out exactly); the same for f with K_j.
+ * - With u = c * lambda_D and v = s^3 * lambda_H, the first row of the 2x2
+ *   block of xpbdCouple.wgsl:58-71 multiplied by 1/c^2 = |E|^2 h^2 and the
+ *   second by h^6, the block is symmetric:
+ *       [Ge st + aD h^2 |E|^2,  Gef st         ] [u]   [-(|E|^2 + r Re)      ]
+ *       [Gef st,                Gf st + aH h^6 ] [v] = [-(det E - gamma h^3 + r Rf)]
  *   (G.. = sum w e.e, e.f, f.f; R. = sum e.x, f.x; st = 1 + r), using
- *   c * C_D = s^2. Its determinant is the original one times |E|^2 h^2, so
- *   the singular-block test compares with 1e-20 h^2 |E|^2. The corner
- *   update is w_i (e_i u + f_i v), the correction formed before it meets the
- *   position (one rounding at the position's scale).
+ *   c * C_D = s^2. Its determinant times s^6 is the original one times
+ *   |E|^2 h^2, so the singular-block test compares that with 1e-20 h^2
+ *   |E|^2. Step j's edge moves by G_j = A_j u + K_j v, and corner P_j by
+ *   W_j (G_j-1 - G_j) (G_-1 = G_3 = 0), the correction formed before it
+ *   meets the position (one rounding at the position's scale).
  * - A tetrahedron with |F| <= 1e-9 (|E|^2 <= 1e-18 h^2) has no deviatoric
- *   gradient: u = 0, and the block is the original diagonal one,
- *   [aD, 0; 0, s^6 Gf st + aH], tested against 1e-20 unscaled.
+ *   gradient: u = 0, and the block is the original diagonal one scaled,
+ *   [aD, 0; 0, Gf st + aH h^6], whose determinant times s^6 is tested
+ *   against 1e-20.
  * Products feeding a sum are fused (one rounding), and the block divides
  * once. */
 static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real h, real gamma, real aD, real aH,
                               real relax)
 {
-    const int *tv = KUHN_TETS[t];
-    real wt[4] = { w[tv[0]], w[tv[1]], w[tv[2]], w[tv[3]] };
+    const int *path = PATH[t];
+    real W0 = w[path[0]], W1 = w[path[1]], W2 = w[path[2]], W3 = w[path[3]];
     /* Weights are inverse masses, never negative: the sum is 0 exactly when
-     * the WGSL's is. Grouped by path position (ends, middle) for Gef below. */
-    real wend = wt[0] + wt[3], wmid = wt[1] + wt[2];
+     * the WGSL's is. */
+    real wend = W0 + W3, wmid = W1 + W2;
     if (wend + wmid == (real)0) return;
     /* Per-body constants (the kernel forms them once per tile). */
-    real s = (real)1 / h, s2 = s * s, s3 = s2 * s, s6 = s3 * s3, h2 = h * h;
-    real aDh2 = aD * h2, devMin = (real)(float)1e-18 * h2, detMin = (real)(float)1e-20 * h2;
-    v3 E[3];
-    for (int j = 0; j < 3; j++) E[AXIS[t][j]] = sub(p[PATH[t][j + 1]], p[PATH[t][j]]);
-    real e2 = E[0].x * E[0].x;
-    e2 = FMA(E[0].y, E[0].y, e2);
-    e2 = FMA(E[0].z, E[0].z, e2);
-    for (int c = 1; c < 3; c++) {
-        e2 = FMA(E[c].x, E[c].x, e2);
-        e2 = FMA(E[c].y, E[c].y, e2);
-        e2 = FMA(E[c].z, E[c].z, e2);
+    real s = (real)1 / h, s2 = s * s, s3 = s2 * s, s6 = s3 * s3, h2 = h * h, h3 = h2 * h, h6 = h3 * h3;
+    real aDh2 = aD * h2, aHh6 = aH * h6, gh3 = gamma * h3;
+    real devMin = (real)(float)1e-18 * h2, detMin = (real)(float)1e-20 * h2;
+    v3 A[3], E[3], K[3], Kp[3];
+    for (int j = 0; j < 3; j++) {
+        A[j] = sub(p[path[j + 1]], p[path[j]]);
+        E[AXIS[t][j]] = A[j];
     }
-    v3 K[3] = { crossf(E[1], E[2]), crossf(E[2], E[0]), crossf(E[0], E[1]) };
+    K[0] = crossf(E[1], E[2]);
+    K[1] = crossf(E[2], E[0]);
+    K[2] = crossf(E[0], E[1]);
+    real n[3];
+    for (int j = 0; j < 3; j++) {
+        Kp[j] = K[AXIS[t][j]];
+        n[j] = dotf(A[j], A[j]);
+    }
+    real d01 = dotf(A[0], A[1]), d12 = dotf(A[1], A[2]), d02 = dotf(A[0], A[2]);
+    real e2 = (n[0] + n[1]) + n[2];
     real detE = dotf(E[0], K[0]);
-    real cHyd = FMA(detE, s3, -gamma);
     int dev = e2 > devMin;
-    v3 e[4], f[4], x[4];
-    for (int i = 0; i < 4; i++) {
-        int j = POS[t][i];
-        if (j == 0) {
-            e[i] = neg(E[AXIS[t][0]]);
-            f[i] = neg(K[AXIS[t][0]]);
-        } else if (j == 3) {
-            e[i] = E[AXIS[t][2]];
-            f[i] = K[AXIS[t][2]];
-        } else {
-            e[i] = sub(E[AXIS[t][j - 1]], E[AXIS[t][j]]);
-            f[i] = sub(K[AXIS[t][j - 1]], K[AXIS[t][j]]);
-        }
-        x[i] = sub(p[tv[i]], q[tv[i]]);
-    }
-    real ge = 0, gf = 0, re = 0, rf = 0;
-    for (int i = 0; i < 4; i++) {
-        ge = FMA(wt[i], dotf(e[i], e[i]), ge);
-        gf = FMA(wt[i], dotf(f[i], f[i]), gf);
-        re = dotacc(e[i], x[i], re);
-        rf = dotacc(f[i], x[i], rf);
-    }
+    real W01 = W0 + W1, W23 = W2 + W3;
+    real ge = FMA(W23, n[2], FMA(wmid, n[1], W01 * n[0]));
+    ge = FMA((real)-2, FMA(W2, d12, W1 * d01), ge);
+    /* The K_j are the path's edges crossed pairwise (K_0 = +-A1 x A2, K_1 =
+     * +-A2 x A0, K_2 = +-A0 x A1, one sign for all three: + when the path's
+     * axes run cyclically), so by Binet-Cauchy |K_0|^2 = |A1|^2 |A2|^2 -
+     * (A1.A2)^2, K_0.K_1 = (A1.A2)(A0.A2) - (A0.A1)|A2|^2, and so on. */
+    real m0 = FMA(n[1], n[2], -(d12 * d12)), m1 = FMA(n[2], n[0], -(d02 * d02)), m2 = FMA(n[0], n[1], -(d01 * d01));
+    real k01 = FMA(d12, d02, -(d01 * n[2])), k12 = FMA(d02, d01, -(d12 * n[0]));
+    real gf = FMA(W23, m2, FMA(wmid, m1, W01 * m0));
+    gf = FMA((real)-2, FMA(W2, k12, W1 * k01), gf);
+    v3 D[3];
+    for (int j = 0; j < 3; j++) D[j] = sub(A[j], sub(q[path[j + 1]], q[path[j]]));
+    real re = dotacc(A[2], D[2], dotacc(A[1], D[1], dotf(A[0], D[0])));
+    real rf = dotacc(Kp[2], D[2], dotacc(Kp[1], D[1], dotf(Kp[0], D[0])));
     /* e_i . f_i is det E at the path's ends and 2 det E inside it (E_a . K_a =
      * det E, E_a . K_b = 0 for a != b), so Gef = det E * sum of the weights
-     * by path position (1, 2, 2, 1). Every Kuhn tetrahedron lists its path's
-     * ends, corners 0 and 7, first and last. */
+     * by path position (1, 2, 2, 1). */
     real gef = detE * FMA((real)2, wmid, wend);
     real st = (real)1 + relax;
-    real A = dev ? FMA(ge, st, aDh2 * e2) : aD;
-    real B = dev ? gef * st : (real)0;
-    real C = dev ? (s6 * gef) * st : (real)0;
-    real D = FMA(s6 * gf, st, aH);
-    real det = FMA(A, D, -(B * C));
-    if (det < (dev ? detMin * e2 : (real)(float)1e-20)) return;
+    real Ab = dev ? FMA(ge, st, aDh2 * e2) : aD;
+    real Bb = dev ? gef * st : (real)0;
+    real Db = FMA(gf, st, aHh6);
+    real det = FMA(Ab, Db, -(Bb * Bb));
+    if (det * s6 < (dev ? detMin * e2 : (real)(float)1e-20)) return;
     real P = dev ? -FMA(relax, re, e2) : (real)0;
-    real Q = -(s3 * FMA(relax, s3 * rf, cHyd));
+    real Q = -FMA(relax, rf, detE - gh3);
     real inv = (real)1 / det;
-    real u = FMA(D, P, -(B * Q)) * inv;
-    real v = FMA(A, Q, -(C * P)) * inv;
-    for (int i = 0; i < 4; i++) {
-        v3 g = scale(e[i], u);
-        g = mk(FMA(f[i].x, v, g.x), FMA(f[i].y, v, g.y), FMA(f[i].z, v, g.z));
-        v3 *pi = &p[tv[i]];
-        *pi = mk(FMA(g.x, wt[i], pi->x), FMA(g.y, wt[i], pi->y), FMA(g.z, wt[i], pi->z));
+    real u = FMA(Db, P, -(Bb * Q)) * inv;
+    real v = FMA(Ab, Q, -(Bb * P)) * inv;
+    v3 G[3];
+    for (int j = 0; j < 3; j++) G[j] = mk(FMA(Kp[j].x, v, A[j].x * u), FMA(Kp[j].y, v, A[j].y * u), FMA(Kp[j].z, v, A[j].z * u));
+    v3 c[4] = { neg(G[0]), sub(G[0], G[1]), sub(G[1], G[2]), G[2] };
+    real W[4] = { W0, W1, W2, W3 };
+    for (int j = 0; j < 4; j++) {
+        v3 *pj = &p[path[j]];
+        *pj = mk(FMA(c[j].x, W[j], pj->x), FMA(c[j].y, W[j], pj->y), FMA(c[j].z, W[j], pj->z));
     }
 }
 

@@ -38,18 +38,17 @@ __device__ __forceinline__ V3 lerpf(V3 a, V3 d, float l) { return mk(fmaf(l, d.x
 // F = Ds * Dm^-1 has the path's edge differences times 1/h as its columns,
 // axis by axis, and (dC/dF) * B^T hands each path corner the difference of
 // the per-axis gradients of the steps into and out of it. `path` lists the
-// corners in path order, `axis[j]` the axis of step j, `pos[i]` the path
-// position of the tet's i-th vertex; all computed at compile time.
+// corners in path order, `axis[j]` the axis of step j; both computed at
+// compile time.
 struct Tet {
     int v[4];
     int path[4];
     int axis[3];
-    int pos[4];
 };
 
 constexpr Tet make_tet(int a, int b, int c, int d)
 {
-    Tet t{ { a, b, c, d }, { 0, 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0, 0 } };
+    Tet t{ { a, b, c, d }, { 0, 0, 0, 0 }, { 0, 0, 0 } };
     int at = 0;
     for (int j = 0; j < 3; j++) {
         for (int ax = 0; ax < 3; ax++) {
@@ -62,9 +61,6 @@ constexpr Tet make_tet(int a, int b, int c, int d)
         }
         t.path[j + 1] = at;
     }
-    for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++)
-            if (t.path[j] == t.v[i]) t.pos[i] = j;
     return t;
 }
 
@@ -80,94 +76,93 @@ __device__ __forceinline__ V3 crossf(V3 a, V3 b)
 
 // One tetrahedron of project.wgsl:75-153 (the stable Neo-Hookean pair solved
 // as the 2x2 block of xpbdCouple.wgsl:58-71), in the evaluation order the
-// oracle's project_tet_fused defines. With E the path's edge differences
-// (F = s E, s = 1/h), K = cof(E) (cof F = s^2 K), c = s/|E|, the scaled
-// multipliers u = c lambda_D, v = s^3 lambda_H, and the block's first row
-// multiplied by 1/c^2 = |E|^2 h^2, the block is
-//     [Ge st + aD h^2 |E|^2,  Gef st        ] [u]   [-(|E|^2 + r Re)      ]
-//     [s^6 Gef st,            s^6 Gf st + aH] [v] = [-s^3 (C_H + r s^3 Rf)]
-// over the corner differences e of E and f of K (G.. weighted Gram sums,
-// R. rates, st = 1 + r), and corner i moves by w_i (e_i u + f_i v). Same
-// mathematics as the WGSL form, with no square root, one division and no
-// 3x3 scaling of F or of the gradients. |F| <= 1e-9 leaves the deviatoric
-// row out (u = 0), as the WGSL's zero gradient does.
+// oracle's project_tet_fused defines. A_j = p(P_j+1) - p(P_j) are the path's
+// edges (E: the same by axis, F = s E, s = 1/h), K = cof(E) (cof F = s^2 K),
+// K_j the column of step j's axis. Corner sums become edge sums: with W_j the
+// weight at path position j,
+//     Ge = (W0+W1)|A0|^2 + (W1+W2)|A1|^2 + (W2+W3)|A2|^2 - 2(W1 A0.A1 + W2 A1.A2)
+// (Gf the same over K), and the rates Re = sum_j A_j.D_j, Rf = sum_j K_j.D_j
+// over each edge's change since the substep's start D_j = A_j - (q(P_j+1) -
+// q(P_j)). With c = s/|E|, the scaled multipliers u = c lambda_D, v = s^3
+// lambda_H, the block's first row multiplied by 1/c^2 = |E|^2 h^2 and its
+// second by h^6, the block is symmetric:
+//     [Ge st + aD h^2 |E|^2,  Gef st        ] [u]   [-(|E|^2 + r Re)            ]
+//     [Gef st,                Gf st + aH h^6] [v] = [-(det E - gamma h^3 + r Rf)]
+// (st = 1 + r), and edge j moves by G_j = A_j u + K_j v, corner P_j by
+// W_j (G_j-1 - G_j). Same mathematics as the WGSL form, with no square root,
+// one division and no 3x3 scaling of F or of the gradients. |F| <= 1e-9
+// leaves the deviatoric row out (u = 0), as the WGSL's zero gradient does;
+// the singular-block test scales the determinant back by s^6.
 struct TetConsts {
-    float s3, s6, gamma, aD, aH, aDh2, relax, devMin, detMin;
+    float s6, gh3, aD, aHh6, aDh2, relax, devMin, detMin;
 };
 
 template <int T>
 __device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const float (&w)[8], const TetConsts &k)
 {
     constexpr Tet tet = TETS[T];
-    const float wt[4] = { w[tet.v[0]], w[tet.v[1]], w[tet.v[2]], w[tet.v[3]] };
+    static_assert(tet.path[0] == 0 && tet.path[3] == 7, "a Kuhn tetrahedron runs from corner 0 to corner 7");
+    const float W0 = w[tet.path[0]], W1 = w[tet.path[1]], W2 = w[tet.path[2]], W3 = w[tet.path[3]];
     // Inverse masses are never negative: this sum is 0 exactly when the
-    // WGSL's is. Grouped by path position (ends, middle) for Gef below.
-    static_assert(tet.v[0] == 0 && tet.v[3] == 7, "a Kuhn tetrahedron lists its path's ends first and last");
-    const float wend = wt[0] + wt[3], wmid = wt[1] + wt[2];
+    // WGSL's is.
+    const float wend = W0 + W3, wmid = W1 + W2;
     if (wend + wmid == 0.0f) return;
 
-    V3 E[3];
+    V3 A[3], E[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) E[tet.axis[j]] = sub(p[tet.path[j + 1]], p[tet.path[j]]);
-    float e2 = E[0].x * E[0].x;
-    e2 = fmaf(E[0].y, E[0].y, e2);
-    e2 = fmaf(E[0].z, E[0].z, e2);
-#pragma unroll
-    for (int c = 1; c < 3; c++) {
-        e2 = fmaf(E[c].x, E[c].x, e2);
-        e2 = fmaf(E[c].y, E[c].y, e2);
-        e2 = fmaf(E[c].z, E[c].z, e2);
+    for (int j = 0; j < 3; j++) {
+        A[j] = sub(p[tet.path[j + 1]], p[tet.path[j]]);
+        E[tet.axis[j]] = A[j];
     }
     const V3 K[3] = { crossf(E[1], E[2]), crossf(E[2], E[0]), crossf(E[0], E[1]) };
+    const V3 Kp[3] = { K[tet.axis[0]], K[tet.axis[1]], K[tet.axis[2]] };
+    float n[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) n[j] = dotf(A[j], A[j]);
+    const float d01 = dotf(A[0], A[1]), d12 = dotf(A[1], A[2]), d02 = dotf(A[0], A[2]);
+    const float e2 = (n[0] + n[1]) + n[2];
     const float detE = dotf(E[0], K[0]);
-    const float cHyd = fmaf(detE, k.s3, -k.gamma);
     const bool dev = e2 > k.devMin;
-    V3 e[4], f[4], x[4];
+    const float W01 = W0 + W1, W23 = W2 + W3;
+    float ge = fmaf(W23, n[2], fmaf(wmid, n[1], W01 * n[0]));
+    ge = fmaf(-2.0f, fmaf(W2, d12, W1 * d01), ge);
+    // The K_j are the path's edges crossed pairwise, all with one sign, so
+    // (Binet-Cauchy) |K_0|^2 = |A1|^2 |A2|^2 - (A1.A2)^2, K_0.K_1 = (A1.A2)(A0.A2)
+    // - (A0.A1)|A2|^2, and so on round the path.
+    const float m0 = fmaf(n[1], n[2], -(d12 * d12)), m1 = fmaf(n[2], n[0], -(d02 * d02));
+    const float m2 = fmaf(n[0], n[1], -(d01 * d01));
+    const float k01 = fmaf(d12, d02, -(d01 * n[2])), k12 = fmaf(d02, d01, -(d12 * n[0]));
+    float gf = fmaf(W23, m2, fmaf(wmid, m1, W01 * m0));
+    gf = fmaf(-2.0f, fmaf(W2, k12, W1 * k01), gf);
+    V3 D[3];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int j = tet.pos[i];
-        if (j == 0) {
-            e[i] = neg(E[tet.axis[0]]);
-            f[i] = neg(K[tet.axis[0]]);
-        } else if (j == 3) {
-            e[i] = E[tet.axis[2]];
-            f[i] = K[tet.axis[2]];
-        } else {
-            e[i] = sub(E[tet.axis[j - 1]], E[tet.axis[j]]);
-            f[i] = sub(K[tet.axis[j - 1]], K[tet.axis[j]]);
-        }
-        x[i] = sub(p[tet.v[i]], q[tet.v[i]]);
-    }
-    float ge = 0.f, gf = 0.f, re = 0.f, rf = 0.f;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        ge = fmaf(wt[i], dotf(e[i], e[i]), ge);
-        gf = fmaf(wt[i], dotf(f[i], f[i]), gf);
-        re = dotacc(e[i], x[i], re);
-        rf = dotacc(f[i], x[i], rf);
-    }
+    for (int j = 0; j < 3; j++) D[j] = sub(A[j], sub(q[tet.path[j + 1]], q[tet.path[j]]));
+    const float re = dotacc(A[2], D[2], dotacc(A[1], D[1], dotf(A[0], D[0])));
+    const float rf = dotacc(Kp[2], D[2], dotacc(Kp[1], D[1], dotf(Kp[0], D[0])));
     // e_i . f_i is det E at the path's ends and 2 det E inside it (E_a . K_a
-    // = det E, E_a . K_b = 0 for a != b): Gef = det E * (w0 + 2 w1 + 2 w2 + w3)
-    // by path position.
+    // = det E, E_a . K_b = 0 for a != b): Gef = det E * (W0 + 2 W1 + 2 W2 + W3).
     const float gef = detE * fmaf(2.0f, wmid, wend);
     const float st = 1.0f + k.relax;
-    const float A = dev ? fmaf(ge, st, k.aDh2 * e2) : k.aD;
-    const float B = dev ? gef * st : 0.0f;
-    const float C = dev ? (k.s6 * gef) * st : 0.0f;
-    const float D = fmaf(k.s6 * gf, st, k.aH);
-    const float det = fmaf(A, D, -(B * C));
-    if (det < (dev ? k.detMin * e2 : 1e-20f)) return;
+    const float Ab = dev ? fmaf(ge, st, k.aDh2 * e2) : k.aD;
+    const float Bb = dev ? gef * st : 0.0f;
+    const float Db = fmaf(gf, st, k.aHh6);
+    const float det = fmaf(Ab, Db, -(Bb * Bb));
+    if (det * k.s6 < (dev ? k.detMin * e2 : 1e-20f)) return;
     const float P = dev ? -fmaf(k.relax, re, e2) : 0.0f;
-    const float Q = -(k.s3 * fmaf(k.relax, k.s3 * rf, cHyd));
+    const float Q = -fmaf(k.relax, rf, detE - k.gh3);
     const float inv = 1.0f / det;
-    const float u = fmaf(D, P, -(B * Q)) * inv;
-    const float v = fmaf(A, Q, -(C * P)) * inv;
+    const float u = fmaf(Db, P, -(Bb * Q)) * inv;
+    const float v = fmaf(Ab, Q, -(Bb * P)) * inv;
+    V3 G[3];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        V3 g = mul(e[i], u);
-        g = mk(fmaf(f[i].x, v, g.x), fmaf(f[i].y, v, g.y), fmaf(f[i].z, v, g.z));
-        V3 &pi = p[tet.v[i]];
-        pi = mk(fmaf(g.x, wt[i], pi.x), fmaf(g.y, wt[i], pi.y), fmaf(g.z, wt[i], pi.z));
+    for (int j = 0; j < 3; j++)
+        G[j] = mk(fmaf(Kp[j].x, v, A[j].x * u), fmaf(Kp[j].y, v, A[j].y * u), fmaf(Kp[j].z, v, A[j].z * u));
+    const V3 c[4] = { neg(G[0]), sub(G[0], G[1]), sub(G[1], G[2]), G[2] };
+    const float W[4] = { W0, W1, W2, W3 };
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        V3 &pj = p[tet.path[j]];
+        pj = mk(fmaf(c[j].x, W[j], pj.x), fmaf(c[j].y, W[j], pj.y), fmaf(c[j].z, W[j], pj.z));
     }
 }
 
@@ -398,9 +393,9 @@ struct CellConsts {
 __device__ __forceinline__ CellConsts cell_consts(const vs_body &bd, float dt)
 {
     const float invDt2 = 1.0f / (dt * dt);
-    const float h = bd.h, s = 1.0f / h, s2 = s * s, s3 = s2 * s, h2 = h * h;
-    const float aD = bd.alpha_dev * invDt2;
-    return CellConsts{ TetConsts{ s3, s3 * s3, bd.gamma, aD, bd.alpha_hyd * invDt2, aD * h2, bd.relaxation / dt,
+    const float h = bd.h, s = 1.0f / h, s2 = s * s, s3 = s2 * s, h2 = h * h, h3 = h2 * h;
+    const float aD = bd.alpha_dev * invDt2, aH = bd.alpha_hyd * invDt2;
+    return CellConsts{ TetConsts{ s3 * s3, bd.gamma * h3, aD, aH * (h3 * h3), aD * h2, bd.relaxation / dt,
                                   1e-18f * h2, 1e-20f * h2 },
                        h, bd.max_stretch };
 }
