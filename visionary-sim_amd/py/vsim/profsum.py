@@ -4,6 +4,7 @@
     python -m vsim.profsum pmc <fetch_counter_collection.csv> <write_counter_collection.csv> \
         <copy_fetch.csv> <copy_write.csv>
     python -m vsim.profsum valu <counter_collection.csv with SQ_INSTS_VALU [VALUBusy ...]>
+    python -m vsim.profsum phases <..._kernel_trace.csv> [frames]
 
 `trace` prints per (kernel, launch grid) call counts and average durations, so
 the batched launches (large grids) separate from the per-scene ones. `pmc`
@@ -11,7 +12,8 @@ prints FETCH_SIZE / WRITE_SIZE per dispatch, scaled to bytes by what the same
 counters read for a device-to-device copy of known size. `valu` prints
 SQ_INSTS_VALU (wave instructions) per dispatch against the VALU issue peak
 of the dispatch's duration: one wave instruction per CU per cycle, 256 CUs
-at 2.4 GHz.
+at 2.4 GHz. `phases` splits the batched colour-pair launches of a bench.py
+run (no autotuner) by phase and, in the graph-replayed frames, by row class.
 """
 import collections
 import csv
@@ -124,10 +126,42 @@ def valu(path, batched_min=1 << 19):
     return "\n".join(out)
 
 
+def phases(path, frames=4, per_frame=40, kernel="k_elastic_rows", batched_min=1 << 20):
+    """The batched launches of `kernel` in start-time order, as bench.py
+    makes them with the autotuner off: `frames` timed frames (the first
+    captures the frame graph, the rest replay it), one profiled frame (HIP
+    events around each launch), then one `vs_world_time_pass` burst, each of
+    `per_frame` launches; and the replayed frames' launches by row class
+    (launch index mod 4)."""
+    rows = []
+    for r in csv.DictReader(open(path)):
+        grid = int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0)
+        if short(r["Kernel_Name"]).startswith(kernel) and grid >= batched_min:
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+    rows.sort()
+    t = [d for _, d in rows]
+    avg = lambda v: sum(v) / len(v) / 1e3 if v else float("nan")
+    cut = [0, per_frame, frames * per_frame, (frames + 1) * per_frame, (frames + 2) * per_frame]
+    names = ["first frame (graph capture)", "graph-replayed frames", "profiled frame (HIP events around every launch)",
+             "`vs_world_time_pass` burst (back to back, no folding)"]
+    out = [f"{len(t)} batched `{kernel}` launches by phase (start-time order):", "",
+           "| phase | launches | avg us |", "|---|---:|---:|"]
+    for name, a, b in zip(names, cut, cut[1:]):
+        out.append(f"| {name} | {len(t[a:b])} | {avg(t[a:b]):.2f} |")
+    replay = t[cut[1]:cut[2]]
+    out += ["", "Graph-replayed frames by row class:", "", "| class (colours) | launches | avg us |", "|---|---:|---:|"]
+    for c in range(4):
+        v = replay[c::4]
+        out.append(f"| {c} ({2 * c}, {2 * c + 1}) | {len(v)} | {avg(v):.2f} |")
+    return "\n".join(out)
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
         print(trace(sys.argv[2]))
     elif sys.argv[1] == "valu":
         print(valu(sys.argv[2]))
+    elif sys.argv[1] == "phases":
+        print(phases(sys.argv[2], *(int(a) for a in sys.argv[3:4])))
     else:
         print(pmc(*sys.argv[2:6]))
