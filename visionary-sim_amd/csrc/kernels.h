@@ -69,7 +69,16 @@ struct GridView {
     uint32_t ovb_cap = 0;
     uint32_t *ext = nullptr;
     uint32_t ext_slots = 0;
+    // Per bucket, for collide: the slots it holds (at most the slot count, 6
+    // bits) | the body all its records belong to << 6 (SUMMARY_MIXED when
+    // several, or past 8 records). Sort writes it for every occupied bucket,
+    // settle clears it, so an empty bucket reads 0.
+    uint32_t *summary = nullptr;
 };
+
+constexpr uint32_t SUMMARY_MIXED = 0x3ffffffu;
+__device__ __forceinline__ uint32_t summary_held(uint32_t w) { return w & 63u; }
+__device__ __forceinline__ uint32_t summary_sole(uint32_t w) { return w >> 6; }
 
 // Optional log of the pairs collide accepts (record, mate), `cap` per record.
 struct PairLog {
@@ -192,8 +201,9 @@ hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scen
                               SurfaceView sv, GridView g, TileView t);
 hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv,
                                GridView g, TileView t);
-// With sv.pay, nine lanes a record over tiles of at most `tile_max` records;
-// without, one lane a record through the node ids (the reference's form).
+// With sv.pay and g.summary, nine lanes a record over tiles of at most
+// `tile_max` records; without, one lane a record through the node ids (the
+// reference's form).
 hipError_t launch_contact_collide(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
                                   const vs_scene_params *scenes, SurfaceView sv, GridView g, TileView t,
                                   PairLog log, uint32_t tile_max);

@@ -1047,6 +1047,23 @@ __device__ __forceinline__ bool overflow_members(GridView g, uint32_t *base, uin
     return true;
 }
 
+// Whether the first `held` (2..8) records of bucket slots `base` all belong
+// to body `own`: every load issued before any is compared (slots past
+// `held` read slot 0 again).
+__device__ __forceinline__ bool bucket_of_one_body(const uint32_t *base, uint32_t held, uint32_t own,
+                                                   const uint32_t *__restrict__ body)
+{
+    uint32_t id[8], b[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) id[k] = base[(uint32_t)k < held ? k : 0];
+#pragma unroll
+    for (int k = 0; k < 8; k++) b[k] = body[id[k]];
+    bool same = true;
+#pragma unroll
+    for (int k = 0; k < 8; k++) same = same && b[k] == own;
+    return same;
+}
+
 // sort.wgsl:14-53, run by the record that arrived first in each bucket; the
 // overfull buckets are taken from bin's list of them by whichever wave asks
 // next (or, without the list, by their first record's wave), from their
@@ -1110,6 +1127,16 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restric
                 // together, sorted in registers by a network of 8, 16 or 32
                 // (ids past the bucket's count pad with ~0 and sort last),
                 // written back together.
+                if (g.summary) {
+                    // The bucket's summary for collide: slots held and the
+                    // body all its records belong to. This record is one of
+                    // them; past 8 records the bucket counts as mixed.
+                    const uint32_t own = sv.body[tile.begin + i];
+                    uint32_t sole = over || held > 8u ? SUMMARY_MIXED : own;
+                    if (!over && held > 1u && held <= 8u && !bucket_of_one_body(base, held, own, sv.body))
+                        sole = SUMMARY_MIXED;
+                    g.summary[gb + b] = (held < slots ? held : slots) | sole << 6;
+                }
                 if (!over && held > 1u && held <= 32u) {
                     if (held <= 8u) sort_slots<8>(base, held);
                     else if (held <= 16u) sort_slots<16>(base, held);
@@ -1296,16 +1323,19 @@ __global__ __launch_bounds__(BLOCK) void k_collide(NodeView n, const vs_body *__
 }
 
 // The same pass over bin's payload (sv.pay: a record's position and radius
-// beside it), nine lanes a record. A record's 27 neighbour buckets are nine
-// rows of three along x, (dz, dy) = row / 3 - 1, row % 3 - 1; lane `row` of
-// the record's group walks its row (loads four slots at a time, ids then
-// bodies and payloads), resolves each mate in reach as it meets it, and sums
-// the row's corrections; the group's first lane adds the nine row sums in
-// row order (the oracle's order: the row sums of collide.wgsl's mates). With
-// a pair log, a first walk counts each row's mates in reach, so that every
-// row logs its mates at its place in the record's mate order. A wave holds
-// seven records (lane 63 idles); a workgroup takes WAVES x 7 records of a
-// tile at a time, so a tile is `parts` work items.
+// beside it) and sort's bucket summaries (slots held, sole body), nine lanes
+// a record. A record's 27 neighbour buckets are nine rows of three along x,
+// (dz, dy) = row / 3 - 1, row % 3 - 1; lane `row` of the record's group
+// walks its row (loads four slots at a time, ids then bodies and payloads),
+// resolves each mate in reach as it meets it, and sums the row's
+// corrections; the group's first lane adds the nine row sums in row order
+// (the oracle's order: the row sums of collide.wgsl's mates). A bucket whose
+// records all belong to the record's own body is passed by, as is one beyond
+// reach: every mate there fails the reach test. With a pair log, a first
+// walk counts each row's mates in reach, so that every row logs its mates at
+// its place in the record's mate order. A wave holds seven records (lane 63
+// idles); a workgroup takes WAVES x 7 records of a tile at a time, so a tile
+// is `parts` work items.
 constexpr uint32_t ROW_LANES = 9;
 constexpr uint32_t ROW_GROUPS = 64 / ROW_LANES;
 
@@ -1315,33 +1345,81 @@ constexpr uint32_t collide_parts(uint32_t tile_max)
     return (tile_max + (BLOCK / 64) * ROW_GROUPS - 1) / ((BLOCK / 64) * ROW_GROUPS);
 }
 
-// Walks the row's buckets bk[0..2] (held[ox] slots each) in mate order,
-// four slots at a time, and calls f(other, mateBody, payload) for each.
+// Walks bucket bk's first `held` slots in mate order, four at a time, and
+// calls f(other, mateBody, payload) for each.
+template <typename F>
+__device__ __forceinline__ void walk_bucket(uint32_t bk, uint32_t held, uint32_t slots, uint32_t ownBody, GridView g,
+                                            SurfaceView sv, F f)
+{
+    const uint32_t *base = g.slots + (size_t)bk * slots;
+    for (uint32_t k0 = 0; k0 < held; k0 += 4u) {
+        uint32_t o[4], mb[4];
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) o[u] = k0 + u < held ? base[k0 + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            mb[u] = ownBody;  // past the bucket's end: the record's own body, never in reach
+            q[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (k0 + u < held) {
+                mb[u] = sv.body[o[u]];
+                q[u] = sv.pay[o[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) f(o[u], mb[u], q[u]);
+    }
+}
+
+// The row's buckets bk[0..2] (held[ox] slots each) in mate order.
 template <typename F>
 __device__ __forceinline__ void walk_row(const uint32_t (&bk)[3], const uint32_t (&held)[3], uint32_t slots,
                                          uint32_t ownBody, GridView g, SurfaceView sv, F f)
 {
 #pragma unroll
-    for (int ox = 0; ox < 3; ox++) {
-        const uint32_t *base = g.slots + (size_t)bk[ox] * slots;
-        for (uint32_t k0 = 0; k0 < held[ox]; k0 += 4u) {
-            uint32_t o[4], mb[4];
-            float4 q[4];
+    for (int ox = 0; ox < 3; ox++) walk_bucket(bk[ox], held[ox], slots, ownBody, g, sv, f);
+}
+
+// A record's neighbour buckets out of reach, per axis: the bucket below
+// when the record sits farther than the largest reach (plus the rounding
+// margin) above its bucket's lower face, the bucket above likewise. Mates
+// there fail the reach test, so passing them by changes nothing (with the
+// reference's default grid, buckets twice the widest radius, nothing is
+// ever passed by).
+__device__ __forceinline__ void reach_flags(const vs_scene_params &sc, V3 p, int cx, int cy, int cz, bool (&lo)[3],
+                                            bool (&hi)[3])
+{
 #pragma unroll
-            for (int u = 0; u < 4; u++) o[u] = k0 + u < held[ox] ? base[k0 + u] : 0u;
+    for (int a = 0; a < 3; a++) lo[a] = hi[a] = false;
+    if (sc.reach_max > 0.0f) {
+        const float bound = sc.reach_max + sc.skip_margin, size = sc.bucket_size;
+        const float pc[3] = { p.x, p.y, p.z };
+        const int cell[3] = { cx, cy, cz };
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                mb[u] = ownBody;  // past the bucket's end: the record's own body, never in reach
-                q[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (k0 + u < held[ox]) {
-                    mb[u] = sv.body[o[u]];
-                    q[u] = sv.pay[o[u]];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) f(o[u], mb[u], q[u]);
+        for (int a = 0; a < 3; a++) {
+            const float f = (pc[a] - sc.grid_origin[a]) / size - (float)cell[a];
+            lo[a] = f * size > bound;
+            hi[a] = (1.0f - f) * size > bound;
         }
     }
+}
+
+// Neighbour bucket (ox, oy, oz) of a record in bucket (cx, cy, cz): its
+// index (grid base added), and the slots the record walks there: 0 outside
+// the grid, beyond reach, or when the bucket's records are all of `body`.
+__device__ __forceinline__ uint32_t row_bucket(const vs_scene_params &sc, GridView g, int cx, int cy, int cz, int ox,
+                                               int oy, int oz, const bool (&lo)[3], const bool (&hi)[3], uint32_t body,
+                                               uint32_t &held)
+{
+    const int x = cx + ox, y = cy + oy, z = cz + oz;
+    const int dx = (int)sc.grid_dims[0], dy = (int)sc.grid_dims[1], dz = (int)sc.grid_dims[2];
+    const bool in = x >= 0 && x < dx && y >= 0 && y < dy && z >= 0 && z < dz && !(ox < 0 && lo[0]) &&
+                    !(ox > 0 && hi[0]) && !(oy < 0 && lo[1]) && !(oy > 0 && hi[1]) && !(oz < 0 && lo[2]) &&
+                    !(oz > 0 && hi[2]);
+    const uint32_t bk = in ? sc.grid_base + (uint32_t)(x + dx * (y + dy * z)) : 0u;
+    const uint32_t w = in ? g.summary[bk] : 0u;
+    held = summary_sole(w) == body ? 0u : summary_held(w);
+    return bk;
 }
 
 template <int BLOCK>
@@ -1361,8 +1439,8 @@ __global__ __launch_bounds__(BLOCK) void k_collide_rows(NodeView n, const vs_bod
         const uint32_t first = part * PER_PART;
         if (first >= tile.count) continue;  // the whole workgroup: `it` is its own
         const vs_scene_params &sc = scenes[tile.scene];
-        const uint32_t slots = sc.bucket_slots, gb = sc.grid_base;
-        const int dx = (int)sc.grid_dims[0], dy = (int)sc.grid_dims[1], dz = (int)sc.grid_dims[2];
+        const uint32_t slots = sc.bucket_slots;
+        const int dx = (int)sc.grid_dims[0], dy = (int)sc.grid_dims[1];
         const uint32_t i = first + wv * ROW_GROUPS + group;
         const bool active = group < ROW_GROUPS && i < tile.count;
         Collider c;
@@ -1370,36 +1448,11 @@ __global__ __launch_bounds__(BLOCK) void k_collide_rows(NodeView n, const vs_bod
         if (active) {
             int cx, cy, cz;
             load_collider(c, tile.begin + i, n, bodies, sv, dx, dy, cx, cy, cz);
-            // Neighbour buckets beyond reach: along an axis, the bucket below
-            // when the record sits farther than the largest reach (plus the
-            // rounding margin) above its bucket's lower face, the bucket
-            // above likewise. Mates there fail the reach test, so passing
-            // them by changes nothing (with the reference's default grid,
-            // buckets twice the widest radius, nothing is ever passed by).
-            bool lo[3] = { false, false, false }, hi[3] = { false, false, false };
-            if (sc.reach_max > 0.0f) {
-                const float bound = sc.reach_max + sc.skip_margin, size = sc.bucket_size;
-                const float pc[3] = { c.pi.x, c.pi.y, c.pi.z };
-                const int cell[3] = { cx, cy, cz };
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    const float f = (pc[a] - sc.grid_origin[a]) / size - (float)cell[a];
-                    lo[a] = f * size > bound;
-                    hi[a] = (1.0f - f) * size > bound;
-                }
-            }
+            bool lo[3], hi[3];
+            reach_flags(sc, c.pi, cx, cy, cz, lo, hi);
             const int oz = (int)(row / 3u) - 1, oy = (int)(row % 3u) - 1;
-            const int z = cz + oz, y = cy + oy;
-            const bool in_row = z >= 0 && z < dz && y >= 0 && y < dy && !(oz < 0 && lo[2]) && !(oz > 0 && hi[2]) &&
-                                !(oy < 0 && lo[1]) && !(oy > 0 && hi[1]);
 #pragma unroll
-            for (int ox = 0; ox < 3; ox++) {
-                const int x = cx + ox - 1;
-                const bool in = in_row && x >= 0 && x < dx && !(ox == 0 && lo[0]) && !(ox == 2 && hi[0]);
-                bk[ox] = in ? gb + (uint32_t)(x + dx * (y + dy * z)) : 0u;
-                const uint32_t cnt = in ? g.count[bk[ox]] : 0u;
-                held[ox] = cnt < slots ? cnt : slots;
-            }
+            for (int ox = 0; ox < 3; ox++) bk[ox] = row_bucket(sc, g, cx, cy, cz, ox - 1, oy, oz, lo, hi, c.body, held[ox]);
         }
         if (log.cap) {
             // Each row's mates in reach, so that the rows log in mate order.
@@ -1462,7 +1515,10 @@ __global__ __launch_bounds__(BLOCK) void k_settle(NodeView n, const vs_scene_par
         const uint32_t gb = sc.grid_base;
         for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
             const uint32_t s = tile.begin + i;
-            if (sv.rank[s] == 0u) g.count[gb + sv.key[s]] = 0u;
+            if (sv.rank[s] == 0u) {
+                g.count[gb + sv.key[s]] = 0u;
+                if (g.summary) g.summary[gb + sv.key[s]] = 0u;
+            }
             const float h = sv.hits[s];
             if (h > 0.0f) {
                 const uint32_t node = sv.node[s];
@@ -1742,7 +1798,7 @@ hipError_t launch_contact_collide(hipStream_t s, Shape sh, NodeView n, const vs_
                                   const vs_scene_params *scenes, SurfaceView sv, GridView g, TileView t, PairLog log,
                                   uint32_t tile_max)
 {
-    if (!sv.pay) VSIM_DISPATCH_BLOCK(k_collide, sh, s, n, bodies, scenes, sv, g, t, log);
+    if (!sv.pay || !g.summary) VSIM_DISPATCH_BLOCK(k_collide, sh, s, n, bodies, scenes, sv, g, t, log);
     // Row lanes: each tile is `parts` work items of WAVES x 7 records.
     switch (sh.block) {
     case 64: sh.grid *= collide_parts<64>(tile_max); VSIM_DISPATCH_BLOCK(k_collide_rows, sh, s, n, bodies, scenes, sv, g, t, log, collide_parts<64>(tile_max));

@@ -164,6 +164,7 @@ struct vs_world {
     uint32_t *d_ovf_count = nullptr;   // 4 words: records listed, buckets listed, list full (settle empties them), spare
     uint2 *d_ovb = nullptr;            // overfull buckets this substep: (grid bucket, scene)
     uint32_t *d_ext = nullptr;         // EXT_SLOTS more slots a bucket (arrivals 32 .. 32 + EXT_SLOTS - 1)
+    uint32_t *d_summary = nullptr;     // per bucket: slots held and sole body, for collide (GridView::summary)
     uint32_t *d_drive = nullptr;
     float *d_splat_in = nullptr, *d_splat_out = nullptr;
     int32_t *d_pairlog = nullptr;
@@ -263,7 +264,7 @@ uint32_t ovb_cap(const vs_world *w) { return (uint32_t)(w->cfg.max_surface / 2u 
 GridView grid_view(vs_world *w)
 {
     return GridView{ w->d_grid, w->d_grid + w->cfg.max_buckets, w->d_ovf, w->d_ovf_count,
-                     (uint32_t)w->cfg.max_surface, w->d_ovb, ovb_cap(w), w->d_ext, EXT_SLOTS };
+                     (uint32_t)w->cfg.max_surface, w->d_ovb, ovb_cap(w), w->d_ext, EXT_SLOTS, w->d_summary };
 }
 
 vsim::DriveView drive_view(vs_world *w)
@@ -867,6 +868,7 @@ static int world_init(vs_world *w)
 {
     vs_world_config &c = w->cfg;
     if (c.max_scenes == 0 || c.max_bodies == 0 || c.max_nodes == 0) return VS_E_INVALID;
+    if (c.max_bodies >= vsim::SUMMARY_MIXED) return VS_E_INVALID;  // body ids fit a bucket summary
     // Kernels address node planes by 32-bit byte offsets.
     if (c.max_nodes >= (1ull << 29)) return fail(w, VS_E_INVALID, "max_nodes must stay below 2^29");
     if (c.max_cells == 0) c.max_cells = 1;
@@ -893,6 +895,7 @@ static int world_init(vs_world *w)
     ALLOC(W->d_ovf_count, 4 * sizeof(uint32_t));
     ALLOC(W->d_ovb, sizeof(uint2) * ovb_cap(W));
     ALLOC(W->d_ext, sizeof(uint32_t) * EXT_SLOTS * c.max_buckets);
+    ALLOC(W->d_summary, sizeof(uint32_t) * c.max_buckets);
     ALLOC(W->d_drive, sizeof(uint32_t) * DRIVE_PLANES * c.max_driven);
     ALLOC(W->d_splat_in, sizeof(float) * 10 * c.max_splats);
     ALLOC(W->d_splat_out, sizeof(float) * SPLAT_OUT_PLANES * c.max_splats);
@@ -900,6 +903,7 @@ static int world_init(vs_world *w)
     ALLOC(W->d_branch_counts, sizeof(uint32_t) * NPASS * 2);
     HIPW(W, hipMemset(W->d_nodes, 0, sizeof(float) * NODE_PLANES * c.max_nodes));
     HIPW(W, hipMemset(W->d_grid, 0, sizeof(uint32_t) * c.max_buckets));
+    HIPW(W, hipMemset(W->d_summary, 0, sizeof(uint32_t) * c.max_buckets));
     HIPW(W, hipMemset(W->d_ovf_count, 0, 4 * sizeof(uint32_t)));
     HIPW(W, hipMemset(W->d_scenes, 0, sizeof(vs_scene_params) * c.max_scenes));
     W->nodes.init(c.max_nodes);
@@ -991,7 +995,7 @@ void vs_world_destroy(vs_world *w)
                            w->join })
         if (ev) (void)hipEventDestroy(ev);
     void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf, w->d_surf_pay,
-                     w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_ovb, w->d_ext, w->d_drive, w->d_splat_in, w->d_splat_out,
+                     w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_ovb, w->d_ext, w->d_summary, w->d_drive, w->d_splat_in, w->d_splat_out,
                      w->d_pairlog, w->d_scene_counts, w->d_branch_counts };
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1378,6 +1382,7 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
                                                          w->h_bodies.begin() + r.body_begin + d->n_bodies));
     HIPW(w, hipMemcpyAsync(w->d_bodies + r.body_begin, hb_rec, sizeof(vs_body) * d->n_bodies, hipMemcpyHostToDevice, up));
     if (nbuckets) HIPW(w, hipMemsetAsync(w->d_grid + r.bucket_begin, 0, nbuckets * 4, up));
+    if (nbuckets) HIPW(w, hipMemsetAsync(w->d_summary + r.bucket_begin, 0, nbuckets * 4, up));
     if (d->n_driven) {
         const size_t D = w->cfg.max_driven;
         std::vector<uint32_t> dn(d->n_driven);
@@ -1691,7 +1696,7 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out)
     const vs_world_config &c = w->cfg;
     out->bytes_reserved = 4ull * NODE_PLANES * c.max_nodes + 4ull * c.max_cells + sizeof(vs_body) * c.max_bodies +
                           sizeof(vs_scene_params) * c.max_scenes + (4ull * SURF_PLANES + 24) * c.max_surface +
-                          4ull * (1 + BUCKET_SLOTS + EXT_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
+                          4ull * (2 + BUCKET_SLOTS + EXT_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
                           4ull * (10 + SPLAT_OUT_PLANES) * c.max_splats;
     out->graph_captures = w->captures;
     return VS_OK;
