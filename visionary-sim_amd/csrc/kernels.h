@@ -39,6 +39,9 @@ struct SurfaceView {
     // collide (positions do not change between the two). Null: collide reads
     // them through the node id, as the reference does.
     float4 *pay = nullptr;
+    // Optional with `pay`: the record's node prev position and inverse mass,
+    // written by bin, so that collide reaches a mate's in one load.
+    float4 *prev = nullptr;
 };
 
 // contactGrid (bodyContact.js:125-127): counters, then slots per bucket.

@@ -912,6 +912,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin(NodeView n, const vs_scene_params
             const uint32_t b = x + dx * (y + dy * z);
             sv.key[s] = b;
             if (sv.pay) sv.pay[s] = make_float4(p.x, p.y, p.z, sv.radius[s]);
+            if (sv.prev) sv.prev[s] = make_float4(n.qx[node], n.qy[node], n.qz[node], n.w[node]);
             const uint32_t slot = atomicAdd(&g.count[gb + b], 1u);
             sv.rank[s] = slot;
             if (slot < slots) g.slots[(size_t)(gb + b) * slots + slot] = s;
@@ -1208,17 +1209,33 @@ __device__ __forceinline__ void collide_mate(Collider &c, uint32_t other, uint32
         if ((uint32_t)c.logged < log.cap) log.mates[(size_t)c.s * log.cap + c.logged] = other;
         c.logged++;
     }
-    const uint32_t mate = sv.node[other];
-    const float wj = n.w[mate];
+    uint32_t mate = 0u;
+    float wj;
+    V3 qj;
+    if (sv.prev) {
+        const float4 m = sv.prev[other];
+        wj = m.w;
+        qj = mk(m.x, m.y, m.z);
+    } else {
+        mate = sv.node[other];
+        wj = n.w[mate];
+        qj = ld3(n.qx, n.qy, n.qz, mate);
+    }
     const float total = c.wi + wj;
     if (total <= 0.0f) return;
     const float share = c.wi / total;
     const float dist = sqrtf(dist2);
-    V3 normal = mk(0.f, c.node < mate ? 1.0f : -1.0f, 0.f);
-    if (dist > 1e-9f) normal = mk(d.x / dist, d.y / dist, d.z / dist);
+    V3 normal;
+    if (dist > 1e-9f) {
+        normal = mk(d.x / dist, d.y / dist, d.z / dist);
+    } else {
+        // Coincident: +y or -y by node index (collide.wgsl's tie-break).
+        if (sv.prev) mate = sv.node[other];
+        normal = mk(0.f, c.node < mate ? 1.0f : -1.0f, 0.f);
+    }
     const float depth = reach - dist;
     V3 corr = mul(normal, depth * share);
-    const V3 sliding = sub(c.movedI, sub(pj, ld3(n.qx, n.qy, n.qz, mate)));
+    const V3 sliding = sub(c.movedI, sub(pj, qj));
     const V3 tangent = sub(sliding, mul(normal, dot(sliding, normal)));
     const float slid = sqrtf(dot(tangent, tangent));
     if (slid > 1e-9f) {
@@ -1422,8 +1439,10 @@ __device__ __forceinline__ uint32_t row_bucket(const vs_scene_params &sc, GridVi
     return bk;
 }
 
+// Six waves a SIMD (at most 80 VGPRs; the compiler spills two registers):
+// the pass is latency-bound, and at five it runs slower.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_collide_rows(NodeView n, const vs_body *__restrict__ bodies,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_collide_rows(NodeView n, const vs_body *__restrict__ bodies,
                                                         const vs_scene_params *__restrict__ scenes, SurfaceView sv,
                                                         GridView g, TileView tv, PairLog log, uint32_t parts)
 {

@@ -158,7 +158,7 @@ struct vs_world {
     vs_body *d_bodies = nullptr;
     vs_scene_params *d_scenes = nullptr;
     uint32_t *d_surf = nullptr;
-    float4 *d_surf_pay = nullptr;  // per record: node position and radius at bin time
+    float4 *d_surf_pay = nullptr;  // per record: node position and radius, then prev position and inverse mass, at bin time
     uint32_t *d_grid = nullptr;
     uint2 *d_ovf = nullptr;            // records that found their bucket full: (record, grid bucket)
     uint32_t *d_ovf_count = nullptr;   // 4 words: records listed, buckets listed, list full (settle empties them), spare
@@ -254,7 +254,8 @@ SurfaceView surf_view(vs_world *w)
                         reinterpret_cast<float *>(s + 6 * S),
                         reinterpret_cast<float *>(s + 7 * S),
                         reinterpret_cast<float *>(s + 8 * S),
-                        w->d_surf_pay };
+                        w->d_surf_pay,
+                        w->d_surf_pay + w->cfg.max_surface };
 }
 
 // A bucket is overfull past at least one slot, so no more than half the
@@ -889,7 +890,7 @@ static int world_init(vs_world *w)
     ALLOC(W->d_bodies, sizeof(vs_body) * c.max_bodies);
     ALLOC(W->d_scenes, sizeof(vs_scene_params) * c.max_scenes);
     ALLOC(W->d_surf, sizeof(uint32_t) * SURF_PLANES * c.max_surface);
-    ALLOC(W->d_surf_pay, sizeof(float4) * c.max_surface);
+    ALLOC(W->d_surf_pay, 2 * sizeof(float4) * c.max_surface);
     ALLOC(W->d_grid, sizeof(uint32_t) * (1 + BUCKET_SLOTS) * c.max_buckets);
     ALLOC(W->d_ovf, sizeof(uint2) * c.max_surface);
     ALLOC(W->d_ovf_count, 4 * sizeof(uint32_t));
@@ -1695,7 +1696,7 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out)
     }
     const vs_world_config &c = w->cfg;
     out->bytes_reserved = 4ull * NODE_PLANES * c.max_nodes + 4ull * c.max_cells + sizeof(vs_body) * c.max_bodies +
-                          sizeof(vs_scene_params) * c.max_scenes + (4ull * SURF_PLANES + 24) * c.max_surface +
+                          sizeof(vs_scene_params) * c.max_scenes + (4ull * SURF_PLANES + 32) * c.max_surface +
                           4ull * (2 + BUCKET_SLOTS + EXT_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
                           4ull * (10 + SPLAT_OUT_PLANES) * c.max_splats;
     out->graph_captures = w->captures;
