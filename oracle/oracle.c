@@ -247,9 +247,13 @@ static void project_tet(v3 *p, const v3 *prev, const real *w, const int *tet, m3
  * - Sums over the corners are taken over the path's edges instead:
  *   sum_i w_i |e_i|^2 = (W0 + W1)|A0|^2 + (W1 + W2)|A1|^2 + (W2 + W3)|A2|^2
  *   - 2 (W1 A0.A1 + W2 A1.A2) (W_j the weight at path position j), the same
- *   for f over K (its dot products from the edges' by Binet-Cauchy), and sum_i e_i . (p_i - q_i) = sum_j A_j . D_j with D_j =
- *   A_j - (q(P_j+1) - q(P_j)), the edge's own change since the substep's
- *   start (a rigid translation drops out exactly); the same for f with K_j.
+ *   for f over K (its dot products from the edges' by Binet-Cauchy), and
+ *   sum_i e_i . (p_i - q_i) = sum_j A_j . (A_j - Q_j) = |E|^2 - sum_j A_j . Q_j
+ *   with Q_j = q(P_j+1) - q(P_j) the edge at the substep's start; for f,
+ *   sum_j K_j . (A_j - Q_j) = 3 det E - sum_j K_j . Q_j (A_j . K_j = det E).
+ *   The subtraction cancels most of |E|^2 (3 det E); the rate enters the
+ *   block as r Re beside |E|^2, so the multipliers' error stays far below a
+ *   position's rounding.
  * - With u = c * lambda_D and v = s^3 * lambda_H, the first row of the 2x2
  *   block of xpbdCouple.wgsl:58-71 multiplied by 1/c^2 = |E|^2 h^2 and the
  *   second by h^6, the block is symmetric:
@@ -308,10 +312,10 @@ static void project_tet_fused(v3 *p, const v3 *q, const real *w, int t, real h, 
     real k01 = FMA(d12, d02, -(d01 * n[2])), k12 = FMA(d02, d01, -(d12 * n[0]));
     real gf = FMA(W23, m2, FMA(wmid, m1, W01 * m0));
     gf = FMA((real)-2, FMA(W2, k12, W1 * k01), gf);
-    v3 D[3];
-    for (int j = 0; j < 3; j++) D[j] = sub(A[j], sub(q[path[j + 1]], q[path[j]]));
-    real re = dotacc(A[2], D[2], dotacc(A[1], D[1], dotf(A[0], D[0])));
-    real rf = dotacc(Kp[2], D[2], dotacc(Kp[1], D[1], dotf(Kp[0], D[0])));
+    v3 Q0[3];
+    for (int j = 0; j < 3; j++) Q0[j] = sub(q[path[j + 1]], q[path[j]]);
+    real re = e2 - dotacc(A[2], Q0[2], dotacc(A[1], Q0[1], dotf(A[0], Q0[0])));
+    real rf = FMA((real)3, detE, -dotacc(Kp[2], Q0[2], dotacc(Kp[1], Q0[1], dotf(Kp[0], Q0[0]))));
     /* e_i . f_i is det E at the path's ends and 2 det E inside it (E_a . K_a =
      * det E, E_a . K_b = 0 for a != b), so Gef = det E * sum of the weights
      * by path position (1, 2, 2, 1). */

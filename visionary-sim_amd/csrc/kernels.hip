@@ -81,9 +81,10 @@ __device__ __forceinline__ V3 crossf(V3 a, V3 b)
 // K_j the column of step j's axis. Corner sums become edge sums: with W_j the
 // weight at path position j,
 //     Ge = (W0+W1)|A0|^2 + (W1+W2)|A1|^2 + (W2+W3)|A2|^2 - 2(W1 A0.A1 + W2 A1.A2)
-// (Gf the same over K), and the rates Re = sum_j A_j.D_j, Rf = sum_j K_j.D_j
-// over each edge's change since the substep's start D_j = A_j - (q(P_j+1) -
-// q(P_j)). With c = s/|E|, the scaled multipliers u = c lambda_D, v = s^3
+// (Gf the same over K), and the rates Re = sum_j A_j.(A_j - Q_j) = |E|^2 -
+// sum_j A_j.Q_j, Rf = sum_j K_j.(A_j - Q_j) = 3 det E - sum_j K_j.Q_j over
+// the edges at the substep's start Q_j = q(P_j+1) - q(P_j) (A_j.K_j = det E
+// for each j). With c = s/|E|, the scaled multipliers u = c lambda_D, v = s^3
 // lambda_H, the block's first row multiplied by 1/c^2 = |E|^2 h^2 and its
 // second by h^6, the block is symmetric:
 //     [Ge st + aD h^2 |E|^2,  Gef st        ] [u]   [-(|E|^2 + r Re)            ]
@@ -134,11 +135,11 @@ __device__ __forceinline__ void project_tet(V3 (&p)[8], const V3 (&q)[8], const 
     const float k01 = fmaf(d12, d02, -(d01 * n[2])), k12 = fmaf(d02, d01, -(d12 * n[0]));
     float gf = fmaf(W23, m2, fmaf(wmid, m1, W01 * m0));
     gf = fmaf(-2.0f, fmaf(W2, k12, W1 * k01), gf);
-    V3 D[3];
+    V3 Q0[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) D[j] = sub(A[j], sub(q[tet.path[j + 1]], q[tet.path[j]]));
-    const float re = dotacc(A[2], D[2], dotacc(A[1], D[1], dotf(A[0], D[0])));
-    const float rf = dotacc(Kp[2], D[2], dotacc(Kp[1], D[1], dotf(Kp[0], D[0])));
+    for (int j = 0; j < 3; j++) Q0[j] = sub(q[tet.path[j + 1]], q[tet.path[j]]);
+    const float re = e2 - dotacc(A[2], Q0[2], dotacc(A[1], Q0[1], dotf(A[0], Q0[0])));
+    const float rf = fmaf(3.0f, detE, -dotacc(Kp[2], Q0[2], dotacc(Kp[1], Q0[1], dotf(Kp[0], Q0[0]))));
     // e_i . f_i is det E at the path's ends and 2 det E inside it (E_a . K_a
     // = det E, E_a . K_b = 0 for a != b): Gef = det E * (W0 + 2 W1 + 2 W2 + W3).
     const float gef = detE * fmaf(2.0f, wmid, wend);
