@@ -183,8 +183,11 @@ hipError_t launch_finalize(hipStream_t s, Shape sh, NodeView n, const vs_body *b
 // One substep's finalize and the next one's predict in one node pass.
 hipError_t launch_finalize_predict(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
                                   const vs_scene_params *scenes, TileView t);
+// `stretch` false: no body of the launch has maxStretch above 1, so the
+// code of the tensile limit is left out (the elastic launches below too).
 hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
-                          const vs_scene_params *scenes, const uint32_t *cells, TileView t, uint32_t limit_only);
+                          const vs_scene_params *scenes, const uint32_t *cells, TileView t, uint32_t limit_only,
+                          bool stretch = true);
 // Colours 2*cls and 2*cls + 1 over whole x-rows of dense lattices; a tile is
 // `count` rows of the class from row `begin` (row = a + ny*b for y = (cls&1)
 // + 2a, z = (cls>>1) + 2b, ny = the class's rows along y). `node_mode` 1
@@ -193,13 +196,14 @@ hipError_t launch_elastic(hipStream_t s, Shape sh, NodeView n, const vs_body *bo
 // Leftover list from entry `begin`.
 hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
                                const vs_scene_params *scenes, TileView t, uint32_t cls, uint32_t limit_only,
-                               uint32_t node_mode);
+                               uint32_t node_mode, bool stretch);
 // The eight colours of whole small dense bodies (at most 11^3 nodes), one
 // workgroup a body, in LDS; a tile is one body ({0, 0, body, scene}).
 // `node_mode` (scenes flagged VS_SCENE_FOLD_PREDICT): 3 predicts the nodes
 // first, 1 finalizes them (the previous substep) and predicts them first.
 hipError_t launch_elastic_bodies(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
-                                 const vs_scene_params *scenes, TileView t, uint32_t limit_only, uint32_t node_mode);
+                                 const vs_scene_params *scenes, TileView t, uint32_t limit_only, uint32_t node_mode,
+                                 bool stretch);
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes,
                               SurfaceView sv, GridView g, TileView t);
 hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv,

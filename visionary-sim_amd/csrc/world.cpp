@@ -644,19 +644,22 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
         sh = shape(P_DRIVE);
         if (sh.grid) HIPW(w, launch_drive(st, sh, n, w->d_scenes, drive_view(w), tile_view(w, P_DRIVE, scene), -1));
         return VS_OK;
-    case VS_P_ELASTIC:
+    case VS_P_ELASTIC: {
         sh = shape(elastic_table(color));
         if (!sh.grid) return VS_OK;
+        // The tensile limit's code only where a body of the launch has one.
+        const bool stretch = scene >= 0 ? w->recs[scene].stretch : w->any_stretch;
         if (color == 12)
             HIPW(w, launch_elastic_bodies(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_BODY, scene), limit_only,
-                                          node_mode));
+                                          node_mode, stretch));
         else if (color >= 8)
             HIPW(w, launch_elastic_rows(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, elastic_table(color), scene),
-                                        (uint32_t)(color - 8), limit_only, node_mode));
+                                        (uint32_t)(color - 8), limit_only, node_mode, stretch));
         else
             HIPW(w, launch_elastic(st, sh, n, w->d_bodies, w->d_scenes, w->d_cells, tile_view(w, P_CELL + color, scene),
-                                   limit_only));
+                                   limit_only, stretch));
         return VS_OK;
+    }
     case VS_P_FINALIZE:
         // Every node, or, after folded colour pairs, the ones they leave.
         sh = shape(P_LAST);
