@@ -520,12 +520,15 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
 // of a folded body the class's rows leave out). It gets the node arithmetic
 // of `node_mode` alone (nothing when the mode is 0).
 // Four waves per SIMD: at most 128 VGPRs.
-template <int BLOCK, bool STRETCH>
+// NODES false: a launch with `node_mode` 0, whose node-pass code is left
+// out (the colour-pair kernel less than half the size).
+template <int BLOCK, bool STRETCH, bool NODES>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_elastic_rows(NodeView n, const vs_body *__restrict__ bodies,
                                                         const vs_scene_params *__restrict__ scenes, TileView tv,
-                                                        uint32_t cls, uint32_t limit_only, uint32_t node_mode,
+                                                        uint32_t cls, uint32_t limit_only, uint32_t mode,
                                                         uint32_t xcd)
 {
+    const uint32_t node_mode = NODES ? mode : 0u;
     // Per lane: start-of-substep position (12) and inverse mass (4) of its
     // node 2m in the four node rows, then that node's position (12).
     __shared__ float lds[28 * BLOCK];
@@ -1783,6 +1786,19 @@ __global__ __launch_bounds__(BLOCK) void k_splats(const float *__restrict__ px, 
         return hipGetLastError();                                                                                  \
     } while (0)
 
+// The same for a kernel with two more template arguments A and B.
+#define VSIM_DISPATCH_BLOCK3(KERNEL, A, B, SH, STREAM, ...)                                                          \
+    do {                                                                                                             \
+        switch ((SH).block) {                                                                                        \
+        case 64: hipLaunchKernelGGL((KERNEL<64, A, B>), dim3((SH).grid), dim3(64), 0, STREAM, __VA_ARGS__); break;    \
+        case 128: hipLaunchKernelGGL((KERNEL<128, A, B>), dim3((SH).grid), dim3(128), 0, STREAM, __VA_ARGS__); break; \
+        case 256: hipLaunchKernelGGL((KERNEL<256, A, B>), dim3((SH).grid), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+        case 512: hipLaunchKernelGGL((KERNEL<512, A, B>), dim3((SH).grid), dim3(512), 0, STREAM, __VA_ARGS__); break; \
+        default: return hipErrorInvalidConfiguration;                                                                \
+        }                                                                                                            \
+        return hipGetLastError();                                                                                    \
+    } while (0)
+
 hipError_t launch_predict(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, TileView t)
 {
     VSIM_DISPATCH_BLOCK(k_predict, sh, s, n, scenes, t);
@@ -1811,9 +1827,13 @@ hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_bod
                                const vs_scene_params *scenes, TileView t, uint32_t cls, uint32_t limit_only,
                                uint32_t node_mode, bool stretch)
 {
+    if (stretch && node_mode)
+        VSIM_DISPATCH_BLOCK3(k_elastic_rows, true, true, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
     if (stretch)
-        VSIM_DISPATCH_BLOCK2(k_elastic_rows, true, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
-    VSIM_DISPATCH_BLOCK2(k_elastic_rows, false, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
+        VSIM_DISPATCH_BLOCK3(k_elastic_rows, true, false, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
+    if (node_mode)
+        VSIM_DISPATCH_BLOCK3(k_elastic_rows, false, true, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
+    VSIM_DISPATCH_BLOCK3(k_elastic_rows, false, false, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
 }
 
 hipError_t launch_elastic_bodies(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
