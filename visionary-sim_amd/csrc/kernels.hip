@@ -520,15 +520,16 @@ __global__ __launch_bounds__(BLOCK) void k_elastic(NodeView n, const vs_body *__
 // of a folded body the class's rows leave out). It gets the node arithmetic
 // of `node_mode` alone (nothing when the mode is 0).
 // Four waves per SIMD: at most 128 VGPRs.
-// NODES false: a launch with `node_mode` 0, whose node-pass code is left
-// out (the colour-pair kernel less than half the size).
-template <int BLOCK, bool STRETCH, bool NODES>
+// MODES: the node-pass code the form holds: 0 none (launches with
+// `node_mode` 0; the colour-pair kernel less than half the size), 1 the
+// finalize and predict of modes 1 and 2, 3 the predict of mode 3.
+template <int BLOCK, bool STRETCH, int MODES>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_elastic_rows(NodeView n, const vs_body *__restrict__ bodies,
                                                         const vs_scene_params *__restrict__ scenes, TileView tv,
                                                         uint32_t cls, uint32_t limit_only, uint32_t mode,
                                                         uint32_t xcd)
 {
-    const uint32_t node_mode = NODES ? mode : 0u;
+    const uint32_t node_mode = MODES == 0 ? 0u : MODES == 3 ? 3u : mode;
     // Per lane: start-of-substep position (12) and inverse mass (4) of its
     // node 2m in the four node rows, then that node's position (12).
     __shared__ float lds[28 * BLOCK];
@@ -1447,13 +1448,16 @@ __device__ __forceinline__ uint32_t row_bucket(const vs_scene_params &sc, GridVi
     return bk;
 }
 
-// Six waves a SIMD (at most 80 VGPRs; the compiler spills two registers):
-// the pass is latency-bound, and at five it runs slower.
-template <int BLOCK>
+// Six waves a SIMD (at most 80 VGPRs): the pass is latency-bound, and at
+// five it runs slower. LOG false: the form for launches without a pair log,
+// whose logging code is left out (with it, the compiler spills two registers
+// to stay at six waves).
+template <int BLOCK, bool LOG>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_collide_rows(NodeView n, const vs_body *__restrict__ bodies,
                                                         const vs_scene_params *__restrict__ scenes, SurfaceView sv,
                                                         GridView g, TileView tv, PairLog log, uint32_t parts)
 {
+    if (!LOG) log.cap = 0u;
     constexpr uint32_t WAVES = BLOCK / 64;
     constexpr uint32_t PER_PART = WAVES * ROW_GROUPS;
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -1827,13 +1831,19 @@ hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_bod
                                const vs_scene_params *scenes, TileView t, uint32_t cls, uint32_t limit_only,
                                uint32_t node_mode, bool stretch)
 {
-    if (stretch && node_mode)
-        VSIM_DISPATCH_BLOCK3(k_elastic_rows, true, true, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
-    if (stretch)
-        VSIM_DISPATCH_BLOCK3(k_elastic_rows, true, false, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
-    if (node_mode)
-        VSIM_DISPATCH_BLOCK3(k_elastic_rows, false, true, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
-    VSIM_DISPATCH_BLOCK3(k_elastic_rows, false, false, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd);
+#define VSIM_ROWS(S, M) \
+    VSIM_DISPATCH_BLOCK3(k_elastic_rows, S, M, sh, s, n, bodies, scenes, t, cls, limit_only, node_mode, sh.xcd)
+    if (node_mode == 0u) {
+        if (stretch) VSIM_ROWS(true, 0);
+        VSIM_ROWS(false, 0);
+    }
+    if (node_mode == 3u) {
+        if (stretch) VSIM_ROWS(true, 3);
+        VSIM_ROWS(false, 3);
+    }
+    if (stretch) VSIM_ROWS(true, 1);
+    VSIM_ROWS(false, 1);
+#undef VSIM_ROWS
 }
 
 hipError_t launch_elastic_bodies(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
@@ -1862,13 +1872,13 @@ hipError_t launch_contact_collide(hipStream_t s, Shape sh, NodeView n, const vs_
 {
     if (!sv.pay || !g.summary) VSIM_DISPATCH_BLOCK(k_collide, sh, s, n, bodies, scenes, sv, g, t, log);
     // Row lanes: each tile is `parts` work items of WAVES x 7 records.
-    switch (sh.block) {
-    case 64: sh.grid *= collide_parts<64>(tile_max); VSIM_DISPATCH_BLOCK(k_collide_rows, sh, s, n, bodies, scenes, sv, g, t, log, collide_parts<64>(tile_max));
-    case 128: sh.grid *= collide_parts<128>(tile_max); VSIM_DISPATCH_BLOCK(k_collide_rows, sh, s, n, bodies, scenes, sv, g, t, log, collide_parts<128>(tile_max));
-    case 256: sh.grid *= collide_parts<256>(tile_max); VSIM_DISPATCH_BLOCK(k_collide_rows, sh, s, n, bodies, scenes, sv, g, t, log, collide_parts<256>(tile_max));
-    case 512: sh.grid *= collide_parts<512>(tile_max); VSIM_DISPATCH_BLOCK(k_collide_rows, sh, s, n, bodies, scenes, sv, g, t, log, collide_parts<512>(tile_max));
-    default: return hipErrorInvalidConfiguration;
-    }
+    const uint32_t parts = sh.block == 64    ? collide_parts<64>(tile_max)
+                           : sh.block == 128 ? collide_parts<128>(tile_max)
+                           : sh.block == 256 ? collide_parts<256>(tile_max)
+                                             : collide_parts<512>(tile_max);
+    sh.grid *= parts;
+    if (log.cap) VSIM_DISPATCH_BLOCK2(k_collide_rows, true, sh, s, n, bodies, scenes, sv, g, t, log, parts);
+    VSIM_DISPATCH_BLOCK2(k_collide_rows, false, sh, s, n, bodies, scenes, sv, g, t, log, parts);
 }
 
 hipError_t launch_contact_settle(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,
