@@ -1,8 +1,8 @@
 """The eight colours of a small dense body in one launch, the body in LDS
 (k_elastic_bodies), against the same bodies through the colour-pair launches
 (`colour_launches=1`) bit for bit, and against the f32 oracle: contact
-bodies of 8^3 cells (64-lane workgroups), a dragged 10^3 block (11^3 nodes,
-the largest such body, 125 cells a colour: 128-lane workgroups), an 11^3
+bodies of 8^3 cells, a dragged 10^3 block (11^3 nodes, the largest such
+body, 125 cells a colour), an 11^3
 block beside it that is too large and keeps the colour pairs, tensile-limit
 sweeps, and a folded scene (no contact, no drive) whose bodies keep the
 colour pairs, all in one world; batched from a graph and per scene."""

@@ -205,7 +205,7 @@ struct vs_world {
     uint32_t block_node = 256, block_cells = 256, block_surf = 256, block_drive = 64, block_splat = 256;
     uint32_t block_rest = 256;   // the node pass of P_REST: 64 when its runs are short (folded scenes)
     uint32_t block_rest0 = 256;  // the frame's first predict over P_REST0, likewise
-    uint32_t block_body = 64;    // P_BODY: 128 when a body has more than 64 cells of a colour
+    uint32_t block_body = 256;   // P_BODY
     uint32_t block_last = 256;   // the frame's last finalize over P_LAST, as block_rest
     uint32_t tile_nodes = 256, tile_cells = 256, tile_surf = 256, tile_drive = 64, tile_splat = 256;
 };
@@ -492,14 +492,12 @@ int rebuild_tables(vs_world *w)
         }
         w->tile_count[p] = (uint32_t)n;
         if (p == P_BODY) {
-            // Workgroups of 64 lanes when no listed body has more cells of a
-            // colour than that (8^3 cells and less), else 128.
-            uint32_t most = 0;
-            for (uint64_t i = 0; i < n; i++) {
-                const uint32_t *cd = w->h_bodies[tiles[i].body].cell_dims;
-                most = std::max(most, ((cd[0] + 1u) / 2u) * ((cd[1] + 1u) / 2u) * ((cd[2] + 1u) / 2u));
-            }
-            w->block_body = most > 64u ? 128u : 64u;
+            // Workgroups of 256 lanes: a body's colour has at most 125 cells,
+            // but its loads of the nodes into LDS, node passes and stores
+            // run four times as wide as with 64 (the 512-body workload's
+            // sweep: 37.2 us a substep with 64 lanes, 32.7 with 128, 30.8
+            // with 256).
+            w->block_body = 256u;
         }
         if (p == P_REST || p == P_REST0 || p == P_LAST) {
             // Tables of small bodies (tiles of under 128 nodes on average)
