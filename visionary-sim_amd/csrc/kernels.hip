@@ -985,11 +985,18 @@ __device__ __forceinline__ void overflow_scan(const vs_scene_params &sc, const u
 }
 
 // The wave-wide minimum of v.
+// The wave's minimum, in every lane: within each row of 16 lanes by DPP
+// (quad swaps, then rotations by 4 and 8), then the four rows' minima read
+// from their first lanes.
 __device__ __forceinline__ uint32_t wave_min(uint32_t v)
 {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d));
-    return v;
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xb1, 0xf, 0xf, false));   // quad_perm 1,0,3,2
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4e, 0xf, 0xf, false));   // quad_perm 2,3,0,1
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xf, 0xf, false));  // row_ror 4
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xf, 0xf, false));  // row_ror 8
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    return min(min(a, b), min(c, d));
 }
 
 // An overfull bucket (grid bucket `gbk`, `held` records) from its members:
@@ -1108,6 +1115,9 @@ __device__ __forceinline__ void sort_slots(uint32_t *base, uint32_t held)
         if ((uint32_t)k < held) base[k] = v[k];
 }
 
+// Workgroups a sort launch adds past its tiles for the overfull buckets.
+constexpr uint32_t SORT_EXTRA_GROUPS = 32;
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restrict__ scenes, SurfaceView sv, GridView g,
                                                 TileView tv)
@@ -1179,12 +1189,15 @@ __global__ __launch_bounds__(BLOCK) void k_sort(const vs_scene_params *__restric
         }
     }
     if (!queued) return;
-    // Wave w of the grid takes listed buckets w, w + W, ... (W waves in
-    // all): one each while there are fewer than waves. Every lane of the
-    // wave is here (the tile loop is uniform over the workgroup).
+    // Wave w of the grid, counted from its end, takes listed buckets w, w +
+    // W, ... (W waves in all): one each while there are fewer than waves.
+    // The launch adds SORT_EXTRA_GROUPS workgroups past the tiles, which
+    // come here at once and take the first of them beside the tile work.
+    // Every lane of the wave is here (the tile loop is uniform over the
+    // workgroup).
     const uint32_t listed = min(g.ovf_count[1], g.ovb_cap);
     const uint32_t W = gridDim.x * (BLOCK / 64u);
-    for (uint32_t j = blockIdx.x * (BLOCK / 64u) + (threadIdx.x >> 6); j < listed; j += W) {
+    for (uint32_t j = (gridDim.x - 1u - blockIdx.x) * (BLOCK / 64u) + (threadIdx.x >> 6); j < listed; j += W) {
         const uint2 e = g.ovb[j];
         const vs_scene_params &so = scenes[e.y];
         const uint32_t sl = so.bucket_slots;
@@ -1863,6 +1876,7 @@ hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scen
 hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv, GridView g,
                                TileView t)
 {
+    if (g.ovb) sh.grid += SORT_EXTRA_GROUPS;
     VSIM_DISPATCH_BLOCK(k_sort, sh, s, scenes, sv, g, t);
 }
 
