@@ -198,12 +198,16 @@ hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_bod
                                const vs_scene_params *scenes, TileView t, uint32_t cls, uint32_t limit_only,
                                uint32_t node_mode, bool stretch);
 // The eight colours of whole small dense bodies (at most 11^3 nodes), one
-// workgroup a body, in LDS; a tile is one body ({0, 0, body, scene}).
+// workgroup a body, in LDS.
 // `node_mode` (scenes flagged VS_SCENE_FOLD_PREDICT): 3 predicts the nodes
 // first, 1 finalizes them (the previous substep) and predicts them first.
+// A tile is {first contact record, contact records, body, scene}; with
+// `settle`, the body's records get the previous substep's settle (k_settle's
+// arithmetic and counter clearing) after the load and before the node
+// passes, in place of a settle launch.
 hipError_t launch_elastic_bodies(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
                                  const vs_scene_params *scenes, TileView t, uint32_t limit_only, uint32_t node_mode,
-                                 bool stretch);
+                                 bool stretch, SurfaceView sv, GridView g, uint32_t settle);
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes,
                               SurfaceView sv, GridView g, TileView t);
 hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv,

@@ -806,7 +806,8 @@ constexpr uint32_t BODY_SWEEP_NODES = 11 * 11 * 11;  // bodies of up to 10^3 cel
 template <int BLOCK, bool STRETCH>
 __global__ __launch_bounds__(BLOCK) void k_elastic_bodies(NodeView n, const vs_body *__restrict__ bodies,
                                                           const vs_scene_params *__restrict__ scenes, TileView tv,
-                                                          uint32_t limit_only, uint32_t node_mode)
+                                                          uint32_t limit_only, uint32_t node_mode, SurfaceView sv,
+                                                          GridView g, uint32_t settle)
 {
     __shared__ float sp[3][BODY_SWEEP_NODES], sq[3][BODY_SWEEP_NODES], sw[BODY_SWEEP_NODES];
     const uint32_t tid = threadIdx.x;
@@ -823,22 +824,77 @@ __global__ __launch_bounds__(BLOCK) void k_elastic_bodies(NodeView n, const vs_b
         const uint32_t mode = !limit_only && (sc.flags & VS_SCENE_FOLD_PREDICT) ? node_mode : 0u;
         const Tank tk = tank_of(sc, bd);
         const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
-        for (uint32_t i = tid; i < N; i += BLOCK) {
-            V3 p = ld3(n.px, n.py, n.pz, o + i), q = ld3(n.qx, n.qy, n.qz, o + i);
-            const float w = n.w[o + i];
-            if (mode != 0u && w != 0.0f) {
-                V3 v;
-                if (mode == 3u) v = ld3(n.vx, n.vy, n.vz, o + i);
-                else finalize_node(p, q, v, tk);
-                predict_node(p, q, v, gx, gy, gz, dt);
+        if (!settle) {
+            for (uint32_t i = tid; i < N; i += BLOCK) {
+                V3 p = ld3(n.px, n.py, n.pz, o + i), q = ld3(n.qx, n.qy, n.qz, o + i);
+                const float w = n.w[o + i];
+                if (mode != 0u && w != 0.0f) {
+                    V3 v;
+                    if (mode == 3u) v = ld3(n.vx, n.vy, n.vz, o + i);
+                    else finalize_node(p, q, v, tk);
+                    predict_node(p, q, v, gx, gy, gz, dt);
+                }
+                sp[0][i] = p.x;
+                sp[1][i] = p.y;
+                sp[2][i] = p.z;
+                sq[0][i] = q.x;
+                sq[1][i] = q.y;
+                sq[2][i] = q.z;
+                sw[i] = w;
             }
-            sp[0][i] = p.x;
-            sp[1][i] = p.y;
-            sp[2][i] = p.z;
-            sq[0][i] = q.x;
-            sq[1][i] = q.y;
-            sq[2][i] = q.z;
-            sw[i] = w;
+        } else {
+            // The previous substep's settle (k_settle's arithmetic) on the
+            // body's contact records tile.begin .. + tile.count, between the
+            // load and the node passes: each record's bucket counter and
+            // summary cleared (arrival rank 0), each record with hits moved
+            // by its corrections' sum times gain / hits.
+            for (uint32_t i = tid; i < N; i += BLOCK) {
+                sp[0][i] = n.px[o + i];
+                sp[1][i] = n.py[o + i];
+                sp[2][i] = n.pz[o + i];
+                sq[0][i] = n.qx[o + i];
+                sq[1][i] = n.qy[o + i];
+                sq[2][i] = n.qz[o + i];
+                sw[i] = n.w[o + i];
+            }
+            if (g.ovf_count && blockIdx.x == 0 && tid == 0) {
+                g.ovf_count[0] = 0u;
+                if (g.ovb) g.ovf_count[1] = g.ovf_count[2] = 0u;
+            }
+            __syncthreads();
+            const float gain = sc.gain;
+            const uint32_t gb = sc.grid_base;
+            for (uint32_t i = tid; i < tile.count; i += BLOCK) {
+                const uint32_t s = tile.begin + i;
+                if (sv.rank[s] == 0u) {
+                    g.count[gb + sv.key[s]] = 0u;
+                    if (g.summary) g.summary[gb + sv.key[s]] = 0u;
+                }
+                const float h = sv.hits[s];
+                if (h > 0.0f) {
+                    const uint32_t k = sv.node[s] - o;
+                    const float f = gain / h;
+                    sp[0][k] = sp[0][k] + sv.dx[s] * f;
+                    sp[1][k] = sp[1][k] + sv.dy[s] * f;
+                    sp[2][k] = sp[2][k] + sv.dz[s] * f;
+                }
+            }
+            __syncthreads();
+            if (mode != 0u) {
+                for (uint32_t i = tid; i < N; i += BLOCK) {
+                    if (sw[i] == 0.0f) continue;
+                    V3 p = mk(sp[0][i], sp[1][i], sp[2][i]), q = mk(sq[0][i], sq[1][i], sq[2][i]), v;
+                    if (mode == 3u) v = ld3(n.vx, n.vy, n.vz, o + i);
+                    else finalize_node(p, q, v, tk);
+                    predict_node(p, q, v, gx, gy, gz, dt);
+                    sp[0][i] = p.x;
+                    sp[1][i] = p.y;
+                    sp[2][i] = p.z;
+                    sq[0][i] = q.x;
+                    sq[1][i] = q.y;
+                    sq[2][i] = q.z;
+                }
+            }
         }
         __syncthreads();
         const CellConsts kc = cell_consts(bd, sc.dt);
@@ -1861,10 +1917,11 @@ hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_bod
 
 hipError_t launch_elastic_bodies(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
                                  const vs_scene_params *scenes, TileView t, uint32_t limit_only, uint32_t node_mode,
-                                 bool stretch)
+                                 bool stretch, SurfaceView sv, GridView g, uint32_t settle)
 {
-    if (stretch) VSIM_DISPATCH_BLOCK2(k_elastic_bodies, true, sh, s, n, bodies, scenes, t, limit_only, node_mode);
-    VSIM_DISPATCH_BLOCK2(k_elastic_bodies, false, sh, s, n, bodies, scenes, t, limit_only, node_mode);
+    if (stretch)
+        VSIM_DISPATCH_BLOCK2(k_elastic_bodies, true, sh, s, n, bodies, scenes, t, limit_only, node_mode, sv, g, settle);
+    VSIM_DISPATCH_BLOCK2(k_elastic_bodies, false, sh, s, n, bodies, scenes, t, limit_only, node_mode, sv, g, settle);
 }
 
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,

@@ -193,6 +193,9 @@ struct vs_world {
 
     std::vector<SceneRec> recs;
     bool tables_dirty = true, params_dirty = true, any_stretch = false, any_contact = false;
+    // Every contact scene's bodies are body sweeps that run the node passes:
+    // each substep's settle rides on the next substep's sweep.
+    bool settle_fold = false;
     std::vector<std::vector<char>> pending;  // host copies kept alive until uploaded
 
     // Captured frame.
@@ -346,6 +349,17 @@ bool sweep_fold_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
     return sweep_body(w, r, b) && !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD_PREDICT);
 }
 
+// A contact scene whose bodies are all body sweeps that run the node passes
+// has each substep's settle applied by the next substep's sweep, after its
+// load and before the finalize (nothing runs between them).
+bool scene_settle_fold(const vs_world *w, const SceneRec &r)
+{
+    if (!r.live || !r.contact || r.bodies.empty()) return false;
+    for (const BodyRec &b : r.bodies)
+        if (!sweep_fold_body(w, r, b)) return false;
+    return true;
+}
+
 bool fold_predict_body(const vs_world *w, const SceneRec &r, const BodyRec &b)
 {
     return !w->cfg.separate_node_passes && (r.params.flags & VS_SCENE_FOLD_PREDICT) && row_body(w, b) &&
@@ -439,7 +453,15 @@ int rebuild_tables(vs_world *w)
                                            (uint32_t)(r.body_begin + b));
                     } else if (p == P_BODY) {
                         for (size_t b = 0; b < r.bodies.size() && ok; b++)
-                            if (sweep_body(w, r, r.bodies[b])) ok &= emit(0, 1, (uint32_t)(r.body_begin + b));
+                            if (sweep_body(w, r, r.bodies[b])) {
+                                // One tile a body, carrying its contact records.
+                                if (n >= w->tile_cap[p]) {
+                                    ok = false;
+                                    break;
+                                }
+                                tiles[n++] = vs_tile{ (uint32_t)r.bodies[b].surf_begin, (uint32_t)r.bodies[b].surf_count,
+                                                      (uint32_t)(r.body_begin + b), s };
+                            }
                     } else if (is_row_pass(p)) {
                         // Whole rows of the class, as many as fit one workgroup.
                         const uint32_t cls = (uint32_t)(p - P_ROW);
@@ -532,6 +554,16 @@ int rebuild_tables(vs_world *w)
             HIPW(w, hipMemcpyAsync(w->d_rows_by_work[p - P_ROW], h, 16 + n * sizeof(vs_tile), hipMemcpyHostToDevice,
                                    w->stream));
         }
+    }
+    {
+        bool any = false, all = true;
+        for (uint32_t s = 0; s < S; s++) {
+            const SceneRec &r = w->recs[s];
+            if (!r.live || !r.contact) continue;
+            any = true;
+            all = all && scene_settle_fold(w, r);
+        }
+        w->settle_fold = any && all;
     }
     HIPW(w, hipMemcpyAsync(w->d_scene_counts, w->h_scene_counts.p, sizeof(uint32_t) * NPASS * S,
                            hipMemcpyHostToDevice, w->stream));
@@ -649,7 +681,7 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
         const bool stretch = scene >= 0 ? w->recs[scene].stretch : w->any_stretch;
         if (color == 12)
             HIPW(w, launch_elastic_bodies(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_BODY, scene), limit_only,
-                                          node_mode, stretch));
+                                          node_mode & 3u, stretch, surf_view(w), grid_view(w), node_mode & 4u));
         else if (color >= 8)
             HIPW(w, launch_elastic_rows(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, elastic_table(color), scene),
                                         (uint32_t)(color - 8), limit_only, node_mode, stretch));
@@ -712,6 +744,10 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
         contact = w->recs[scene].contact;
     }
     const bool merge = !w->cfg.separate_node_passes;
+    const bool settle_fold = scene >= 0 ? scene_settle_fold(w, w->recs[scene]) : w->settle_fold;
+    // The launch that takes a substep's settle covers the contact scenes
+    // (with two chains, they are all chain 0's).
+    const bool settle_here = settle_fold && (scene >= 0 || w->cur_branch != 1);
     hipStream_t st = w->cur_stream ? w->cur_stream : w->stream;
     // With a profile, events bracket every launch of a non-empty pass.
     auto pass = [&](int kind, int color, uint32_t limit_only, uint32_t node_mode = 0) -> int {
@@ -752,7 +788,11 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
             // only); at a sweep's start the bodies that fold their node
             // passes there predict (a frame's first substep) or finalize
             // the previous substep and predict.
-            if (int rc = pass(VS_P_ELASTIC, 12, sweep > 0 ? 1u : 0u, merge && sweep == 0 ? (s == 0 ? 3u : 1u) : 0u))
+            // (With the settle folded, the previous substep's settle runs
+            // there too: node_mode bit 4.)
+            const uint32_t sweep_mode = merge && sweep == 0 ? (s == 0 ? 3u : 1u) : 0u;
+            if (int rc = pass(VS_P_ELASTIC, 12, sweep > 0 ? 1u : 0u,
+                              sweep_mode | (settle_here && s > 0 && sweep == 0 ? 4u : 0u)))
                 return rc;
             for (int c = 0; c < 4; c++)
                 for (int e : { 8 + c, 2 * c, 2 * c + 1 }) {
@@ -762,8 +802,10 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
                 }
         }
         if (contact)
-            for (int kind : { VS_P_BIN, VS_P_SORT, VS_P_COLLIDE, VS_P_SETTLE })
+            for (int kind : { VS_P_BIN, VS_P_SORT, VS_P_COLLIDE, VS_P_SETTLE }) {
+                if (kind == VS_P_SETTLE && settle_fold && !last) continue;  // the next sweep's
                 if (int rc = pass(kind, 0, 0)) return rc;
+            }
         if (int rc = pass(merge && !last ? VS_P_FINALIZE_PREDICT : VS_P_FINALIZE, 0, 0)) return rc;
     }
     if (splats)
@@ -1492,7 +1534,8 @@ int vs_world_frame(vs_world *w, uint32_t substeps, int32_t splats, int32_t graph
                                   w->cfg.separate_node_passes, w->pair_cap,
                                   w->block_cells,
                                   w->cfg.xcd_chunked, w->cfg.separate_colours, w->block_rest, w->block_rest0,
-                                  w->cfg.colour_launches, w->block_body, w->block_last, w->cfg.branches };
+                                  w->cfg.colour_launches, w->block_body, w->block_last, w->cfg.branches,
+                                  (uint32_t)w->settle_fold };
     for (int p = 0; p < NPASS; p++) {
         key.push_back(grid_for(w, p, -1));
         for (int b = 0; b < 2; b++) key.push_back(w->branch_ntiles[p][b] != 0);
