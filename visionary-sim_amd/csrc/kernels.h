@@ -201,13 +201,16 @@ hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_bod
 // workgroup a body, in LDS.
 // `node_mode` (scenes flagged VS_SCENE_FOLD_PREDICT): 3 predicts the nodes
 // first, 1 finalizes them (the previous substep) and predicts them first.
-// A tile is {first contact record, contact records, body, scene}; with
-// `settle`, the body's records get the previous substep's settle (k_settle's
-// arithmetic and counter clearing) after the load and before the node
-// passes, in place of a settle launch.
+// A tile is {first contact record, contact records, body, scene}. With
+// `contact` bit 1, the body's records get the previous substep's settle
+// (k_settle's arithmetic and counter clearing, grid `g`) after the load and
+// before the node passes, in place of a settle launch; with bit 2 they are
+// binned (k_bin's, grid `gn`) after the colours, in place of a bin launch.
+// With both, `g` and `gn` must not share bucket counters, summaries or list
+// counters.
 hipError_t launch_elastic_bodies(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
                                  const vs_scene_params *scenes, TileView t, uint32_t limit_only, uint32_t node_mode,
-                                 bool stretch, SurfaceView sv, GridView g, uint32_t settle);
+                                 bool stretch, SurfaceView sv, GridView g, GridView gn, uint32_t contact);
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes,
                               SurfaceView sv, GridView g, TileView t);
 hipError_t launch_contact_sort(hipStream_t s, Shape sh, const vs_scene_params *scenes, SurfaceView sv,

@@ -786,6 +786,60 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
+// gridCoord (contacts.wgsl:84-87)
+__device__ __forceinline__ void grid_coord(const vs_scene_params &sc, V3 p, uint32_t &x, uint32_t &y, uint32_t &z)
+{
+    const float c[3] = { p.x, p.y, p.z };
+    uint32_t o[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float last = (float)sc.grid_dims[a] - 1.0f;
+        float g = floorf((c[a] - sc.grid_origin[a]) / sc.bucket_size);
+        g = g < 0.0f ? 0.0f : g;
+        g = g > last ? last : g;
+        o[a] = (uint32_t)g;
+    }
+    x = o[0];
+    y = o[1];
+    z = o[2];
+}
+
+// Bin (bin.wgsl:17) in two parts, around the caller's atomicAdd on the
+// bucket's counter (the arrival rank): bin_key writes record s's bucket key
+// and the payloads beside it (its position and radius; its node's prev
+// position and inverse mass, `prev`, for collide) and returns the bucket;
+// bin_place stores the rank and puts the record in its slot, extension slot
+// or the overflow list.
+__device__ __forceinline__ uint32_t bin_key(const vs_scene_params &sc, SurfaceView sv, uint32_t s, V3 p, float4 prev)
+{
+    uint32_t x, y, z;
+    grid_coord(sc, p, x, y, z);
+    const uint32_t b = x + sc.grid_dims[0] * (y + sc.grid_dims[1] * z);
+    sv.key[s] = b;
+    if (sv.pay) sv.pay[s] = make_float4(p.x, p.y, p.z, sv.radius[s]);
+    if (sv.prev) sv.prev[s] = prev;
+    return b;
+}
+
+__device__ __forceinline__ void bin_place(const vs_scene_params &sc, uint32_t scene, SurfaceView sv, GridView g,
+                                          uint32_t s, uint32_t b, uint32_t slot)
+{
+    const uint32_t slots = sc.bucket_slots, gb = sc.grid_base;
+    sv.rank[s] = slot;
+    if (slot < slots) g.slots[(size_t)(gb + b) * slots + slot] = s;
+    else if (slot - slots < g.ext_slots) g.ext[(size_t)(gb + b) * g.ext_slots + (slot - slots)] = s;
+    else if (g.ovf) {
+        const uint32_t at = atomicAdd(g.ovf_count, 1u);
+        if (at < g.ovf_cap) g.ovf[at] = make_uint2(s, gb + b);
+    }
+    if (slot == slots && g.ovb) {
+        // This arrival made the bucket overfull: list the bucket once.
+        const uint32_t at = atomicAdd(&g.ovf_count[1], 1u);
+        if (at < g.ovb_cap) g.ovb[at] = make_uint2(gb + b, scene);
+        else atomicOr(&g.ovf_count[2], 1u);
+    }
+}
+
 // The colour passes of whole small dense bodies, one workgroup a body: its
 // nodes' positions, start-of-substep positions and inverse masses are
 // loaded into LDS once, colours 0..7 run from LDS in order (a barrier
@@ -801,13 +855,20 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 // (a frame's first substep) and `node_mode` 1 runs the previous substep's
 // finalize and then this one's predict (k_finalize_predict's arithmetic):
 // nothing runs between them but this pass's load.
+// `contact` bit 1 runs the previous substep's settle on the body's contact
+// records (tile.begin .. + tile.count) against grid `g`, between the load and
+// the node passes; bit 2 bins the records against grid `gn` after the
+// colours, from the positions in LDS (k_bin's arithmetic; nothing runs
+// between the sweep and the bin). With both, `g` and `gn` are different
+// grids (bucket counters, summaries and list counters by substep parity),
+// since one body's bin may run before another body's settle.
 constexpr uint32_t BODY_SWEEP_NODES = 11 * 11 * 11;  // bodies of up to 10^3 cells
 
 template <int BLOCK, bool STRETCH>
 __global__ __launch_bounds__(BLOCK) void k_elastic_bodies(NodeView n, const vs_body *__restrict__ bodies,
                                                           const vs_scene_params *__restrict__ scenes, TileView tv,
                                                           uint32_t limit_only, uint32_t node_mode, SurfaceView sv,
-                                                          GridView g, uint32_t settle)
+                                                          GridView g, GridView gn, uint32_t contact)
 {
     __shared__ float sp[3][BODY_SWEEP_NODES], sq[3][BODY_SWEEP_NODES], sw[BODY_SWEEP_NODES];
     const uint32_t tid = threadIdx.x;
@@ -824,7 +885,7 @@ __global__ __launch_bounds__(BLOCK) void k_elastic_bodies(NodeView n, const vs_b
         const uint32_t mode = !limit_only && (sc.flags & VS_SCENE_FOLD_PREDICT) ? node_mode : 0u;
         const Tank tk = tank_of(sc, bd);
         const float dt = sc.dt, gx = sc.gravity[0], gy = sc.gravity[1], gz = sc.gravity[2];
-        if (!settle) {
+        if (!(contact & 1u)) {
             for (uint32_t i = tid; i < N; i += BLOCK) {
                 V3 p = ld3(n.px, n.py, n.pz, o + i), q = ld3(n.qx, n.qy, n.qz, o + i);
                 const float w = n.w[o + i];
@@ -936,29 +997,18 @@ __global__ __launch_bounds__(BLOCK) void k_elastic_bodies(NodeView n, const vs_b
                 n.qz[o + i] = sq[2][i];
             }
         }
+        if (contact & 2u)
+            for (uint32_t i = tid; i < tile.count; i += BLOCK) {
+                const uint32_t s = tile.begin + i, k = sv.node[s] - o;
+                const uint32_t b = bin_key(sc, sv, s, mk(sp[0][k], sp[1][k], sp[2][k]),
+                                           make_float4(sq[0][k], sq[1][k], sq[2][k], sw[k]));
+                bin_place(sc, bd.scene, sv, gn, s, b, atomicAdd(&gn.count[sc.grid_base + b], 1u));
+            }
         __syncthreads();  // the next body reuses the LDS
     }
 }
 
 // -------------------------------------------------------------- contact
-// gridCoord (contacts.wgsl:84-87)
-__device__ __forceinline__ void grid_coord(const vs_scene_params &sc, V3 p, uint32_t &x, uint32_t &y, uint32_t &z)
-{
-    const float c[3] = { p.x, p.y, p.z };
-    uint32_t o[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const float last = (float)sc.grid_dims[a] - 1.0f;
-        float g = floorf((c[a] - sc.grid_origin[a]) / sc.bucket_size);
-        g = g < 0.0f ? 0.0f : g;
-        g = g > last ? last : g;
-        o[a] = (uint32_t)g;
-    }
-    x = o[0];
-    y = o[1];
-    z = o[2];
-}
-
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_bin(NodeView n, const vs_scene_params *__restrict__ scenes, SurfaceView sv,
                                                GridView g, TileView tv)
@@ -967,31 +1017,13 @@ __global__ __launch_bounds__(BLOCK) void k_bin(NodeView n, const vs_scene_params
     for (uint32_t t = r.first; t < r.end; t += r.stride) {
         const vs_tile tile = tv.tiles[t];
         const vs_scene_params &sc = scenes[tile.scene];
-        const uint32_t slots = sc.bucket_slots, dx = sc.grid_dims[0], dy = sc.grid_dims[1], gb = sc.grid_base;
         for (uint32_t i = threadIdx.x; i < tile.count; i += BLOCK) {
             const uint32_t s = tile.begin + i;
             const uint32_t node = sv.node[s];
-            uint32_t x, y, z;
-            const V3 p = ld3(n.px, n.py, n.pz, node);
-            grid_coord(sc, p, x, y, z);
-            const uint32_t b = x + dx * (y + dy * z);
-            sv.key[s] = b;
-            if (sv.pay) sv.pay[s] = make_float4(p.x, p.y, p.z, sv.radius[s]);
-            if (sv.prev) sv.prev[s] = make_float4(n.qx[node], n.qy[node], n.qz[node], n.w[node]);
-            const uint32_t slot = atomicAdd(&g.count[gb + b], 1u);
-            sv.rank[s] = slot;
-            if (slot < slots) g.slots[(size_t)(gb + b) * slots + slot] = s;
-            else if (slot - slots < g.ext_slots) g.ext[(size_t)(gb + b) * g.ext_slots + (slot - slots)] = s;
-            else if (g.ovf) {
-                const uint32_t at = atomicAdd(g.ovf_count, 1u);
-                if (at < g.ovf_cap) g.ovf[at] = make_uint2(s, gb + b);
-            }
-            if (slot == slots && g.ovb) {
-                // This arrival made the bucket overfull: list the bucket once.
-                const uint32_t at = atomicAdd(&g.ovf_count[1], 1u);
-                if (at < g.ovb_cap) g.ovb[at] = make_uint2(gb + b, tile.scene);
-                else atomicOr(&g.ovf_count[2], 1u);
-            }
+            const float4 prev =
+                sv.prev ? make_float4(n.qx[node], n.qy[node], n.qz[node], n.w[node]) : make_float4(0.f, 0.f, 0.f, 0.f);
+            const uint32_t b = bin_key(sc, sv, s, ld3(n.px, n.py, n.pz, node), prev);
+            bin_place(sc, tile.scene, sv, g, s, b, atomicAdd(&g.count[sc.grid_base + b], 1u));
         }
     }
 }
@@ -1917,11 +1949,13 @@ hipError_t launch_elastic_rows(hipStream_t s, Shape sh, NodeView n, const vs_bod
 
 hipError_t launch_elastic_bodies(hipStream_t s, Shape sh, NodeView n, const vs_body *bodies,
                                  const vs_scene_params *scenes, TileView t, uint32_t limit_only, uint32_t node_mode,
-                                 bool stretch, SurfaceView sv, GridView g, uint32_t settle)
+                                 bool stretch, SurfaceView sv, GridView g, GridView gn, uint32_t contact)
 {
     if (stretch)
-        VSIM_DISPATCH_BLOCK2(k_elastic_bodies, true, sh, s, n, bodies, scenes, t, limit_only, node_mode, sv, g, settle);
-    VSIM_DISPATCH_BLOCK2(k_elastic_bodies, false, sh, s, n, bodies, scenes, t, limit_only, node_mode, sv, g, settle);
+        VSIM_DISPATCH_BLOCK2(k_elastic_bodies, true, sh, s, n, bodies, scenes, t, limit_only, node_mode, sv, g, gn,
+                             contact);
+    VSIM_DISPATCH_BLOCK2(k_elastic_bodies, false, sh, s, n, bodies, scenes, t, limit_only, node_mode, sv, g, gn,
+                         contact);
 }
 
 hipError_t launch_contact_bin(hipStream_t s, Shape sh, NodeView n, const vs_scene_params *scenes, SurfaceView sv,

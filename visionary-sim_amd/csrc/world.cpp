@@ -161,10 +161,16 @@ struct vs_world {
     float4 *d_surf_pay = nullptr;  // per record: node position and radius, then prev position and inverse mass, at bin time
     uint32_t *d_grid = nullptr;
     uint2 *d_ovf = nullptr;            // records that found their bucket full: (record, grid bucket)
-    uint32_t *d_ovf_count = nullptr;   // 4 words: records listed, buckets listed, list full (settle empties them), spare
+    uint32_t *d_ovf_count = nullptr;   // 4 words: records listed, buckets listed, list full (settle empties them), spare;
+                                       // 4 more for the parity-1 grid
     uint2 *d_ovb = nullptr;            // overfull buckets this substep: (grid bucket, scene)
     uint32_t *d_ext = nullptr;         // EXT_SLOTS more slots a bucket (arrivals 32 .. 32 + EXT_SLOTS - 1)
     uint32_t *d_summary = nullptr;     // per bucket: slots held and sole body, for collide (GridView::summary)
+    // The parity-1 grid's bucket counters, then its summaries (2 x max_buckets):
+    // with the bin folded into the body sweep, substep s bins into grid s & 1
+    // while the same launch settles substep s - 1 in the other; slots,
+    // extension slots and list entries are shared.
+    uint32_t *d_parity1 = nullptr;
     uint32_t *d_drive = nullptr;
     float *d_splat_in = nullptr, *d_splat_out = nullptr;
     int32_t *d_pairlog = nullptr;
@@ -196,6 +202,7 @@ struct vs_world {
     // Every contact scene's bodies are body sweeps that run the node passes:
     // each substep's settle rides on the next substep's sweep.
     bool settle_fold = false;
+    uint32_t grid_parity = 0;  // the grid the contact launches being enqueued use
     std::vector<std::vector<char>> pending;  // host copies kept alive until uploaded
 
     // Captured frame.
@@ -265,11 +272,22 @@ SurfaceView surf_view(vs_world *w)
 // records can make buckets overfull.
 uint32_t ovb_cap(const vs_world *w) { return (uint32_t)(w->cfg.max_surface / 2u + 1u); }
 
-GridView grid_view(vs_world *w)
+GridView grid_view(vs_world *w, uint32_t parity)
 {
-    return GridView{ w->d_grid, w->d_grid + w->cfg.max_buckets, w->d_ovf, w->d_ovf_count,
-                     (uint32_t)w->cfg.max_surface, w->d_ovb, ovb_cap(w), w->d_ext, EXT_SLOTS, w->d_summary };
+    const size_t B = w->cfg.max_buckets;
+    return GridView{ parity ? w->d_parity1 : w->d_grid,
+                     w->d_grid + B,
+                     w->d_ovf,
+                     w->d_ovf_count + (parity ? 4 : 0),
+                     (uint32_t)w->cfg.max_surface,
+                     w->d_ovb,
+                     ovb_cap(w),
+                     w->d_ext,
+                     EXT_SLOTS,
+                     parity ? w->d_parity1 + B : w->d_summary };
 }
+
+GridView grid_view(vs_world *w) { return grid_view(w, w->grid_parity); }
 
 vsim::DriveView drive_view(vs_world *w)
 {
@@ -681,7 +699,9 @@ int enqueue_pass(vs_world *w, int kind, int color, int scene, uint32_t limit_onl
         const bool stretch = scene >= 0 ? w->recs[scene].stretch : w->any_stretch;
         if (color == 12)
             HIPW(w, launch_elastic_bodies(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, P_BODY, scene), limit_only,
-                                          node_mode & 3u, stretch, surf_view(w), grid_view(w), node_mode & 4u));
+                                          node_mode & 3u, stretch, surf_view(w),
+                                          grid_view(w, node_mode & 8u ? w->grid_parity ^ 1u : w->grid_parity),
+                                          grid_view(w, w->grid_parity), (node_mode >> 2) & 3u));
         else if (color >= 8)
             HIPW(w, launch_elastic_rows(st, sh, n, w->d_bodies, w->d_scenes, tile_view(w, elastic_table(color), scene),
                                         (uint32_t)(color - 8), limit_only, node_mode, stretch));
@@ -748,6 +768,11 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
     // The launch that takes a substep's settle covers the contact scenes
     // (with two chains, they are all chain 0's).
     const bool settle_here = settle_fold && (scene >= 0 || w->cur_branch != 1);
+    // Without the tensile sweeps the same launch also bins the substep's
+    // records (after its colours): substep s bins into grid s & 1 while the
+    // launch settles substep s - 1 in the other, and sort, collide and
+    // settle of substep s use grid s & 1.
+    const bool bin_fold = settle_fold && !stretch;
     hipStream_t st = w->cur_stream ? w->cur_stream : w->stream;
     // With a profile, events bracket every launch of a non-empty pass.
     auto pass = [&](int kind, int color, uint32_t limit_only, uint32_t node_mode = 0) -> int {
@@ -778,6 +803,7 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
     // predicts the nodes it holds and the predict pass takes the rest.
     for (uint32_t s = 0; s < substeps; s++) {
         const bool last = s + 1 == substeps;
+        w->grid_parity = bin_fold ? s & 1u : 0u;
         if (s == 0 || !merge)
             if (int rc = pass(VS_P_PREDICT, merge ? 1 : 0, 0)) return rc;
         if (int rc = pass(VS_P_DRIVE, 0, 0)) return rc;
@@ -789,10 +815,12 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
             // passes there predict (a frame's first substep) or finalize
             // the previous substep and predict.
             // (With the settle folded, the previous substep's settle runs
-            // there too: node_mode bit 4.)
+            // there too: node_mode bit 4; with the bin folded, this
+            // substep's bin after the colours: bit 8.)
             const uint32_t sweep_mode = merge && sweep == 0 ? (s == 0 ? 3u : 1u) : 0u;
             if (int rc = pass(VS_P_ELASTIC, 12, sweep > 0 ? 1u : 0u,
-                              sweep_mode | (settle_here && s > 0 && sweep == 0 ? 4u : 0u)))
+                              sweep_mode | (settle_here && s > 0 && sweep == 0 ? 4u : 0u) |
+                                  (settle_here && bin_fold ? 8u : 0u)))
                 return rc;
             for (int c = 0; c < 4; c++)
                 for (int e : { 8 + c, 2 * c, 2 * c + 1 }) {
@@ -804,10 +832,12 @@ int enqueue_frame(vs_world *w, uint32_t substeps, uint32_t splats, int scene, Pr
         if (contact)
             for (int kind : { VS_P_BIN, VS_P_SORT, VS_P_COLLIDE, VS_P_SETTLE }) {
                 if (kind == VS_P_SETTLE && settle_fold && !last) continue;  // the next sweep's
+                if (kind == VS_P_BIN && bin_fold) continue;                 // the sweep's
                 if (int rc = pass(kind, 0, 0)) return rc;
             }
         if (int rc = pass(merge && !last ? VS_P_FINALIZE_PREDICT : VS_P_FINALIZE, 0, 0)) return rc;
     }
+    w->grid_parity = 0;
     if (splats)
         if (int rc = pass(VS_P_SPLAT, 0, 0)) return rc;
     return VS_OK;
@@ -936,10 +966,11 @@ static int world_init(vs_world *w)
     ALLOC(W->d_surf_pay, 2 * sizeof(float4) * c.max_surface);
     ALLOC(W->d_grid, sizeof(uint32_t) * (1 + BUCKET_SLOTS) * c.max_buckets);
     ALLOC(W->d_ovf, sizeof(uint2) * c.max_surface);
-    ALLOC(W->d_ovf_count, 4 * sizeof(uint32_t));
+    ALLOC(W->d_ovf_count, 8 * sizeof(uint32_t));
     ALLOC(W->d_ovb, sizeof(uint2) * ovb_cap(W));
     ALLOC(W->d_ext, sizeof(uint32_t) * EXT_SLOTS * c.max_buckets);
     ALLOC(W->d_summary, sizeof(uint32_t) * c.max_buckets);
+    ALLOC(W->d_parity1, sizeof(uint32_t) * 2 * c.max_buckets);
     ALLOC(W->d_drive, sizeof(uint32_t) * DRIVE_PLANES * c.max_driven);
     ALLOC(W->d_splat_in, sizeof(float) * 10 * c.max_splats);
     ALLOC(W->d_splat_out, sizeof(float) * SPLAT_OUT_PLANES * c.max_splats);
@@ -948,7 +979,8 @@ static int world_init(vs_world *w)
     HIPW(W, hipMemset(W->d_nodes, 0, sizeof(float) * NODE_PLANES * c.max_nodes));
     HIPW(W, hipMemset(W->d_grid, 0, sizeof(uint32_t) * c.max_buckets));
     HIPW(W, hipMemset(W->d_summary, 0, sizeof(uint32_t) * c.max_buckets));
-    HIPW(W, hipMemset(W->d_ovf_count, 0, 4 * sizeof(uint32_t)));
+    HIPW(W, hipMemset(W->d_parity1, 0, sizeof(uint32_t) * 2 * c.max_buckets));
+    HIPW(W, hipMemset(W->d_ovf_count, 0, 8 * sizeof(uint32_t)));
     HIPW(W, hipMemset(W->d_scenes, 0, sizeof(vs_scene_params) * c.max_scenes));
     W->nodes.init(c.max_nodes);
     W->cells.init(c.max_cells);
@@ -1039,7 +1071,7 @@ void vs_world_destroy(vs_world *w)
                            w->join })
         if (ev) (void)hipEventDestroy(ev);
     void *ptrs[] = { w->d_nodes, w->d_cells, w->d_bodies, w->d_scenes, w->d_surf, w->d_surf_pay,
-                     w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_ovb, w->d_ext, w->d_summary, w->d_drive, w->d_splat_in, w->d_splat_out,
+                     w->d_grid,  w->d_ovf, w->d_ovf_count, w->d_ovb, w->d_ext, w->d_summary, w->d_parity1, w->d_drive, w->d_splat_in, w->d_splat_out,
                      w->d_pairlog, w->d_scene_counts, w->d_branch_counts };
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1427,6 +1459,8 @@ int vs_world_add_scene(vs_world *w, const vs_scene_desc *d, uint32_t *scene_id)
     HIPW(w, hipMemcpyAsync(w->d_bodies + r.body_begin, hb_rec, sizeof(vs_body) * d->n_bodies, hipMemcpyHostToDevice, up));
     if (nbuckets) HIPW(w, hipMemsetAsync(w->d_grid + r.bucket_begin, 0, nbuckets * 4, up));
     if (nbuckets) HIPW(w, hipMemsetAsync(w->d_summary + r.bucket_begin, 0, nbuckets * 4, up));
+    if (nbuckets) HIPW(w, hipMemsetAsync(w->d_parity1 + r.bucket_begin, 0, nbuckets * 4, up));
+    if (nbuckets) HIPW(w, hipMemsetAsync(w->d_parity1 + w->cfg.max_buckets + r.bucket_begin, 0, nbuckets * 4, up));
     if (d->n_driven) {
         const size_t D = w->cfg.max_driven;
         std::vector<uint32_t> dn(d->n_driven);
@@ -1741,7 +1775,7 @@ int vs_world_stats_get(const vs_world *w, vs_world_stats *out)
     const vs_world_config &c = w->cfg;
     out->bytes_reserved = 4ull * NODE_PLANES * c.max_nodes + 4ull * c.max_cells + sizeof(vs_body) * c.max_bodies +
                           sizeof(vs_scene_params) * c.max_scenes + (4ull * SURF_PLANES + 32) * c.max_surface +
-                          4ull * (2 + BUCKET_SLOTS + EXT_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
+                          4ull * (4 + BUCKET_SLOTS + EXT_SLOTS) * c.max_buckets + 4ull * DRIVE_PLANES * c.max_driven +
                           4ull * (10 + SPLAT_OUT_PLANES) * c.max_splats;
     out->graph_captures = w->captures;
     return VS_OK;
